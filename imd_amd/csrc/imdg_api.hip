@@ -1,0 +1,1117 @@
+// imdg_api.hip -- host side of the C ABI declared in include/imdg.h:
+// context, device memory, table re-layout, replica geometry (make_box /
+// init_cells semantics), launch sequencing and result download.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/imdg.h"
+#include "imdg_internal.h"
+
+namespace imdg {
+template <typename R>
+void launch_density(const StepArgs<R>&, const TabDev<R>*, bool, int,
+                    hipStream_t);
+template <typename R>
+void launch_force(const StepArgs<R>&, const TabDev<R>*, bool, bool, int,
+                  hipStream_t);
+template <typename R>
+void launch_move(const StepArgs<R>&, int, hipStream_t);
+void launch_build(const BuildArgs&, int, hipStream_t);
+void launch_check(const RepDev*, RepState*, Ctl*, const int*, double4* const*,
+                  const double4*, double*, int, int, hipStream_t);
+}  // namespace imdg
+
+using namespace imdg;
+
+namespace {
+
+enum { KC_BUILD = 0, KC_DENSITY, KC_FORCE, KC_MOVE, KC_N };
+const char* kKernelNames[KC_N] = {"rebuild", "density", "force", "move"};
+
+struct HostTable {
+  int ncols = 0;
+  std::vector<double> begin, end, step, invstep;
+  std::vector<int> len;
+  std::vector<double> table;  // (maxsteps + 2) * ncols
+  int maxsteps = 0;
+  bool present = false;
+};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct imdg_ctx {
+  imdg_config cfg;
+  std::string err;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  bool mixed = false;
+  int xcd_remap = 1;
+  // tables
+  HostTable tab[3];  // pair, rho, embed
+  bool tables_set = false;
+  double cellsz_tab = 0.0;
+  // geometry
+  int nrep = 0;
+  std::vector<RepDev> rep;
+  std::vector<int> natoms;      // per replica
+  std::vector<long> in_off;     // input index of the first atom of a replica
+  long natoms_tot = 0;
+  int nslots = 0, nblk = 0, ncells = 0;
+  int kmax = 0;
+  // device memory
+  std::vector<DevBuf> bufs;
+  RepDev* d_rep = nullptr;
+  RepState* d_rst = nullptr;
+  Ctl* d_ctl = nullptr;
+  int *d_blk_rep = nullptr, *d_cell_rep = nullptr;
+  double4 *d_pos[2] = {nullptr, nullptr}, *d_mom[2] = {nullptr, nullptr};
+  int* d_aid[2] = {nullptr, nullptr};
+  double4* d_xnbl = nullptr;
+  int *d_cellid = nullptr, *d_cslot = nullptr, *d_acell = nullptr,
+      *d_perm = nullptr;
+  int *d_cell_cnt = nullptr, *d_cell_beg = nullptr, *d_cell_num = nullptr;
+  uint32_t* d_nbr = nullptr;
+  size_t nbr_bytes = 0;
+  int* d_nnb = nullptr;
+  void *d_rho = nullptr, *d_df = nullptr, *d_epot = nullptr, *d_frc = nullptr;
+  double *d_partA = nullptr, *d_partB = nullptr, *d_maxd = nullptr;
+  ThermoRec* d_ring = nullptr;
+  void* d_tabs = nullptr;        // TabDev<R>[3]
+  void* d_seg = nullptr;         // segments of all tables
+  double4** d_posp = nullptr;    // device copy of {d_pos[0], d_pos[1]}
+  bool atoms_set = false;
+  long target_steps = 0;
+  // profiling
+  bool profiling = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> event_pool;
+  double kc_ms[KC_N] = {0, 0, 0, 0};
+  long kc_n[KC_N] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(imdg_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                    \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(ctx, IMDG_ERR_HIP, "%s: %s (%s:%d)", #call,                \
+                  hipGetErrorString(e_), __FILE__, __LINE__);                \
+  } while (0)
+
+int dalloc(imdg_ctx* c, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess)
+    return fail(c, IMDG_ERR_NOMEM, "hipMalloc(%zu): %s", bytes,
+                hipGetErrorString(e));
+  c->bufs.push_back({*p, bytes});
+  return IMDG_OK;
+}
+
+void free_all(imdg_ctx* c) {
+  for (auto& b : c->bufs) hipFree(b.p);
+  c->bufs.clear();
+  c->d_nbr = nullptr;
+  c->nbr_bytes = 0;
+}
+
+void vec_prod(const double* u, const double* v, double* w) {
+  w[0] = u[1] * v[2] - u[2] * v[1];
+  w[1] = u[2] * v[0] - u[0] * v[2];
+  w[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+double sprod(const double* a, const double* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+int copy_table(imdg_ctx* c, HostTable& h, const imdg_table* t, int ncols,
+               const char* what) {
+  h = HostTable();
+  if (!t) return IMDG_OK;
+  if (t->ncols != ncols)
+    return fail(c, IMDG_ERR_NOTABLE, "%s table has %d columns, need %d", what,
+                t->ncols, ncols);
+  if (!t->begin || !t->end || !t->invstep || !t->len || !t->table)
+    return fail(c, IMDG_ERR_ARG, "%s table has NULL arrays", what);
+  h.ncols = ncols;
+  h.maxsteps = t->maxsteps;
+  h.begin.assign(t->begin, t->begin + ncols);
+  h.end.assign(t->end, t->end + ncols);
+  h.step.assign(t->step ? t->step : t->begin, (t->step ? t->step : t->begin) + ncols);
+  h.invstep.assign(t->invstep, t->invstep + ncols);
+  h.len.assign(t->len, t->len + ncols);
+  for (int k = 0; k < ncols; k++)
+    if (h.len[k] < 3 || h.len[k] > h.maxsteps)
+      return fail(c, IMDG_ERR_NOTABLE, "%s column %d has length %d", what, k,
+                  h.len[k]);
+  h.table.assign(t->table, t->table + (size_t)(h.maxsteps + 2) * ncols);
+  h.present = true;
+  return IMDG_OK;
+}
+
+// Segment coefficients of IMD's 3-point interpolation: rows k, k+1, k+2 of
+// a column give p0, dv = p1 - p0, d2v = p2 - 2 p1 + p0 (src/potaccess.h:
+// 345-349), computed in FP64 and then rounded to R.
+template <typename R>
+int upload_tables(imdg_ctx* c) {
+  std::vector<Seg<R>> segs;
+  TabDev<R> td[3];
+  memset(td, 0, sizeof td);
+  for (int t = 0; t < 3; t++) {
+    HostTable& h = c->tab[t];
+    td[t].ncols = h.present ? h.ncols : 0;
+    if (!h.present) continue;
+    for (int col = 0; col < h.ncols; col++) {
+      td[t].off[col] = (int)segs.size();
+      td[t].len[col] = h.len[col];
+      td[t].begin[col] = (R)h.begin[col];
+      td[t].end[col] = (R)h.end[col];
+      td[t].invstep[col] = (R)h.invstep[col];
+      td[t].twoistep[col] = (R)(2 * h.invstep[col]);
+      int nc = h.ncols;
+      for (int k = 0; k < h.len[col]; k++) {
+        double p0 = h.table[(size_t)k * nc + col];
+        double p1 = h.table[(size_t)(k + 1) * nc + col];
+        double p2 = h.table[(size_t)(k + 2) * nc + col];
+        Seg<R> s;
+        s.p0 = (R)p0;
+        s.dv = (R)(p1 - p0);
+        s.d2v = (R)(p2 - 2 * p1 + p0);
+        s.pad = (R)0;
+        segs.push_back(s);
+      }
+    }
+  }
+  int rc = dalloc(c, &c->d_seg, segs.size() * sizeof(Seg<R>));
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpy(c->d_seg, segs.data(), segs.size() * sizeof(Seg<R>),
+                      hipMemcpyHostToDevice));
+  for (int t = 0; t < 3; t++) td[t].seg = (const Seg<R>*)c->d_seg;
+  rc = dalloc(c, &c->d_tabs, sizeof td);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpy(c->d_tabs, td, sizeof td, hipMemcpyHostToDevice));
+  return IMDG_OK;
+}
+
+template <typename R>
+StepArgs<R> step_args(imdg_ctx* c) {
+  StepArgs<R> a;
+  memset(&a, 0, sizeof a);
+  a.rep = c->d_rep;
+  a.rst = c->d_rst;
+  a.ctl = c->d_ctl;
+  a.blk_rep = c->d_blk_rep;
+  for (int k = 0; k < 2; k++) {
+    a.pos[k] = c->d_pos[k];
+    a.mom[k] = c->d_mom[k];
+    a.aid[k] = c->d_aid[k];
+  }
+  a.xnbl = c->d_xnbl;
+  a.nbr = c->d_nbr;
+  a.nnb = c->d_nnb;
+  a.seg = (const Seg<R>*)c->d_seg;
+  a.rho = (R*)c->d_rho;
+  a.df = (R*)c->d_df;
+  a.epot = (R*)c->d_epot;
+  a.frc = (R*)c->d_frc;
+  a.partA = c->d_partA;
+  a.partB = c->d_partB;
+  a.ring = c->d_ring;
+  a.nrep = c->nrep;
+  a.nblk = c->nblk;
+  a.kmax = c->kmax;
+  a.ntypes = c->cfg.ntypes;
+  a.ensemble = c->cfg.ensemble;
+  a.dt = c->cfg.timestep;
+  a.has_pair = c->tab[0].present;
+  a.has_eam = c->tab[1].present;
+  a.xcd_remap = c->xcd_remap;
+  return a;
+}
+
+BuildArgs build_args(imdg_ctx* c) {
+  BuildArgs b;
+  memset(&b, 0, sizeof b);
+  b.rep = c->d_rep;
+  b.rst = c->d_rst;
+  b.ctl = c->d_ctl;
+  b.blk_rep = c->d_blk_rep;
+  b.cell_rep = c->d_cell_rep;
+  for (int k = 0; k < 2; k++) {
+    b.pos[k] = c->d_pos[k];
+    b.mom[k] = c->d_mom[k];
+    b.aid[k] = c->d_aid[k];
+  }
+  b.xnbl = c->d_xnbl;
+  b.cellid = c->d_cellid;
+  b.cslot = c->d_cslot;
+  b.acell = c->d_acell;
+  b.perm = c->d_perm;
+  b.cell_cnt = c->d_cell_cnt;
+  b.cell_beg = c->d_cell_beg;
+  b.cell_num = c->d_cell_num;
+  b.nbr = c->d_nbr;
+  b.nnb = c->d_nnb;
+  b.nrep = c->nrep;
+  b.nslots = c->nslots;
+  b.ncells = c->ncells;
+  b.kmax = c->kmax;
+  return b;
+}
+
+hipEvent_t get_event(imdg_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+struct ProfScope {
+  imdg_ctx* c;
+  int kc;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(imdg_ctx* c_, int kc_) : c(c_), kc(kc_) {
+    if (c->profiling) {
+      a = get_event(c);
+      b = get_event(c);
+      hipEventRecord(a, c->stream);
+    }
+  }
+  ~ProfScope() {
+    if (c->profiling) {
+      hipEventRecord(b, c->stream);
+      c->pending.push_back({kc, {a, b}});
+    }
+  }
+};
+
+void collect_profile(imdg_ctx* c) {
+  for (auto& p : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.second.first, p.second.second) ==
+        hipSuccess) {
+      c->kc_ms[p.first] += ms;
+      c->kc_n[p.first] += 1;
+    }
+    c->event_pool.push_back(p.second.first);
+    c->event_pool.push_back(p.second.second);
+  }
+  c->pending.clear();
+}
+
+int grid_build(imdg_ctx* c) {
+  int g = (c->nslots + IMDG_BLOCK - 1) / IMDG_BLOCK;
+  return std::max(1, std::min(g, 2048));
+}
+
+int enqueue_build(imdg_ctx* c) {
+  ProfScope ps(c, KC_BUILD);
+  launch_build(build_args(c), grid_build(c), c->stream);
+  HIPCHK(c, hipGetLastError());
+  return IMDG_OK;
+}
+
+int enqueue_forces(imdg_ctx* c, bool integrate) {
+  bool nt1 = c->cfg.ntypes == 1;
+  if (c->mixed) {
+    StepArgs<float> a = step_args<float>(c);
+    const TabDev<float>* t = (const TabDev<float>*)c->d_tabs;
+    if (a.has_eam) {
+      ProfScope ps(c, KC_DENSITY);
+      launch_density<float>(a, t, nt1, c->nblk, c->stream);
+    }
+    ProfScope ps(c, KC_FORCE);
+    launch_force<float>(a, t, nt1, integrate, c->nblk, c->stream);
+  } else {
+    StepArgs<double> a = step_args<double>(c);
+    const TabDev<double>* t = (const TabDev<double>*)c->d_tabs;
+    if (a.has_eam) {
+      ProfScope ps(c, KC_DENSITY);
+      launch_density<double>(a, t, nt1, c->nblk, c->stream);
+    }
+    ProfScope ps(c, KC_FORCE);
+    launch_force<double>(a, t, nt1, integrate, c->nblk, c->stream);
+  }
+  HIPCHK(c, hipGetLastError());
+  return IMDG_OK;
+}
+
+int read_ctl(imdg_ctx* c, Ctl* h) {
+  HIPCHK(c, hipMemcpyAsync(h, c->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return IMDG_OK;
+}
+
+// Grow the list after an overflow and rebuild; returns 1 if it had to.
+int fix_overflow(imdg_ctx* c, int* fixed) {
+  *fixed = 0;
+  Ctl h;
+  int rc = read_ctl(c, &h);
+  if (rc) return rc;
+  if (!h.frozen) return IMDG_OK;
+  int newk = std::max(c->kmax + 8, (int)ceil(h.kmax_seen * 1.2));
+  newk = (newk + 7) & ~7;
+  hipFree(c->d_nbr);
+  for (auto it = c->bufs.begin(); it != c->bufs.end(); ++it)
+    if (it->p == c->d_nbr) {
+      c->bufs.erase(it);
+      break;
+    }
+  c->d_nbr = nullptr;
+  c->kmax = newk;
+  size_t bytes = (size_t)(c->nslots / IMDG_WAVE) * newk * IMDG_WAVE * 4;
+  if ((rc = dalloc(c, (void**)&c->d_nbr, bytes))) return rc;
+  c->nbr_bytes = bytes;
+  int zero2[2] = {0, 0};  // frozen, overflow
+  HIPCHK(c, hipMemcpyAsync(&c->d_ctl->frozen, zero2, 2 * sizeof(int),
+                           hipMemcpyHostToDevice, c->stream));
+  if ((rc = enqueue_build(c))) return rc;
+  *fixed = 1;
+  return IMDG_OK;
+}
+
+int ensure_ready(imdg_ctx* c) {
+  if (!c->atoms_set) return fail(c, IMDG_ERR_STATE, "no atoms set");
+  if (!c->d_nbr) {
+    size_t bytes = (size_t)(c->nslots / IMDG_WAVE) * c->kmax * IMDG_WAVE * 4;
+    int rc = dalloc(c, (void**)&c->d_nbr, bytes);
+    if (rc) return rc;
+    c->nbr_bytes = bytes;
+  }
+  return IMDG_OK;
+}
+
+// Run the build chain until it fits (calc_forces / make_nblist path).
+int build_until_fits(imdg_ctx* c) {
+  int rc = enqueue_build(c);
+  if (rc) return rc;
+  for (int tries = 0; tries < 8; tries++) {
+    int fixed = 0;
+    if ((rc = fix_overflow(c, &fixed))) return rc;
+    if (!fixed) return IMDG_OK;
+  }
+  return fail(c, IMDG_ERR_NOMEM, "neighbour list keeps overflowing");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* imdg_last_error(const imdg_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+int imdg_ctx_create(const imdg_config* cfg, imdg_ctx** out) {
+  if (!cfg || !out) return IMDG_ERR_ARG;
+  *out = nullptr;
+  if (cfg->precision != IMDG_FP64 && cfg->precision != IMDG_MIXED)
+    return IMDG_ERR_ARG;
+  if (cfg->ensemble != IMDG_NVE && cfg->ensemble != IMDG_NVT)
+    return IMDG_ERR_ARG;
+  if (cfg->ntypes < 1 || cfg->ntypes > IMDG_MAXTYPES) return IMDG_ERR_ARG;
+  if (!(cfg->timestep > 0) || !(cfg->nbl_margin >= 0)) return IMDG_ERR_ARG;
+  imdg_ctx* c = new imdg_ctx();
+  c->cfg = *cfg;
+  if (!(c->cfg.nbl_size > 0)) c->cfg.nbl_size = 1.1;
+  c->mixed = cfg->precision == IMDG_MIXED;
+  const char* remap = getenv("IMDG_XCD_REMAP");
+  c->xcd_remap = remap ? atoi(remap) != 0 : 1;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 ||
+      cfg->device < 0 || cfg->device >= ndev) {
+    delete c;
+    return IMDG_ERR_HIP;
+  }
+  c->dev = cfg->device;
+  if (hipSetDevice(c->dev) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) !=
+          hipSuccess) {
+    delete c;
+    return IMDG_ERR_HIP;
+  }
+  *out = c;
+  return IMDG_OK;
+}
+
+int imdg_ctx_destroy(imdg_ctx* c) {
+  if (!c) return IMDG_ERR_ARG;
+  hipSetDevice(c->dev);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  collect_profile(c);
+  for (auto e : c->event_pool) hipEventDestroy(e);
+  free_all(c);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return IMDG_OK;
+}
+
+int imdg_set_tables(imdg_ctx* c, const imdg_table* pair, const imdg_table* rho,
+                    const imdg_table* embed) {
+  if (!c) return IMDG_ERR_ARG;
+  if ((rho == nullptr) != (embed == nullptr))
+    return fail(c, IMDG_ERR_NOTABLE, "rho and embed go together");
+  if (!pair && !rho) return fail(c, IMDG_ERR_NOTABLE, "no tables");
+  if (c->atoms_set) return fail(c, IMDG_ERR_STATE, "set tables before atoms");
+  int nt = c->cfg.ntypes, rc;
+  if ((rc = copy_table(c, c->tab[0], pair, nt * nt, "pair"))) return rc;
+  if ((rc = copy_table(c, c->tab[1], rho, nt * nt, "rho"))) return rc;
+  if ((rc = copy_table(c, c->tab[2], embed, nt, "embed"))) return rc;
+  // IMD's cellsz: the largest end of the radial tables
+  // (src/imd_potential.c:367, 406)
+  c->cellsz_tab = 0.0;
+  for (int t = 0; t < 2; t++)
+    for (int k = 0; k < c->tab[t].ncols; k++)
+      c->cellsz_tab = std::max(c->cellsz_tab, c->tab[t].end[k]);
+  c->tables_set = true;
+  return IMDG_OK;
+}
+
+int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
+                   const int* natoms, const double* x, const double* p,
+                   const int* type, const double* mass) {
+  if (!c) return IMDG_ERR_ARG;
+  if (!c->tables_set) return fail(c, IMDG_ERR_STATE, "set tables first");
+  if (nrep < 1 || !reps || !natoms || !x || !type || !mass)
+    return fail(c, IMDG_ERR_ARG, "bad set_atoms arguments");
+  hipSetDevice(c->dev);
+  free_all(c);
+  c->atoms_set = false;
+  c->nrep = nrep;
+  c->rep.assign(nrep, RepDev());
+  c->natoms.assign(natoms, natoms + nrep);
+  c->in_off.assign(nrep, 0);
+  long tot = 0;
+  int slot = 0, cell = 0;
+  double kest = 0.0;
+  for (int r = 0; r < nrep; r++) {
+    RepDev& R = c->rep[r];
+    memset(&R, 0, sizeof R);
+    if (natoms[r] < 0) return fail(c, IMDG_ERR_ARG, "negative atom count");
+    const double* bx[3] = {reps[r].box_x, reps[r].box_y, reps[r].box_z};
+    for (int d = 0; d < 3; d++)
+      for (int e = 0; e < 3; e++) R.box[d][e] = bx[d][e];
+    // make_box (src/imd_geom_3d.c:52-104)
+    vec_prod(R.box[1], R.box[2], R.tbox[0]);
+    vec_prod(R.box[2], R.box[0], R.tbox[1]);
+    vec_prod(R.box[0], R.box[1], R.tbox[2]);
+    double vol = sprod(R.box[0], R.tbox[0]);
+    if (vol == 0) return fail(c, IMDG_ERR_GEOM, "replica %d: flat box", r);
+    double height[3];
+    for (int d = 0; d < 3; d++)
+      for (int e = 0; e < 3; e++) R.tbox[d][e] /= vol;
+    for (int d = 0; d < 3; d++) height[d] = 1.0 / sprod(R.tbox[d], R.tbox[d]);
+    // init_cells: skin added once, grid (src/imd_geom_3d.c:122-144)
+    double cs = sqrt(c->cellsz_tab) + c->cfg.nbl_margin;
+    R.cellsz = cs * cs;
+    for (int d = 0; d < 3; d++) {
+      R.pbc[d] = reps[r].pbc[d] ? 1 : 0;
+      double scale = sqrt(R.cellsz / height[d]);
+      R.gdim[d] = (int)(1.0 / scale);
+      if (R.gdim[d] < 1 || (R.pbc[d] && R.gdim[d] < 3))
+        return fail(c, IMDG_ERR_GEOM,
+                    "replica %d: %d cells along axis %d (periodic axes need "
+                    "at least 3 cells of the list radius)",
+                    r, R.gdim[d], d);
+    }
+    double h = 0.5 * c->cfg.nbl_margin;
+    R.skin2 = h * h;
+    R.temperature = reps[r].temperature;
+    R.isq_tau_eta = reps[r].isq_tau_eta;
+    for (int sx = -1; sx <= 1; sx++)
+      for (int sy = -1; sy <= 1; sy++)
+        for (int sz = -1; sz <= 1; sz++) {
+          int code = (sx + 1) * 9 + (sy + 1) * 3 + (sz + 1);
+          for (int e = 0; e < 3; e++)
+            R.shift[code][e] =
+                sx * R.box[0][e] + sy * R.box[1][e] + sz * R.box[2][e];
+        }
+    for (int e = 0; e < 3; e++) R.shift[IMDG_CODE_PAD][e] = IMDG_PAD_SHIFT;
+    R.atom_off = slot;
+    R.natoms = natoms[r];
+    R.nslots = ((natoms[r] + IMDG_BLOCK - 1) / IMDG_BLOCK) * IMDG_BLOCK;
+    if (R.nslots == 0) R.nslots = IMDG_BLOCK;
+    R.cell_off = cell;
+    R.ncells = R.gdim[0] * R.gdim[1] * R.gdim[2];
+    R.blk_off = slot / IMDG_BLOCK;
+    R.nblk = R.nslots / IMDG_BLOCK;
+    R.nactive = 3 * natoms[r];
+    c->in_off[r] = tot;
+    tot += natoms[r];
+    slot += R.nslots;
+    cell += R.ncells;
+    double dens = natoms[r] / fabs(vol);
+    double rl = sqrt(R.cellsz);
+    kest = std::max(kest, dens * 4.18879020478639 * rl * rl * rl);
+  }
+  if (slot >= (1 << IMDG_IDX_BITS))
+    return fail(c, IMDG_ERR_ARG, "too many atoms for 27-bit list indices");
+  c->natoms_tot = tot;
+  c->nslots = slot;
+  c->nblk = slot / IMDG_BLOCK;
+  c->ncells = cell;
+  c->kmax = std::max(16, (int)ceil(kest * c->cfg.nbl_size * 1.25));
+  c->kmax = (c->kmax + 7) & ~7;
+
+  int rc;
+#define ALLOC(ptr, bytes)                                        \
+  if ((rc = dalloc(c, (void**)&(ptr), (bytes)))) return rc
+  size_t ns = c->nslots, rs = c->mixed ? sizeof(float) : sizeof(double);
+  ALLOC(c->d_rep, sizeof(RepDev) * nrep);
+  ALLOC(c->d_rst, sizeof(RepState) * nrep);
+  ALLOC(c->d_ctl, sizeof(Ctl));
+  ALLOC(c->d_blk_rep, sizeof(int) * c->nblk);
+  ALLOC(c->d_cell_rep, sizeof(int) * c->ncells);
+  for (int k = 0; k < 2; k++) {
+    ALLOC(c->d_pos[k], sizeof(double4) * ns);
+    ALLOC(c->d_mom[k], sizeof(double4) * ns);
+    ALLOC(c->d_aid[k], sizeof(int) * ns);
+  }
+  ALLOC(c->d_posp, sizeof(double4*) * 2);
+  ALLOC(c->d_xnbl, sizeof(double4) * ns);
+  ALLOC(c->d_cellid, sizeof(int) * ns);
+  ALLOC(c->d_cslot, sizeof(int) * ns);
+  ALLOC(c->d_acell, sizeof(int) * ns);
+  ALLOC(c->d_perm, sizeof(int) * ns);
+  ALLOC(c->d_cell_cnt, sizeof(int) * c->ncells);
+  ALLOC(c->d_cell_beg, sizeof(int) * c->ncells);
+  ALLOC(c->d_cell_num, sizeof(int) * c->ncells);
+  ALLOC(c->d_nnb, sizeof(int) * ns);
+  ALLOC(c->d_rho, rs * ns);
+  ALLOC(c->d_df, rs * ns);
+  ALLOC(c->d_epot, rs * ns);
+  ALLOC(c->d_frc, rs * 4 * ns);
+  ALLOC(c->d_partA, sizeof(double) * c->nblk);
+  ALLOC(c->d_partB, sizeof(double) * PB_N * c->nblk);
+  ALLOC(c->d_maxd, sizeof(double) * c->nblk);
+  ALLOC(c->d_ring, sizeof(ThermoRec) * IMDG_RING * nrep);
+#undef ALLOC
+  if ((rc = c->mixed ? upload_tables<float>(c) : upload_tables<double>(c)))
+    return rc;
+
+  // host staging in slot order
+  std::vector<double4> hpos(ns), hmom(ns);
+  std::vector<int> haid(ns, -1), hblk(c->nblk), hcrep(c->ncells);
+  for (size_t k = 0; k < ns; k++) {
+    hpos[k] = make_double4(0, 0, 0, -1.0);
+    hmom[k] = make_double4(0, 0, 0, 1.0);
+  }
+  for (int r = 0; r < nrep; r++) {
+    RepDev& R = c->rep[r];
+    for (int b = 0; b < R.nblk; b++) hblk[R.blk_off + b] = r;
+    for (int q = 0; q < R.ncells; q++) hcrep[R.cell_off + q] = r;
+    for (int k = 0; k < R.natoms; k++) {
+      long in = c->in_off[r] + k;
+      int t = type[in];
+      if (t < 0 || t >= c->cfg.ntypes)
+        return fail(c, IMDG_ERR_ARG, "atom %ld has type %d", in, t);
+      if (!(mass[in] > 0))
+        return fail(c, IMDG_ERR_ARG, "atom %ld has mass %g", in, mass[in]);
+      double q[3] = {x[3 * in], x[3 * in + 1], x[3 * in + 2]};
+      // back_into_box (src/imd_geom_3d.c:1083-1110)
+      for (int d = 0; d < 3; d++) {
+        if (R.pbc[d] != 1) continue;
+        double i = floor(q[0] * R.tbox[d][0] + q[1] * R.tbox[d][1] +
+                         q[2] * R.tbox[d][2]);
+        q[0] -= i * R.box[d][0];
+        q[1] -= i * R.box[d][1];
+        q[2] -= i * R.box[d][2];
+      }
+      int s = R.atom_off + k;
+      hpos[s] = make_double4(q[0], q[1], q[2], (double)t);
+      hmom[s] = make_double4(p ? p[3 * in] : 0.0, p ? p[3 * in + 1] : 0.0,
+                             p ? p[3 * in + 2] : 0.0, mass[in]);
+      haid[s] = (int)in;
+    }
+  }
+  std::vector<RepState> hrst(nrep);
+  for (int r = 0; r < nrep; r++) {
+    memset(&hrst[r], 0, sizeof(RepState));
+    hrst[r].eta = reps[r].eta;
+    hrst[r].need_rebuild = 1;
+  }
+  Ctl hc;
+  memset(&hc, 0, sizeof hc);
+  hc.any_rebuild = 1;
+  double4* pp[2] = {c->d_pos[0], c->d_pos[1]};
+  hipStream_t st = c->stream;
+  HIPCHK(c, hipMemcpyAsync(c->d_rep, c->rep.data(), sizeof(RepDev) * nrep,
+                           hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->d_rst, hrst.data(), sizeof(RepState) * nrep,
+                           hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->d_ctl, &hc, sizeof hc, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->d_blk_rep, hblk.data(), sizeof(int) * c->nblk,
+                           hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->d_cell_rep, hcrep.data(),
+                           sizeof(int) * c->ncells, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->d_posp, pp, sizeof pp, hipMemcpyHostToDevice, st));
+  for (int k = 0; k < 2; k++) {
+    HIPCHK(c, hipMemcpyAsync(c->d_pos[k], hpos.data(), sizeof(double4) * ns,
+                             hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(c->d_mom[k], hmom.data(), sizeof(double4) * ns,
+                             hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(c->d_aid[k], haid.data(), sizeof(int) * ns,
+                             hipMemcpyHostToDevice, st));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_xnbl, hpos.data(), sizeof(double4) * ns,
+                           hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemsetAsync(c->d_cell_cnt, 0, sizeof(int) * c->ncells, st));
+  HIPCHK(c, hipMemsetAsync(c->d_cell_beg, 0, sizeof(int) * c->ncells, st));
+  HIPCHK(c, hipMemsetAsync(c->d_cell_num, 0, sizeof(int) * c->ncells, st));
+  HIPCHK(c, hipMemsetAsync(c->d_acell, 0, sizeof(int) * ns, st));
+  HIPCHK(c, hipMemsetAsync(c->d_nnb, 0, sizeof(int) * ns, st));
+  HIPCHK(c, hipMemsetAsync(c->d_perm, 0, sizeof(int) * ns, st));
+  HIPCHK(c, hipMemsetAsync(c->d_rho, 0, rs * ns, st));
+  HIPCHK(c, hipMemsetAsync(c->d_df, 0, rs * ns, st));
+  HIPCHK(c, hipMemsetAsync(c->d_epot, 0, rs * ns, st));
+  HIPCHK(c, hipMemsetAsync(c->d_frc, 0, rs * 4 * ns, st));
+  HIPCHK(c, hipMemsetAsync(c->d_ring, 0, sizeof(ThermoRec) * IMDG_RING * nrep,
+                           st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  c->atoms_set = true;
+  c->target_steps = 0;
+  return ensure_ready(c);
+}
+
+int imdg_set_nvt(imdg_ctx* c, int rep, double temperature, double eta,
+                 double isq_tau_eta) {
+  if (!c || !c->atoms_set || rep < 0 || rep >= c->nrep) return IMDG_ERR_ARG;
+  c->rep[rep].temperature = temperature;
+  c->rep[rep].isq_tau_eta = isq_tau_eta;
+  HIPCHK(c, hipMemcpyAsync(&c->d_rep[rep], &c->rep[rep], sizeof(RepDev),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&c->d_rst[rep].eta, &eta, sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return IMDG_OK;
+}
+
+int imdg_calc_forces(imdg_ctx* c, int steps) {
+  (void)steps;
+  if (!c) return IMDG_ERR_ARG;
+  int rc = ensure_ready(c);
+  if (rc) return rc;
+  if ((rc = build_until_fits(c))) return rc;
+  if ((rc = enqueue_forces(c, false))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_profile(c);
+  return IMDG_OK;
+}
+
+int imdg_move_atoms(imdg_ctx* c) {
+  if (!c) return IMDG_ERR_ARG;
+  int rc = ensure_ready(c);
+  if (rc) return rc;
+  {
+    ProfScope ps(c, KC_MOVE);
+    if (c->mixed)
+      launch_move<float>(step_args<float>(c), c->nblk, c->stream);
+    else
+      launch_move<double>(step_args<double>(c), c->nblk, c->stream);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_profile(c);
+  c->target_steps += 1;
+  return IMDG_OK;
+}
+
+int imdg_check_nblist(imdg_ctx* c, int* invalid) {
+  if (!c) return IMDG_ERR_ARG;
+  int rc = ensure_ready(c);
+  if (rc) return rc;
+  launch_check(c->d_rep, c->d_rst, c->d_ctl, c->d_blk_rep, c->d_posp,
+               c->d_xnbl, c->d_maxd, c->nrep, c->nblk, c->stream);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> maxd(c->nblk);
+  std::vector<RepState> st(c->nrep);
+  HIPCHK(c, hipMemcpyAsync(maxd.data(), c->d_maxd, sizeof(double) * c->nblk,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(st.data(), c->d_rst, sizeof(RepState) * c->nrep,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int any = 0;
+  for (int r = 0; r < c->nrep; r++) {
+    double m = 0.0;
+    for (int b = 0; b < c->rep[r].nblk; b++)
+      m = std::max(m, maxd[c->rep[r].blk_off + b]);
+    st[r].need_rebuild = (m > c->rep[r].skin2) ? 1 : st[r].need_rebuild;
+    any |= st[r].need_rebuild;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_rst, st.data(), sizeof(RepState) * c->nrep,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&c->d_ctl->any_rebuild, &any, sizeof(int),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (invalid) *invalid = any;
+  return IMDG_OK;
+}
+
+int imdg_make_nblist(imdg_ctx* c) {
+  if (!c) return IMDG_ERR_ARG;
+  int rc = ensure_ready(c);
+  if (rc) return rc;
+  std::vector<RepState> st(c->nrep);
+  HIPCHK(c, hipMemcpyAsync(st.data(), c->d_rst, sizeof(RepState) * c->nrep,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto& s : st) s.need_rebuild = 1;
+  int one = 1;
+  HIPCHK(c, hipMemcpyAsync(c->d_rst, st.data(), sizeof(RepState) * c->nrep,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&c->d_ctl->any_rebuild, &one, sizeof(int),
+                           hipMemcpyHostToDevice, c->stream));
+  if ((rc = build_until_fits(c))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_profile(c);
+  return IMDG_OK;
+}
+
+int imdg_deallocate_nblist(imdg_ctx* c) {
+  if (!c) return IMDG_ERR_ARG;
+  if (!c->atoms_set) return IMDG_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto it = c->bufs.begin(); it != c->bufs.end(); ++it)
+    if (it->p == c->d_nbr) {
+      hipFree(it->p);
+      c->bufs.erase(it);
+      break;
+    }
+  c->d_nbr = nullptr;
+  c->nbr_bytes = 0;
+  std::vector<RepState> st(c->nrep);
+  HIPCHK(c, hipMemcpy(st.data(), c->d_rst, sizeof(RepState) * c->nrep,
+                      hipMemcpyDeviceToHost));
+  for (auto& s : st) s.need_rebuild = 1;
+  int one = 1;
+  HIPCHK(c, hipMemcpy(c->d_rst, st.data(), sizeof(RepState) * c->nrep,
+                      hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(&c->d_ctl->any_rebuild, &one, sizeof(int),
+                      hipMemcpyHostToDevice));
+  return IMDG_OK;
+}
+
+int imdg_run(imdg_ctx* c, long nsteps) {
+  if (!c || nsteps < 0) return IMDG_ERR_ARG;
+  int rc = ensure_ready(c);
+  if (rc) return rc;
+  for (long s = 0; s < nsteps; s++) {
+    if ((rc = enqueue_build(c))) return rc;
+    if ((rc = enqueue_forces(c, true))) return rc;
+  }
+  c->target_steps += nsteps;
+  return IMDG_OK;
+}
+
+int imdg_sync(imdg_ctx* c) {
+  if (!c) return IMDG_ERR_ARG;
+  hipSetDevice(c->dev);
+  for (int guard = 0; guard < 64; guard++) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    collect_profile(c);
+    Ctl h;
+    int rc = read_ctl(c, &h);
+    if (rc) return rc;
+    if (!h.frozen) {
+      if (h.steps_done != c->target_steps)
+        return fail(c, IMDG_ERR_STATE, "device did %ld steps, expected %ld",
+                    h.steps_done, c->target_steps);
+      return IMDG_OK;
+    }
+    int fixed = 0;
+    if ((rc = fix_overflow(c, &fixed))) return rc;
+    long missing = c->target_steps - h.steps_done;
+    c->target_steps = h.steps_done;
+    if ((rc = imdg_run(c, missing))) return rc;
+  }
+  return fail(c, IMDG_ERR_NOMEM, "neighbour list keeps overflowing");
+}
+
+int imdg_get_atoms(imdg_ctx* c, double* x, double* p, double* f, double* epot,
+                   double* rho, double* df) {
+  if (!c || !c->atoms_set) return IMDG_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  Ctl h;
+  int rc = read_ctl(c, &h);
+  if (rc) return rc;
+  size_t ns = c->nslots;
+  std::vector<double4> hp(ns), hm(ns);
+  std::vector<int> ha(ns);
+  HIPCHK(c, hipMemcpy(hp.data(), c->d_pos[h.pos_cur], sizeof(double4) * ns,
+                      hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(hm.data(), c->d_mom[h.rec_cur], sizeof(double4) * ns,
+                      hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(ha.data(), c->d_aid[h.rec_cur], sizeof(int) * ns,
+                      hipMemcpyDeviceToHost));
+  std::vector<double> hf(4 * ns), he(ns), hr(ns), hd(ns);
+  if (c->mixed) {
+    std::vector<float> t4(4 * ns), t(ns);
+    HIPCHK(c, hipMemcpy(t4.data(), c->d_frc, 4 * ns * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < 4 * ns; k++) hf[k] = t4[k];
+    HIPCHK(c, hipMemcpy(t.data(), c->d_epot, ns * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < ns; k++) he[k] = t[k];
+    HIPCHK(c, hipMemcpy(t.data(), c->d_rho, ns * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < ns; k++) hr[k] = t[k];
+    HIPCHK(c, hipMemcpy(t.data(), c->d_df, ns * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < ns; k++) hd[k] = t[k];
+  } else {
+    HIPCHK(c, hipMemcpy(hf.data(), c->d_frc, 4 * ns * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(he.data(), c->d_epot, ns * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hr.data(), c->d_rho, ns * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hd.data(), c->d_df, ns * 8, hipMemcpyDeviceToHost));
+  }
+  for (int r = 0; r < c->nrep; r++) {
+    const RepDev& R = c->rep[r];
+    for (int k = 0; k < R.natoms; k++) {
+      int s = R.atom_off + k;
+      long in = ha[s];
+      if (in < 0 || in >= c->natoms_tot)
+        return fail(c, IMDG_ERR_STATE, "slot %d has no atom", s);
+      if (x) {
+        x[3 * in] = hp[s].x;
+        x[3 * in + 1] = hp[s].y;
+        x[3 * in + 2] = hp[s].z;
+      }
+      if (p) {
+        p[3 * in] = hm[s].x;
+        p[3 * in + 1] = hm[s].y;
+        p[3 * in + 2] = hm[s].z;
+      }
+      if (f) {
+        f[3 * in] = hf[4 * s];
+        f[3 * in + 1] = hf[4 * s + 1];
+        f[3 * in + 2] = hf[4 * s + 2];
+      }
+      if (epot) epot[in] = he[s];
+      if (rho) rho[in] = hr[s];
+      if (df) df[in] = hd[s];
+    }
+  }
+  return IMDG_OK;
+}
+
+int imdg_get_thermo(imdg_ctx* c, int rep, imdg_thermo* out) {
+  if (!c || !c->atoms_set || rep < 0 || rep >= c->nrep || !out)
+    return IMDG_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  Ctl h;
+  int rc = read_ctl(c, &h);
+  if (rc) return rc;
+  RepState st;
+  HIPCHK(c, hipMemcpy(&st, &c->d_rst[rep], sizeof st, hipMemcpyDeviceToHost));
+  memset(out, 0, sizeof *out);
+  out->step = h.steps_done;
+  out->epot = st.epot;
+  out->virial = st.virial;
+  out->eta = st.eta;
+  out->natoms = c->rep[rep].natoms;
+  if (h.steps_done > 0) {
+    ThermoRec tr;
+    long slot = (h.steps_done - 1) % IMDG_RING;
+    HIPCHK(c, hipMemcpy(&tr, &c->d_ring[slot * c->nrep + rep], sizeof tr,
+                        hipMemcpyDeviceToHost));
+    out->ekin = tr.ekin;
+    out->max_disp2 = tr.max_disp2;
+    out->rebuilt = tr.rebuilt;
+  }
+  return IMDG_OK;
+}
+
+int imdg_get_thermo_history(imdg_ctx* c, long first, long count,
+                            imdg_thermo* out) {
+  if (!c || !c->atoms_set || first < 0 || count < 0 || !out)
+    return IMDG_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  Ctl h;
+  int rc = read_ctl(c, &h);
+  if (rc) return rc;
+  if (first + count > h.steps_done || first < h.steps_done - IMDG_RING)
+    return fail(c, IMDG_ERR_ARG, "steps [%ld,%ld) not in the ring (done %ld)",
+                first, first + count, h.steps_done);
+  std::vector<ThermoRec> ring((size_t)IMDG_RING * c->nrep);
+  HIPCHK(c, hipMemcpy(ring.data(), c->d_ring, sizeof(ThermoRec) * ring.size(),
+                      hipMemcpyDeviceToHost));
+  for (long s = 0; s < count; s++)
+    for (int r = 0; r < c->nrep; r++) {
+      const ThermoRec& t = ring[((first + s) % IMDG_RING) * c->nrep + r];
+      imdg_thermo& o = out[s * c->nrep + r];
+      o.step = t.step;
+      o.epot = t.epot;
+      o.ekin = t.ekin;
+      o.virial = t.virial;
+      o.eta = t.eta;
+      o.max_disp2 = t.max_disp2;
+      o.rebuilt = t.rebuilt;
+      o.natoms = t.natoms;
+    }
+  return IMDG_OK;
+}
+
+int imdg_get_cells(imdg_ctx* c, int* cell3) {
+  if (!c || !c->atoms_set || !cell3) return IMDG_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  Ctl h;
+  int rc = read_ctl(c, &h);
+  if (rc) return rc;
+  size_t ns = c->nslots;
+  std::vector<int> ha(ns), hc(ns);
+  HIPCHK(c, hipMemcpy(ha.data(), c->d_aid[h.rec_cur], sizeof(int) * ns,
+                      hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(hc.data(), c->d_acell, sizeof(int) * ns,
+                      hipMemcpyDeviceToHost));
+  for (int r = 0; r < c->nrep; r++) {
+    const RepDev& R = c->rep[r];
+    for (int k = 0; k < R.natoms; k++) {
+      int s = R.atom_off + k;
+      int lc = hc[s] - R.cell_off;
+      long in = ha[s];
+      cell3[3 * in + 2] = lc % R.gdim[2];
+      cell3[3 * in + 1] = (lc / R.gdim[2]) % R.gdim[1];
+      cell3[3 * in + 0] = lc / (R.gdim[1] * R.gdim[2]);
+    }
+  }
+  return IMDG_OK;
+}
+
+int imdg_get_pairs(imdg_ctx* c, long* out, long cap, long* npairs) {
+  if (!c || !c->atoms_set) return IMDG_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  Ctl h;
+  int rc = read_ctl(c, &h);
+  if (rc) return rc;
+  if (!c->d_nbr) return fail(c, IMDG_ERR_STATE, "no list");
+  size_t ns = c->nslots;
+  std::vector<int> ha(ns), hn(ns);
+  std::vector<uint32_t> hl(c->nbr_bytes / 4);
+  HIPCHK(c, hipMemcpy(ha.data(), c->d_aid[h.rec_cur], sizeof(int) * ns,
+                      hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(hn.data(), c->d_nnb, sizeof(int) * ns,
+                      hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(hl.data(), c->d_nbr, c->nbr_bytes, hipMemcpyDeviceToHost));
+  long np = 0;
+  for (int r = 0; r < c->nrep; r++) {
+    const RepDev& R = c->rep[r];
+    for (int k = 0; k < R.natoms; k++) {
+      int s = R.atom_off + k;
+      long ii = ha[s] - c->in_off[r];
+      for (int m = 0; m < hn[s]; m++) {
+        size_t at = (((size_t)(s >> 6) * (c->kmax >> 2) + (m >> 2)) * IMDG_WAVE +
+                     (s & 63)) * IMDG_GROUP + (m & 3);
+        uint32_t e = hl[at];
+        int j = (int)(e & IMDG_IDX_MASK), code = (int)(e >> IMDG_IDX_BITS);
+        if (code == IMDG_CODE_PAD) continue;
+        long jj = ha[j] - c->in_off[r];
+        if (ii > jj) continue;  // each unordered pair once, from its lower index
+        int sx = code / 9 - 1, sy = (code / 3) % 3 - 1, sz = code % 3 - 1;
+        if (ii == jj && (sx < 0 || (sx == 0 && (sy < 0 || (sy == 0 && sz < 0)))))
+          continue;
+        if (np < cap) {
+          long* o = out + 6 * np;
+          o[0] = r;
+          o[1] = ii;
+          o[2] = jj;
+          o[3] = sx;
+          o[4] = sy;
+          o[5] = sz;
+        }
+        np++;
+      }
+    }
+  }
+  if (npairs) *npairs = np;
+  return IMDG_OK;
+}
+
+int imdg_get_grid(imdg_ctx* c, int rep, int* gdim3) {
+  if (!c || !c->atoms_set || rep < 0 || rep >= c->nrep || !gdim3)
+    return IMDG_ERR_ARG;
+  for (int d = 0; d < 3; d++) gdim3[d] = c->rep[rep].gdim[d];
+  return IMDG_OK;
+}
+
+int imdg_get_info(imdg_ctx* c, imdg_info* info) {
+  if (!c || !info) return IMDG_ERR_ARG;
+  memset(info, 0, sizeof *info);
+  info->nrep = c->nrep;
+  info->natoms = c->natoms_tot;
+  info->nslots = c->nslots;
+  info->kmax = c->kmax;
+  info->lanes_per_atom = 1;
+  info->list_bytes = (double)c->nbr_bytes;
+  if (!c->atoms_set) return IMDG_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  Ctl h;
+  int rc = read_ctl(c, &h);
+  if (rc) return rc;
+  RepState st;
+  HIPCHK(c, hipMemcpy(&st, c->d_rst, sizeof st, hipMemcpyDeviceToHost));
+  info->kmax_seen = h.kmax_seen;
+  info->steps_done = h.steps_done;
+  info->is_short = h.is_short;
+  info->nbl_count = st.nbl_count;
+  std::vector<int> hn(c->nslots);
+  HIPCHK(c, hipMemcpy(hn.data(), c->d_nnb, sizeof(int) * c->nslots,
+                      hipMemcpyDeviceToHost));
+  double tot = 0.0;
+  for (int r = 0; r < c->nrep; r++)
+    for (int k = 0; k < c->rep[r].natoms; k++) tot += hn[c->rep[r].atom_off + k];
+  info->mean_k = c->natoms_tot ? tot / c->natoms_tot : 0.0;
+  return IMDG_OK;
+}
+
+int imdg_profile(imdg_ctx* c, int enable) {
+  if (!c) return IMDG_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_profile(c);
+  c->profiling = enable != 0;
+  for (int k = 0; k < KC_N; k++) {
+    c->kc_ms[k] = 0;
+    c->kc_n[k] = 0;
+  }
+  return IMDG_OK;
+}
+
+int imdg_kernel_times(imdg_ctx* c, const char** names, double* ms,
+                      long* launches, int n) {
+  if (!c) return IMDG_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_profile(c);
+  for (int k = 0; k < KC_N && k < n; k++) {
+    if (names) names[k] = kKernelNames[k];
+    if (ms) ms[k] = c->kc_ms[k];
+    if (launches) launches[k] = c->kc_n[k];
+  }
+  return KC_N;
+}
+
+}  // extern "C"

@@ -1,0 +1,948 @@
+/*
+ * imd_oracle.c -- CPU restatement of IMD's neighbour-list pair/EAM path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).  See
+ * imd_oracle.h for the reference lines each part follows.  Build with
+ * -ffp-contract=off so every product and sum rounds separately, as the
+ * reference's generic x86-64 build does.
+ */
+#include "imd_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PSTEP 50 /* src/config.h:251 */
+
+/* ------------------------------------------------------------------------ */
+/* tables                                                                    */
+/* ------------------------------------------------------------------------ */
+
+static int table_alloc_info(orc_table *pt, int ncols) {
+  memset(pt, 0, sizeof(*pt));
+  pt->ncols = ncols;
+  pt->begin = calloc(ncols, sizeof(double));
+  pt->end = calloc(ncols, sizeof(double));
+  pt->step = calloc(ncols, sizeof(double));
+  pt->invstep = calloc(ncols, sizeof(double));
+  pt->len = calloc(ncols, sizeof(int));
+  if (!pt->begin || !pt->end || !pt->step || !pt->invstep || !pt->len)
+    return ORC_ERR_NOMEM;
+  return ORC_OK;
+}
+
+void orc_table_free(orc_table *pt) {
+  if (!pt) return;
+  free(pt->begin);
+  free(pt->end);
+  free(pt->step);
+  free(pt->invstep);
+  free(pt->len);
+  free(pt->table);
+  memset(pt, 0, sizeof(*pt));
+}
+
+int orc_table_nrows(const orc_table *pt) { return pt->maxsteps + 2; }
+
+/* init_threepoint (src/imd_potential.c:1256-1272): continue the last
+   interpolation parabola by two rows. */
+static int init_threepoint(orc_table *pt) {
+  int nc = pt->ncols;
+  for (int col = 0; col < nc; col++) {
+    double *y = pt->table + col;
+    int n = pt->len[col];
+    if (n < 3) return ORC_ERR_FORMAT;
+    y[n * nc] = 3 * y[(n - 1) * nc] - 3 * y[(n - 2) * nc] + y[(n - 3) * nc];
+    y[(n + 1) * nc] =
+        6 * y[(n - 1) * nc] - 8 * y[(n - 2) * nc] + 3 * y[(n - 3) * nc];
+  }
+  return ORC_OK;
+}
+
+/* format 1 (src/imd_potential.c:297-376): rows "r2 v0 v1 ..." */
+static int read_format1(orc_table *pt, FILE *f, int radial, double *cellsz) {
+  int nc = pt->ncols, npot = 0;
+  double r2 = 0.0, r2_start = 0.0, val;
+  pt->maxsteps = PSTEP;
+  pt->table = malloc(sizeof(double) * nc * pt->maxsteps);
+  if (!pt->table) return ORC_ERR_NOMEM;
+  for (;;) {
+    if (npot > 0 && npot % PSTEP == 0) {
+      pt->maxsteps += PSTEP;
+      double *t = realloc(pt->table, sizeof(double) * nc * pt->maxsteps);
+      if (!t) return ORC_ERR_NOMEM;
+      pt->table = t;
+    }
+    if (fscanf(f, "%lf", &r2) != 1) break;
+    if (npot == 0) r2_start = r2;
+    for (int i = 0; i < nc; i++) {
+      if (fscanf(f, "%lf", &val) != 1) return ORC_ERR_FORMAT;
+      pt->table[npot * nc + i] = val;
+      if (val != 0.0) {
+        pt->end[i] = r2;
+        pt->len[i] = npot + 1;
+      }
+    }
+    npot++;
+  }
+  if (npot < 2) return ORC_ERR_FORMAT;
+  double r2_step = (r2 - r2_start) / (npot - 1);
+  for (int i = 0; i < nc; i++) {
+    pt->begin[i] = r2_start;
+    pt->step[i] = r2_step;
+    pt->invstep[i] = 1.0 / r2_step;
+    double delta = pt->table[(npot - 1) * nc + i];
+    if (radial && delta != 0.0)
+      for (int k = 0; k < npot; k++) pt->table[k * nc + i] -= delta;
+    if (radial && pt->end[i] > *cellsz) *cellsz = pt->end[i];
+  }
+  double *t = realloc(pt->table, sizeof(double) * nc * (pt->maxsteps + 2));
+  if (!t) return ORC_ERR_NOMEM;
+  pt->table = t;
+  return ORC_OK;
+}
+
+/* format 2 (src/imd_potential.c:394-462): per column "begin end step",
+   then the values column after column. */
+static int read_format2(orc_table *pt, FILE *f, int radial, double *cellsz) {
+  int nc = pt->ncols;
+  pt->maxsteps = 0;
+  for (int i = 0; i < nc; i++) {
+    if (fscanf(f, "%lf %lf %lf", &pt->begin[i], &pt->end[i], &pt->step[i]) !=
+        3)
+      return ORC_ERR_FORMAT;
+    if (radial && pt->end[i] > *cellsz) *cellsz = pt->end[i];
+    pt->invstep[i] = 1.0 / pt->step[i];
+    double numstep = 1 + (pt->end[i] - pt->begin[i]) / pt->step[i];
+    pt->len[i] = (int)(numstep + 0.49);
+    if (pt->len[i] > pt->maxsteps) pt->maxsteps = pt->len[i];
+  }
+  pt->table = calloc((size_t)nc * (pt->maxsteps + 2), sizeof(double));
+  if (!pt->table) return ORC_ERR_NOMEM;
+  for (int i = 0; i < nc; i++)
+    for (int k = 0; k < pt->len[i]; k++) {
+      double val;
+      if (fscanf(f, "%lf", &val) != 1) return ORC_ERR_FORMAT;
+      pt->table[k * nc + i] = val;
+    }
+  if (radial)
+    for (int i = 0; i < nc; i++) {
+      double delta = pt->table[(pt->len[i] - 1) * nc + i];
+      if (delta != 0.0)
+        for (int k = 0; k < pt->len[i]; k++) pt->table[k * nc + i] -= delta;
+    }
+  return ORC_OK;
+}
+
+/* read_pot_table (src/imd_potential.c:161-282) */
+int orc_table_read(orc_table *pt, const char *path, int ncols, int radial,
+                   int default_format, double *cellsz) {
+  char buf[1024];
+  int have_header = 0, have_format = 0, end_header = 0;
+  int format = default_format, size = ncols, rc;
+  FILE *f = fopen(path, "r");
+  if (!f) return ORC_ERR_IO;
+  do {
+    if (!fgets(buf, sizeof buf, f)) {
+      fclose(f);
+      return ORC_ERR_FORMAT;
+    }
+    if (buf[0] == '#') {
+      have_header = 1;
+      end_header = (buf[1] == 'E');
+      if (buf[1] == 'F') {
+        if (sscanf(buf + 2, "%d%d", &format, &size) != 2 || size != ncols ||
+            (format != 1 && format != 2)) {
+          fclose(f);
+          return ORC_ERR_FORMAT;
+        }
+        have_format = 1;
+      }
+    } else if (have_header) {
+      fclose(f);
+      return ORC_ERR_FORMAT;
+    } else {
+      end_header = 1;
+    }
+  } while (!end_header);
+  if (have_header && !have_format) {
+    fclose(f);
+    return ORC_ERR_FORMAT;
+  }
+  if (!have_header) rewind(f);
+  if ((rc = table_alloc_info(pt, ncols)) != ORC_OK) {
+    fclose(f);
+    return rc;
+  }
+  rc = (format == 1) ? read_format1(pt, f, radial, cellsz)
+                     : read_format2(pt, f, radial, cellsz);
+  fclose(f);
+  if (rc == ORC_OK) rc = init_threepoint(pt);
+  if (rc != ORC_OK) orc_table_free(pt);
+  return rc;
+}
+
+/* Common argument handling of the three access macros: clamp r2 into
+   [begin, end], index, fraction, three consecutive rows. */
+static inline void tab_locate(const orc_table *pt, int col, int inc, double r2,
+                              int *is_short, double *chi, double *istep_out,
+                              double *p0, double *p1, double *p2) {
+  double r2a = (r2 < pt->end[col]) ? r2 : pt->end[col];
+  r2a = r2a - pt->begin[col];
+  if (r2a < 0) {
+    r2a = 0;
+    *is_short = 1;
+  }
+  double istep = pt->invstep[col];
+  r2a = r2a * istep;
+  int k = (int)r2a;
+  *chi = r2a - k;
+  const double *ptr = pt->table + (long)k * inc + col;
+  *p0 = ptr[0];
+  *p1 = ptr[inc];
+  *p2 = ptr[2 * inc];
+  *istep_out = istep;
+}
+
+void orc_pair_int2(const orc_table *pt, int col, int inc, double r2,
+                   double *pot, double *grad, int *is_short) {
+  double chi, istep, p0, p1, p2;
+  tab_locate(pt, col, inc, r2, is_short, &chi, &istep, &p0, &p1, &p2);
+  double dv = p1 - p0;
+  double d2v = p2 - 2 * p1 + p0;
+  *pot = p0 + chi * dv + 0.5 * chi * (chi - 1) * d2v;
+  *grad = 2 * istep * (dv + (chi - 0.5) * d2v);
+}
+
+double orc_val_func2(const orc_table *pt, int col, int inc, double r2,
+                     int *is_short) {
+  double chi, istep, p0, p1, p2;
+  tab_locate(pt, col, inc, r2, is_short, &chi, &istep, &p0, &p1, &p2);
+  double dv = p1 - p0;
+  double d2v = p2 - 2 * p1 + p0;
+  return p0 + chi * dv + 0.5 * chi * (chi - 1) * d2v;
+}
+
+double orc_deriv_func2(const orc_table *pt, int col, int inc, double r2,
+                       int *is_short) {
+  double chi, istep, p0, p1, p2;
+  tab_locate(pt, col, inc, r2, is_short, &chi, &istep, &p0, &p1, &p2);
+  double dv = p1 - p0;
+  double d2v = p2 - 2 * p1 + p0;
+  return 2 * istep * (dv + (chi - 0.5) * d2v);
+}
+
+/* ------------------------------------------------------------------------ */
+/* system                                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int n, cap;
+  int *a;
+} ivec;
+
+static int ivec_push(ivec *v, int x) {
+  if (v->n == v->cap) {
+    int nc = v->cap ? 2 * v->cap : 16;
+    int *a = realloc(v->a, sizeof(int) * nc);
+    if (!a) return ORC_ERR_NOMEM;
+    v->a = a;
+    v->cap = nc;
+  }
+  v->a[v->n++] = x;
+  return ORC_OK;
+}
+
+struct orc_sys {
+  /* geometry (src/imd_geom_3d.c:52-104) */
+  double box[3][3], tbox[3][3], height[3], volume;
+  int pbc[3];
+  double nbl_margin, cellsz; /* cellsz: squared list radius incl. skin */
+  int gdim[3], cdim[3], nall, ncells;
+  int ensemble, ntypes;
+  double timestep, temperature, eta, isq_tau_eta;
+  const orc_table *pair, *rho_t, *embed;
+  /* real atoms, caller order */
+  int n;
+  long *id;
+  int *type;
+  double *mass, *x, *p, *x_nbl;
+  int *cell3;
+  /* slots: 0..n-1 real atoms, then buffer copies in creation order */
+  int nslot, slotcap;
+  double *sx;   /* 3 per slot */
+  int *stype, *ssrc, *sroot, *sshift; /* sshift: 3 per slot */
+  double *svec; /* shift vector added when the slot was copied, 3 per slot */
+  double *sf, *sepot, *srho, *sdf;
+  /* cells */
+  ivec *cells;
+  int *inner;      /* inner cell indices, IMD order */
+  int (*nq)[14];   /* stencil per inner cell */
+  /* half list (src/imd_forces_nbl.c:25-43) */
+  int *tl, *tb, *tl_atom;
+  long tbn, tbcap;
+  int have_valid_nbl, nbl_count, is_short;
+  double tot_pot_energy, virial, tot_kin_energy;
+};
+
+static void vec_prod(const double *u, const double *v, double *w) {
+  w[0] = u[1] * v[2] - u[2] * v[1];
+  w[1] = u[2] * v[0] - u[0] * v[2];
+  w[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+static double sprod(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+orc_sys *orc_new(int ntypes, const double *box9, const int *pbc3,
+                 double nbl_margin, double cellsz_tab, int ensemble,
+                 double timestep) {
+  orc_sys *s = calloc(1, sizeof(orc_sys));
+  if (!s) return NULL;
+  s->ntypes = ntypes;
+  for (int d = 0; d < 3; d++) {
+    for (int e = 0; e < 3; e++) s->box[d][e] = box9[3 * d + e];
+    s->pbc[d] = pbc3[d];
+  }
+  s->nbl_margin = nbl_margin;
+  s->ensemble = ensemble;
+  s->timestep = timestep;
+  /* make_box: dual vectors, volume, heights */
+  vec_prod(s->box[1], s->box[2], s->tbox[0]);
+  vec_prod(s->box[2], s->box[0], s->tbox[1]);
+  vec_prod(s->box[0], s->box[1], s->tbox[2]);
+  s->volume = sprod(s->box[0], s->tbox[0]);
+  if (s->volume == 0) {
+    free(s);
+    return NULL;
+  }
+  for (int d = 0; d < 3; d++)
+    for (int e = 0; e < 3; e++) s->tbox[d][e] /= s->volume;
+  for (int d = 0; d < 3; d++) s->height[d] = 1.0 / sprod(s->tbox[d], s->tbox[d]);
+  if (s->volume < 0) s->volume = -s->volume;
+  /* init_cells: add the skin once, size the grid (tol = 1, cpu_dim = 1) */
+  s->cellsz = sqrt(cellsz_tab) + nbl_margin;
+  s->cellsz = s->cellsz * s->cellsz;
+  for (int d = 0; d < 3; d++) {
+    double cs = sqrt(s->cellsz / s->height[d]);
+    s->gdim[d] = (int)(1.0 / cs);
+    if (s->gdim[d] < 1) {
+      free(s);
+      return NULL;
+    }
+    s->cdim[d] = s->gdim[d] + 2;
+  }
+  s->nall = s->cdim[0] * s->cdim[1] * s->cdim[2];
+  s->ncells = s->gdim[0] * s->gdim[1] * s->gdim[2];
+  s->cells = calloc(s->nall, sizeof(ivec));
+  s->inner = malloc(sizeof(int) * s->ncells);
+  s->nq = malloc(sizeof(int[14]) * s->ncells);
+  if (!s->cells || !s->inner || !s->nq) {
+    orc_free(s);
+    return NULL;
+  }
+  /* make_cell_lists, NBL version (src/imd_geom_3d.c:823-949) */
+  int l = 0;
+  for (int i = 1; i < s->cdim[0] - 1; i++)
+    for (int j = 1; j < s->cdim[1] - 1; j++)
+      for (int k = 1; k < s->cdim[2] - 1; k++) {
+        int np = (i * s->cdim[1] + j) * s->cdim[2] + k;
+        s->inner[l] = np;
+        s->nq[l][0] = np;
+        int nn = 1;
+        for (int a = 0; a <= 1; a++)
+          for (int b = -a; b <= 1; b++)
+            for (int c = (a == 0 ? -b : -a); c <= 1; c++) {
+              int r = i + a, t = j + b, u = k + c;
+              int qq = (r * s->cdim[1] + t) * s->cdim[2] + u;
+              if (qq == np) continue;
+              int g[3] = {r - 1, t - 1, u - 1}, ok = 1;
+              for (int d = 0; d < 3; d++) {
+                int ip = (g[d] < 0) ? -1 : (g[d] > s->gdim[d] - 1 ? 1 : 0);
+                if (!(s->pbc[d] == 1 || ip == 0)) ok = 0;
+              }
+              s->nq[l][nn++] = ok ? qq : -1;
+            }
+        l++;
+      }
+  return s;
+}
+
+void orc_free(orc_sys *s) {
+  if (!s) return;
+  if (s->cells)
+    for (int c = 0; c < s->nall; c++) free(s->cells[c].a);
+  free(s->cells);
+  free(s->inner);
+  free(s->nq);
+  free(s->id);
+  free(s->type);
+  free(s->mass);
+  free(s->x);
+  free(s->p);
+  free(s->x_nbl);
+  free(s->cell3);
+  free(s->sx);
+  free(s->stype);
+  free(s->ssrc);
+  free(s->sroot);
+  free(s->sshift);
+  free(s->svec);
+  free(s->sf);
+  free(s->sepot);
+  free(s->srho);
+  free(s->sdf);
+  free(s->tl);
+  free(s->tb);
+  free(s->tl_atom);
+  free(s);
+}
+
+int orc_set_tables(orc_sys *s, const orc_table *pair, const orc_table *rho,
+                   const orc_table *embed) {
+  if ((rho == NULL) != (embed == NULL)) return ORC_ERR_ARG;
+  int nt2 = s->ntypes * s->ntypes;
+  if (pair && pair->ncols != nt2) return ORC_ERR_ARG;
+  if (rho && rho->ncols != nt2) return ORC_ERR_ARG;
+  if (embed && embed->ncols != s->ntypes) return ORC_ERR_ARG;
+  s->pair = pair;
+  s->rho_t = rho;
+  s->embed = embed;
+  return ORC_OK;
+}
+
+void orc_set_nvt(orc_sys *s, double temperature, double eta,
+                 double isq_tau_eta) {
+  s->temperature = temperature;
+  s->eta = eta;
+  s->isq_tau_eta = isq_tau_eta;
+}
+
+/* back_into_box (src/imd_geom_3d.c:1083-1110) */
+static void back_into_box(const orc_sys *s, double *pos) {
+  for (int d = 0; d < 3; d++) {
+    if (s->pbc[d] != 1) continue;
+    double i = floor(sprod(pos, s->tbox[d]));
+    pos[0] -= i * s->box[d][0];
+    pos[1] -= i * s->box[d][1];
+    pos[2] -= i * s->box[d][2];
+  }
+}
+
+int orc_set_atoms(orc_sys *s, int n, const long *id, const int *type,
+                  const double *mass, const double *x, const double *p) {
+  s->n = n;
+  s->id = malloc(sizeof(long) * n);
+  s->type = malloc(sizeof(int) * n);
+  s->mass = malloc(sizeof(double) * n);
+  s->x = malloc(sizeof(double) * 3 * n);
+  s->p = malloc(sizeof(double) * 3 * n);
+  s->x_nbl = malloc(sizeof(double) * 3 * n);
+  s->cell3 = malloc(sizeof(int) * 3 * n);
+  if (!s->id || !s->type || !s->mass || !s->x || !s->p || !s->x_nbl ||
+      !s->cell3)
+    return ORC_ERR_NOMEM;
+  for (int i = 0; i < n; i++) {
+    if (type[i] < 0 || type[i] >= s->ntypes || !(mass[i] > 0))
+      return ORC_ERR_ARG;
+    s->id[i] = id[i];
+    s->type[i] = type[i];
+    s->mass[i] = mass[i];
+    double pos[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+    back_into_box(s, pos);
+    for (int d = 0; d < 3; d++) {
+      s->x[3 * i + d] = pos[d];
+      s->p[3 * i + d] = p ? p[3 * i + d] : 0.0;
+    }
+  }
+  s->have_valid_nbl = 0;
+  return ORC_OK;
+}
+
+/* cell_coord (src/imd_geom_3d.c:1054-1074) */
+static void cell_coord(const orc_sys *s, const double *x, int *c) {
+  for (int d = 0; d < 3; d++) {
+    c[d] = (int)(s->gdim[d] * (x[0] * s->tbox[d][0] + x[1] * s->tbox[d][1] +
+                               x[2] * s->tbox[d][2]));
+    if (c[d] >= s->gdim[d])
+      c[d] = s->gdim[d] - 1;
+    else if (c[d] < 0)
+      c[d] = 0;
+  }
+}
+
+/* fix_cells + do_boundaries: wrap in x, then y, then z, and re-bin every
+   atom (src/imd_fix_cells_3d.c:36-201, src/imd_main_3d.c:1972-2065).
+   Within a cell atoms are kept in caller order. */
+static void fix_cells(orc_sys *s) {
+  for (int i = 0; i < s->n; i++) {
+    double *x = s->x + 3 * i;
+    for (int d = 0; d < 3; d++) {
+      if (s->pbc[d] != 1) continue;
+      double f = -floor(x[0] * s->tbox[d][0] + x[1] * s->tbox[d][1] +
+                        x[2] * s->tbox[d][2]);
+      x[0] += f * s->box[d][0];
+      x[1] += f * s->box[d][1];
+      x[2] += f * s->box[d][2];
+    }
+  }
+  for (int c = 0; c < s->nall; c++) s->cells[c].n = 0;
+  for (int i = 0; i < s->n; i++) {
+    int *c = s->cell3 + 3 * i;
+    cell_coord(s, s->x + 3 * i, c);
+    int q = ((c[0] + 1) * s->cdim[1] + (c[1] + 1)) * s->cdim[2] + (c[2] + 1);
+    ivec_push(&s->cells[q], i);
+  }
+  s->have_valid_nbl = 0;
+}
+
+static int slot_reserve(orc_sys *s, int need) {
+  if (need <= s->slotcap) return ORC_OK;
+  int nc = need + need / 2 + 64;
+#define GROW(ptr, T, m)                                   \
+  do {                                                    \
+    T *q_ = realloc(ptr, sizeof(T) * (size_t)(m) * nc);   \
+    if (!q_) return ORC_ERR_NOMEM;                        \
+    ptr = q_;                                             \
+  } while (0)
+  GROW(s->sx, double, 3);
+  GROW(s->stype, int, 1);
+  GROW(s->ssrc, int, 1);
+  GROW(s->sroot, int, 1);
+  GROW(s->sshift, int, 3);
+  GROW(s->svec, double, 3);
+  GROW(s->sf, double, 3);
+  GROW(s->sepot, double, 1);
+  GROW(s->srho, double, 1);
+  GROW(s->sdf, double, 1);
+#undef GROW
+  s->slotcap = nc;
+  return ORC_OK;
+}
+
+static int cidx(const orc_sys *s, int i, int j, int k) {
+  return (i * s->cdim[1] + j) * s->cdim[2] + k;
+}
+
+/* copy_cell (src/imd_comm_force_3d.c:726-782): the destination buffer cell
+   receives shifted copies; dir/sgn record the integer image shift. */
+static int copy_cell(orc_sys *s, int from, int to, const double *v, int dir,
+                     int sgn) {
+  ivec *f = &s->cells[from], *t = &s->cells[to];
+  t->n = 0;
+  for (int m = 0; m < f->n; m++) {
+    int src = f->a[m];
+    if (slot_reserve(s, s->nslot + 1) != ORC_OK) return ORC_ERR_NOMEM;
+    int q = s->nslot++;
+    s->ssrc[q] = src;
+    s->sroot[q] = s->sroot[src];
+    s->stype[q] = s->stype[src];
+    for (int d = 0; d < 3; d++) {
+      s->svec[3 * q + d] = v[d];
+      s->sshift[3 * q + d] = s->sshift[3 * src + d];
+      s->sx[3 * q + d] = s->sx[3 * src + d] + v[d];
+    }
+    s->sshift[3 * q + dir] += sgn;
+    if (ivec_push(t, q) != ORC_OK) return ORC_ERR_NOMEM;
+  }
+  return ORC_OK;
+}
+
+/* send_cells, serial branch (src/imd_comm_force_3d.c:222-397): z faces both
+   ways, y faces both ways, x only into the high buffer (AR). */
+static int build_buffers(orc_sys *s) {
+  if (slot_reserve(s, s->n) != ORC_OK) return ORC_ERR_NOMEM;
+  s->nslot = s->n;
+  for (int i = 0; i < s->n; i++) {
+    s->ssrc[i] = -1;
+    s->sroot[i] = i;
+    s->stype[i] = s->type[i];
+    for (int d = 0; d < 3; d++) {
+      s->sx[3 * i + d] = s->x[3 * i + d];
+      s->sshift[3 * i + d] = 0;
+      s->svec[3 * i + d] = 0.0;
+    }
+  }
+  double up[3] = {0, 0, 0}, dn[3] = {0, 0, 0}, no[3] = {0, 0, 0},
+         so[3] = {0, 0, 0}, ea[3] = {0, 0, 0};
+  for (int d = 0; d < 3; d++) {
+    if (s->pbc[2] == 1) {
+      up[d] = s->box[2][d];
+      dn[d] = -s->box[2][d];
+    }
+    if (s->pbc[1] == 1) {
+      no[d] = s->box[1][d];
+      so[d] = -s->box[1][d];
+    }
+    if (s->pbc[0] == 1) ea[d] = s->box[0][d];
+  }
+  int cx = s->cdim[0], cy = s->cdim[1], cz = s->cdim[2], rc;
+  int zs = s->pbc[2] == 1, ys = s->pbc[1] == 1, xs = s->pbc[0] == 1;
+  for (int i = 1; i < cx - 1; i++)
+    for (int j = 1; j < cy - 1; j++) {
+      if ((rc = copy_cell(s, cidx(s, i, j, 1), cidx(s, i, j, cz - 1), up, 2,
+                          zs)))
+        return rc;
+      if ((rc = copy_cell(s, cidx(s, i, j, cz - 2), cidx(s, i, j, 0), dn, 2,
+                          -zs)))
+        return rc;
+    }
+  for (int i = 1; i < cx - 1; i++)
+    for (int k = 0; k < cz; k++) {
+      if ((rc = copy_cell(s, cidx(s, i, 1, k), cidx(s, i, cy - 1, k), no, 1,
+                          ys)))
+        return rc;
+      if ((rc = copy_cell(s, cidx(s, i, cy - 2, k), cidx(s, i, 0, k), so, 1,
+                          -ys)))
+        return rc;
+    }
+  for (int j = 0; j < cy; j++)
+    for (int k = 0; k < cz; k++)
+      if ((rc = copy_cell(s, cidx(s, 1, j, k), cidx(s, cx - 1, j, k), ea, 0,
+                          xs)))
+        return rc;
+  return ORC_OK;
+}
+
+/* refresh buffer positions in creation order (every calc_forces) */
+static void refresh_buffers(orc_sys *s) {
+  for (int i = 0; i < s->n; i++)
+    for (int d = 0; d < 3; d++) s->sx[3 * i + d] = s->x[3 * i + d];
+  for (int q = s->n; q < s->nslot; q++) {
+    int src = s->ssrc[q];
+    for (int d = 0; d < 3; d++)
+      s->sx[3 * q + d] = s->sx[3 * src + d] + s->svec[3 * q + d];
+  }
+}
+
+/* make_nblist (src/imd_forces_nbl.c:136-273) */
+static int make_nblist(orc_sys *s) {
+  for (int i = 0; i < 3 * s->n; i++) s->x_nbl[i] = s->x[i];
+  free(s->tl);
+  free(s->tl_atom);
+  s->tl = malloc(sizeof(int) * (s->n + 1));
+  s->tl_atom = malloc(sizeof(int) * (s->n + 1));
+  if (!s->tl || !s->tl_atom) return ORC_ERR_NOMEM;
+  long tn = 0;
+  int n = 0;
+  s->tl[0] = 0;
+  for (int c = 0; c < s->ncells; c++) {
+    int c1 = s->inner[c];
+    ivec *p = &s->cells[c1];
+    for (int i = 0; i < p->n; i++) {
+      int ia = p->a[i];
+      double d1[3] = {s->sx[3 * ia], s->sx[3 * ia + 1], s->sx[3 * ia + 2]};
+      for (int m = 0; m < 14; m++) {
+        int c2 = s->nq[c][m];
+        if (c2 < 0) continue;
+        int jstart = (c2 == c1) ? i + 1 : 0;
+        ivec *q = &s->cells[c2];
+        for (int j = jstart; j < q->n; j++) {
+          int js = q->a[j];
+          double d[3] = {s->sx[3 * js] - d1[0], s->sx[3 * js + 1] - d1[1],
+                         s->sx[3 * js + 2] - d1[2]};
+          double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          if (r2 < s->cellsz) {
+            if (tn == s->tbcap) {
+              long nc = s->tbcap ? 2 * s->tbcap : 1 << 16;
+              int *t = realloc(s->tb, sizeof(int) * nc);
+              if (!t) return ORC_ERR_NOMEM;
+              s->tb = t;
+              s->tbcap = nc;
+            }
+            s->tb[tn++] = js;
+          }
+        }
+      }
+      s->tl_atom[n] = ia;
+      s->tl[++n] = (int)tn;
+    }
+  }
+  s->tbn = tn;
+  s->have_valid_nbl = 1;
+  s->nbl_count++;
+  return ORC_OK;
+}
+
+/* calc_forces, PAIR + EAM2 branches (src/imd_forces_nbl.c:281-1323) */
+int orc_calc_forces(orc_sys *s, int steps) {
+  (void)steps;
+  int rc, nt = s->ntypes, inc = nt * nt;
+  if (!s->have_valid_nbl) {
+    fix_cells(s);
+    if ((rc = build_buffers(s))) return rc;
+    if ((rc = make_nblist(s))) return rc;
+  } else {
+    refresh_buffers(s);
+  }
+  s->tot_pot_energy = 0.0;
+  s->virial = 0.0;
+  for (int q = 0; q < s->nslot; q++) {
+    s->sf[3 * q] = s->sf[3 * q + 1] = s->sf[3 * q + 2] = 0.0;
+    s->sepot[q] = 0.0;
+    s->srho[q] = 0.0;
+  }
+  const orc_table *pp = s->pair, *rt = s->rho_t, *et = s->embed;
+  int is_short = 0;
+  /* loop A: pair forces and host electron density */
+  for (int n = 0; n < s->n; n++) {
+    int ia = s->tl_atom[n];
+    const double *d1 = s->sx + 3 * ia;
+    double ff[3] = {0, 0, 0}, ee = 0.0, eam_r = 0.0;
+    int it = s->stype[ia];
+    for (int m = s->tl[n]; m < s->tl[n + 1]; m++) {
+      int js = s->tb[m];
+      double d[3] = {s->sx[3 * js] - d1[0], s->sx[3 * js + 1] - d1[1],
+                     s->sx[3 * js + 2] - d1[2]};
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      int jt = s->stype[js], col = it * nt + jt, col2 = jt * nt + it;
+      if (pp && r2 <= pp->end[col]) {
+        double pot, grad;
+        orc_pair_int2(pp, col, inc, r2, &pot, &grad, &is_short);
+        s->tot_pot_energy += pot;
+        double fc[3] = {d[0] * grad, d[1] * grad, d[2] * grad};
+        s->sf[3 * js] -= fc[0];
+        s->sf[3 * js + 1] -= fc[1];
+        s->sf[3 * js + 2] -= fc[2];
+        ff[0] += fc[0];
+        ff[1] += fc[1];
+        ff[2] += fc[2];
+        pot *= 0.5;
+        ee += pot;
+        s->sepot[js] += pot;
+        s->virial -= r2 * grad;
+      }
+      if (rt) {
+        double rho_h = 0.0;
+        if (r2 < rt->end[col]) {
+          rho_h = orc_val_func2(rt, col, inc, r2, &is_short);
+          eam_r += rho_h;
+        }
+        if (it == jt) {
+          if (r2 < rt->end[col]) s->srho[js] += rho_h;
+        } else if (r2 < rt->end[col2]) {
+          rho_h = orc_val_func2(rt, col2, inc, r2, &is_short);
+          s->srho[js] += rho_h;
+        }
+      }
+    }
+    s->sf[3 * ia] += ff[0];
+    s->sf[3 * ia + 1] += ff[1];
+    s->sf[3 * ia + 2] += ff[2];
+    s->sepot[ia] += ee;
+    s->srho[ia] += eam_r;
+  }
+  if (rt) {
+    /* add_rho: buffer copies back to their sources (x, then y, then z) */
+    for (int q = s->nslot - 1; q >= s->n; q--) s->srho[s->ssrc[q]] += s->srho[q];
+    /* embedding energy and 2F' (src/imd_forces_nbl.c:1079-1112) */
+    int idummy = 0;
+    for (int c = 0; c < s->ncells; c++) {
+      ivec *p = &s->cells[s->inner[c]];
+      for (int i = 0; i < p->n; i++) {
+        int ia = p->a[i];
+        double pot, df;
+        orc_pair_int2(et, s->stype[ia], nt, s->srho[ia], &pot, &df, &idummy);
+        s->sdf[ia] = df;
+        s->sepot[ia] += pot;
+        s->tot_pot_energy += pot;
+      }
+    }
+    /* copy_dF into buffer cells (creation order) */
+    for (int q = s->n; q < s->nslot; q++) s->sdf[q] = s->sdf[s->ssrc[q]];
+    /* loop B: EAM forces */
+    for (int n = 0; n < s->n; n++) {
+      int ia = s->tl_atom[n];
+      const double *d1 = s->sx + 3 * ia;
+      double ff[3] = {0, 0, 0};
+      int it = s->stype[ia];
+      for (int m = s->tl[n]; m < s->tl[n + 1]; m++) {
+        int js = s->tb[m];
+        double d[3] = {s->sx[3 * js] - d1[0], s->sx[3 * js + 1] - d1[1],
+                       s->sx[3 * js + 2] - d1[2]};
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        int jt = s->stype[js], col1 = jt * nt + it, col2 = it * nt + jt;
+        if (r2 < rt->end[col1] || r2 < rt->end[col2]) {
+          double rho_i_s = orc_deriv_func2(rt, col1, inc, r2, &is_short);
+          double rho_j_s = (col1 == col2)
+                               ? rho_i_s
+                               : orc_deriv_func2(rt, col2, inc, r2, &is_short);
+          double grad = 0.5 * (s->sdf[ia] * rho_j_s + s->sdf[js] * rho_i_s);
+          double fc[3] = {d[0] * grad, d[1] * grad, d[2] * grad};
+          s->sf[3 * js] -= fc[0];
+          s->sf[3 * js + 1] -= fc[1];
+          s->sf[3 * js + 2] -= fc[2];
+          ff[0] += fc[0];
+          ff[1] += fc[1];
+          ff[2] += fc[2];
+          s->virial -= d[0] * fc[0] + d[1] * fc[1] + d[2] * fc[2];
+        }
+      }
+      s->sf[3 * ia] += ff[0];
+      s->sf[3 * ia + 1] += ff[1];
+      s->sf[3 * ia + 2] += ff[2];
+    }
+  }
+  /* add_forces: buffer copies back to their sources (x, then y, then z) */
+  for (int q = s->nslot - 1; q >= s->n; q--) {
+    int src = s->ssrc[q];
+    s->sf[3 * src] += s->sf[3 * q];
+    s->sf[3 * src + 1] += s->sf[3 * q + 1];
+    s->sf[3 * src + 2] += s->sf[3 * q + 2];
+    s->sepot[src] += s->sepot[q];
+  }
+  s->is_short |= is_short;
+  return ORC_OK;
+}
+
+/* move_atoms_nve (src/imd_integrate.c:32-497) and move_atoms_nvt
+   (src/imd_integrate.c:891-1147), without restrictions (all ones). */
+void orc_move_atoms(orc_sys *s) {
+  double dt = s->timestep;
+  if (s->ensemble == ORC_NVE) {
+    s->tot_kin_energy = 0.0;
+    for (int c = 0; c < s->ncells; c++) {
+      ivec *cl = &s->cells[s->inner[c]];
+      for (int a = 0; a < cl->n; a++) {
+        int i = cl->a[a];
+        double *p = s->p + 3 * i, *x = s->x + 3 * i, *f = s->sf + 3 * i;
+        double k1 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        p[0] += dt * f[0];
+        p[1] += dt * f[1];
+        p[2] += dt * f[2];
+        double k2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        s->tot_kin_energy += (k1 + k2) / (4 * s->mass[i]);
+        double tmp = dt / s->mass[i];
+        x[0] += tmp * p[0];
+        x[1] += tmp * p[1];
+        x[2] += tmp * p[2];
+      }
+    }
+  } else {
+    double e1 = 0.0, e2 = 0.0;
+    double reibung = 1.0 - s->eta * dt / 2.0;
+    double eins_d_reib = 1.0 / (1.0 + s->eta * dt / 2.0);
+    for (int c = 0; c < s->ncells; c++) {
+      ivec *cl = &s->cells[s->inner[c]];
+      for (int a = 0; a < cl->n; a++) {
+        int i = cl->a[a];
+        double *p = s->p + 3 * i, *x = s->x + 3 * i, *f = s->sf + 3 * i;
+        e1 += (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / s->mass[i];
+        p[0] = (p[0] * reibung + dt * f[0]) * eins_d_reib;
+        p[1] = (p[1] * reibung + dt * f[1]) * eins_d_reib;
+        p[2] = (p[2] * reibung + dt * f[2]) * eins_d_reib;
+        e2 += (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / s->mass[i];
+        double tmp = dt / s->mass[i];
+        x[0] += tmp * p[0];
+        x[1] += tmp * p[1];
+        x[2] += tmp * p[2];
+      }
+    }
+    s->tot_kin_energy = (e1 + e2) / 4.0;
+    double ttt = 3.0 * s->n * s->temperature;
+    s->eta += dt * (e2 / ttt - 1.0) * s->isq_tau_eta;
+  }
+}
+
+/* check_nblist (src/imd_forces_nbl.c:2007-2037) */
+void orc_check_nblist(orc_sys *s) {
+  double max1 = 0.0;
+  for (int i = 0; i < s->n; i++) {
+    double d[3] = {s->x[3 * i] - s->x_nbl[3 * i],
+                   s->x[3 * i + 1] - s->x_nbl[3 * i + 1],
+                   s->x[3 * i + 2] - s->x_nbl[3 * i + 2]};
+    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (r2 > max1) max1 = r2;
+  }
+  double h = 0.5 * s->nbl_margin;
+  if (max1 > h * h) s->have_valid_nbl = 0;
+}
+
+void orc_invalidate_nblist(orc_sys *s) { s->have_valid_nbl = 0; }
+
+int orc_run(orc_sys *s, int nsteps, double *thermo) {
+  for (int st = 0; st < nsteps; st++) {
+    int before = s->nbl_count, rc;
+    if ((rc = orc_calc_forces(s, st))) return rc;
+    orc_move_atoms(s);
+    orc_check_nblist(s);
+    if (thermo) {
+      double *t = thermo + 5 * st;
+      t[0] = s->tot_pot_energy;
+      t[1] = s->tot_kin_energy;
+      t[2] = s->virial;
+      t[3] = s->eta;
+      t[4] = (double)(s->nbl_count - before);
+    }
+  }
+  return ORC_OK;
+}
+
+void orc_get_atoms(const orc_sys *s, double *x, double *p, double *f,
+                   double *epot, double *rho, double *df) {
+  for (int i = 0; i < s->n; i++) {
+    for (int d = 0; d < 3; d++) {
+      if (x) x[3 * i + d] = s->x[3 * i + d];
+      if (p) p[3 * i + d] = s->p[3 * i + d];
+      if (f) f[3 * i + d] = s->sf ? s->sf[3 * i + d] : 0.0;
+    }
+    if (epot) epot[i] = s->sepot ? s->sepot[i] : 0.0;
+    if (rho) rho[i] = s->srho ? s->srho[i] : 0.0;
+    if (df) df[i] = s->sdf ? s->sdf[i] : 0.0;
+  }
+}
+
+double orc_get_scalar(const orc_sys *s, int which) {
+  switch (which) {
+    case 0: return s->tot_pot_energy;
+    case 1: return s->virial;
+    case 2: return s->tot_kin_energy;
+    case 3: return s->eta;
+    case 4: return s->cellsz;
+    case 5: return s->volume;
+    case 6: return s->is_short;
+    case 7: return s->nbl_count;
+    case 8: return s->n;
+    default: return 0.0;
+  }
+}
+
+void orc_get_grid(const orc_sys *s, int *gdim3) {
+  for (int d = 0; d < 3; d++) gdim3[d] = s->gdim[d];
+}
+
+void orc_get_cells(const orc_sys *s, int *cell3) {
+  memcpy(cell3, s->cell3, sizeof(int) * 3 * s->n);
+}
+
+long orc_get_pairs(const orc_sys *s, long *out, long cap) {
+  long np = 0;
+  if (!s->have_valid_nbl && s->nbl_count == 0) return 0;
+  for (int n = 0; n < s->n; n++) {
+    int ia = s->tl_atom[n];
+    for (int m = s->tl[n]; m < s->tl[n + 1]; m++) {
+      int js = s->tb[m], jr = s->sroot[js];
+      const int *sh = s->sshift + 3 * js;
+      if (np < cap) {
+        long *o = out + 5 * np;
+        if (s->id[ia] < s->id[jr]) {
+          o[0] = s->id[ia];
+          o[1] = s->id[jr];
+          o[2] = sh[0];
+          o[3] = sh[1];
+          o[4] = sh[2];
+        } else {
+          o[0] = s->id[jr];
+          o[1] = s->id[ia];
+          o[2] = -sh[0];
+          o[3] = -sh[1];
+          o[4] = -sh[2];
+        }
+      }
+      np++;
+    }
+  }
+  return np;
+}
