@@ -24,9 +24,11 @@ void launch_force(const StepArgs<R>&, const TabDev<R>*, bool, bool, int,
                   hipStream_t);
 template <typename R>
 void launch_move(const StepArgs<R>&, int, hipStream_t);
+template <typename R>
+void launch_copyrec(const StepArgs<R>&, int, hipStream_t);
 void launch_build(const BuildArgs&, int, hipStream_t);
-void launch_check(const RepDev*, RepState*, Ctl*, const int*, double4* const*,
-                  const double4*, double*, int, int, hipStream_t);
+void launch_check(const RepDev*, Ctl*, const int*, const double4*,
+                  const double4*, const double4*, double*, int, hipStream_t);
 }  // namespace imdg
 
 using namespace imdg;
@@ -82,16 +84,18 @@ struct imdg_ctx {
   double4* d_xnbl = nullptr;
   int *d_cellid = nullptr, *d_cslot = nullptr, *d_acell = nullptr,
       *d_perm = nullptr;
+  signed char* d_atype = nullptr;
   int *d_cell_cnt = nullptr, *d_cell_beg = nullptr, *d_cell_num = nullptr;
+  Tile* d_tiles = nullptr;
   uint32_t* d_nbr = nullptr;
   size_t nbr_bytes = 0;
   int* d_nnb = nullptr;
-  void *d_rho = nullptr, *d_df = nullptr, *d_epot = nullptr, *d_frc = nullptr;
+  void *d_reca = nullptr, *d_recb = nullptr;
+  void *d_rho = nullptr, *d_epot = nullptr, *d_frc = nullptr;
   double *d_partA = nullptr, *d_partB = nullptr, *d_maxd = nullptr;
   ThermoRec* d_ring = nullptr;
   void* d_tabs = nullptr;        // TabDev<R>[3]
   void* d_seg = nullptr;         // segments of all tables
-  double4** d_posp = nullptr;    // device copy of {d_pos[0], d_pos[1]}
   bool atoms_set = false;
   long target_steps = 0;
   // profiling
@@ -133,6 +137,15 @@ int dalloc(imdg_ctx* c, void** p, size_t bytes) {
   return IMDG_OK;
 }
 
+void dfree(imdg_ctx* c, void* p) {
+  for (auto it = c->bufs.begin(); it != c->bufs.end(); ++it)
+    if (it->p == p) {
+      hipFree(p);
+      c->bufs.erase(it);
+      return;
+    }
+}
+
 void free_all(imdg_ctx* c) {
   for (auto& b : c->bufs) hipFree(b.p);
   c->bufs.clear();
@@ -163,7 +176,6 @@ int copy_table(imdg_ctx* c, HostTable& h, const imdg_table* t, int ncols,
   h.maxsteps = t->maxsteps;
   h.begin.assign(t->begin, t->begin + ncols);
   h.end.assign(t->end, t->end + ncols);
-  h.step.assign(t->step ? t->step : t->begin, (t->step ? t->step : t->begin) + ncols);
   h.invstep.assign(t->invstep, t->invstep + ncols);
   h.len.assign(t->len, t->len + ncols);
   for (int k = 0; k < ncols; k++)
@@ -212,7 +224,6 @@ int upload_tables(imdg_ctx* c) {
   if (rc) return rc;
   HIPCHK(c, hipMemcpy(c->d_seg, segs.data(), segs.size() * sizeof(Seg<R>),
                       hipMemcpyHostToDevice));
-  for (int t = 0; t < 3; t++) td[t].seg = (const Seg<R>*)c->d_seg;
   rc = dalloc(c, &c->d_tabs, sizeof td);
   if (rc) return rc;
   HIPCHK(c, hipMemcpy(c->d_tabs, td, sizeof td, hipMemcpyHostToDevice));
@@ -221,6 +232,7 @@ int upload_tables(imdg_ctx* c) {
 
 template <typename R>
 StepArgs<R> step_args(imdg_ctx* c) {
+  typedef typename V4<R>::T R4;
   StepArgs<R> a;
   memset(&a, 0, sizeof a);
   a.rep = c->d_rep;
@@ -230,16 +242,19 @@ StepArgs<R> step_args(imdg_ctx* c) {
   for (int k = 0; k < 2; k++) {
     a.pos[k] = c->d_pos[k];
     a.mom[k] = c->d_mom[k];
-    a.aid[k] = c->d_aid[k];
   }
   a.xnbl = c->d_xnbl;
+  a.acell = c->d_acell;
+  a.atype = c->d_atype;
+  a.rec_a = (R4*)c->d_reca;
+  a.rec_b = (R4*)c->d_recb;
+  a.tiles = c->d_tiles;
   a.nbr = c->d_nbr;
   a.nnb = c->d_nnb;
   a.seg = (const Seg<R>*)c->d_seg;
   a.rho = (R*)c->d_rho;
-  a.df = (R*)c->d_df;
   a.epot = (R*)c->d_epot;
-  a.frc = (R*)c->d_frc;
+  a.frc = (R4*)c->d_frc;
   a.partA = c->d_partA;
   a.partB = c->d_partB;
   a.ring = c->d_ring;
@@ -269,17 +284,22 @@ BuildArgs build_args(imdg_ctx* c) {
     b.aid[k] = c->d_aid[k];
   }
   b.xnbl = c->d_xnbl;
+  b.rec_a = c->d_reca;
+  b.mixed = c->mixed ? 1 : 0;
   b.cellid = c->d_cellid;
   b.cslot = c->d_cslot;
   b.acell = c->d_acell;
+  b.atype = c->d_atype;
   b.perm = c->d_perm;
   b.cell_cnt = c->d_cell_cnt;
   b.cell_beg = c->d_cell_beg;
   b.cell_num = c->d_cell_num;
+  b.tiles = c->d_tiles;
   b.nbr = c->d_nbr;
   b.nnb = c->d_nnb;
   b.nrep = c->nrep;
   b.nslots = c->nslots;
+  b.nblk = c->nblk;
   b.ncells = c->ncells;
   b.kmax = c->kmax;
   return b;
@@ -330,7 +350,7 @@ void collect_profile(imdg_ctx* c) {
 }
 
 int grid_build(imdg_ctx* c) {
-  int g = (c->nslots + IMDG_BLOCK - 1) / IMDG_BLOCK;
+  int g = (c->nslots + 255) / 256;
   return std::max(1, std::min(g, 2048));
 }
 
@@ -341,27 +361,29 @@ int enqueue_build(imdg_ctx* c) {
   return IMDG_OK;
 }
 
-int enqueue_forces(imdg_ctx* c, bool integrate) {
+template <typename R>
+void enqueue_forces_t(imdg_ctx* c, bool integrate) {
+  StepArgs<R> a = step_args<R>(c);
+  const TabDev<R>* t = (const TabDev<R>*)c->d_tabs;
   bool nt1 = c->cfg.ntypes == 1;
-  if (c->mixed) {
-    StepArgs<float> a = step_args<float>(c);
-    const TabDev<float>* t = (const TabDev<float>*)c->d_tabs;
-    if (a.has_eam) {
-      ProfScope ps(c, KC_DENSITY);
-      launch_density<float>(a, t, nt1, c->nblk, c->stream);
-    }
-    ProfScope ps(c, KC_FORCE);
-    launch_force<float>(a, t, nt1, integrate, c->nblk, c->stream);
-  } else {
-    StepArgs<double> a = step_args<double>(c);
-    const TabDev<double>* t = (const TabDev<double>*)c->d_tabs;
-    if (a.has_eam) {
-      ProfScope ps(c, KC_DENSITY);
-      launch_density<double>(a, t, nt1, c->nblk, c->stream);
-    }
-    ProfScope ps(c, KC_FORCE);
-    launch_force<double>(a, t, nt1, integrate, c->nblk, c->stream);
+  {
+    // the force loop gathers (u, 2F') records: from the density loop with
+    // EAM, a copy of the (u, type) records without
+    ProfScope ps(c, KC_DENSITY);
+    if (a.has_eam)
+      launch_density<R>(a, t, nt1, c->nblk, c->stream);
+    else
+      launch_copyrec<R>(a, c->nslots, c->stream);
   }
+  ProfScope ps(c, KC_FORCE);
+  launch_force<R>(a, t, nt1, integrate, c->nblk, c->stream);
+}
+
+int enqueue_forces(imdg_ctx* c, bool integrate) {
+  if (c->mixed)
+    enqueue_forces_t<float>(c, integrate);
+  else
+    enqueue_forces_t<double>(c, integrate);
   HIPCHK(c, hipGetLastError());
   return IMDG_OK;
 }
@@ -373,7 +395,11 @@ int read_ctl(imdg_ctx* c, Ctl* h) {
   return IMDG_OK;
 }
 
-// Grow the list after an overflow and rebuild; returns 1 if it had to.
+size_t list_bytes(imdg_ctx* c, int kmax) {
+  return (size_t)(c->nslots / IMDG_WAVE) * kmax * IMDG_WAVE * 4;
+}
+
+// Grow the list after an overflow and rebuild; *fixed = 1 if it had to.
 int fix_overflow(imdg_ctx* c, int* fixed) {
   *fixed = 0;
   Ctl h;
@@ -382,15 +408,10 @@ int fix_overflow(imdg_ctx* c, int* fixed) {
   if (!h.frozen) return IMDG_OK;
   int newk = std::max(c->kmax + 8, (int)ceil(h.kmax_seen * 1.2));
   newk = (newk + 7) & ~7;
-  hipFree(c->d_nbr);
-  for (auto it = c->bufs.begin(); it != c->bufs.end(); ++it)
-    if (it->p == c->d_nbr) {
-      c->bufs.erase(it);
-      break;
-    }
+  dfree(c, c->d_nbr);
   c->d_nbr = nullptr;
   c->kmax = newk;
-  size_t bytes = (size_t)(c->nslots / IMDG_WAVE) * newk * IMDG_WAVE * 4;
+  size_t bytes = list_bytes(c, newk);
   if ((rc = dalloc(c, (void**)&c->d_nbr, bytes))) return rc;
   c->nbr_bytes = bytes;
   int zero2[2] = {0, 0};  // frozen, overflow
@@ -404,7 +425,7 @@ int fix_overflow(imdg_ctx* c, int* fixed) {
 int ensure_ready(imdg_ctx* c) {
   if (!c->atoms_set) return fail(c, IMDG_ERR_STATE, "no atoms set");
   if (!c->d_nbr) {
-    size_t bytes = (size_t)(c->nslots / IMDG_WAVE) * c->kmax * IMDG_WAVE * 4;
+    size_t bytes = list_bytes(c, c->kmax);
     int rc = dalloc(c, (void**)&c->d_nbr, bytes);
     if (rc) return rc;
     c->nbr_bytes = bytes;
@@ -424,9 +445,49 @@ int build_until_fits(imdg_ctx* c) {
   return fail(c, IMDG_ERR_NOMEM, "neighbour list keeps overflowing");
 }
 
+// Device copies of the per-slot arrays needed to map results back.
+struct SlotView {
+  Ctl ctl;
+  std::vector<double4> pos, mom;
+  std::vector<int> aid, acell, nnb;
+};
+
+int read_slots(imdg_ctx* c, SlotView& v, bool state) {
+  int rc = read_ctl(c, &v.ctl);
+  if (rc) return rc;
+  size_t ns = c->nslots;
+  v.aid.resize(ns);
+  HIPCHK(c, hipMemcpy(v.aid.data(), c->d_aid[v.ctl.rec_cur], sizeof(int) * ns,
+                      hipMemcpyDeviceToHost));
+  if (state) {
+    v.pos.resize(ns);
+    v.mom.resize(ns);
+    HIPCHK(c, hipMemcpy(v.pos.data(), c->d_pos[v.ctl.pos_cur],
+                        sizeof(double4) * ns, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(v.mom.data(), c->d_mom[v.ctl.pos_cur],
+                        sizeof(double4) * ns, hipMemcpyDeviceToHost));
+  }
+  return IMDG_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+long imdg_struct_size(const char* name) {
+  if (!name) return -1;
+  static const struct {
+    const char* n;
+    long size;
+  } tab[] = {{"imdg_config", (long)sizeof(imdg_config)},
+             {"imdg_table", (long)sizeof(imdg_table)},
+             {"imdg_replica", (long)sizeof(imdg_replica)},
+             {"imdg_thermo", (long)sizeof(imdg_thermo)},
+             {"imdg_info", (long)sizeof(imdg_info)}};
+  for (const auto& e : tab)
+    if (!strcmp(e.n, name)) return e.size;
+  return -1;
+}
 
 const char* imdg_last_error(const imdg_ctx* ctx) {
   return ctx ? ctx->err.c_str() : "null context";
@@ -543,20 +604,25 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
                     "replica %d: %d cells along axis %d (periodic axes need "
                     "at least 3 cells of the list radius)",
                     r, R.gdim[d], d);
+      if (R.gdim[d] > 4096)
+        return fail(c, IMDG_ERR_GEOM, "replica %d: %d cells along axis %d", r,
+                    R.gdim[d], d);
     }
     double h = 0.5 * c->cfg.nbl_margin;
     R.skin2 = h * h;
     R.temperature = reps[r].temperature;
     R.isq_tau_eta = reps[r].isq_tau_eta;
-    for (int sx = -1; sx <= 1; sx++)
-      for (int sy = -1; sy <= 1; sy++)
-        for (int sz = -1; sz <= 1; sz++) {
-          int code = (sx + 1) * 9 + (sy + 1) * 3 + (sz + 1);
+    for (int d = 0; d < 3; d++)
+      for (int e = 0; e < 3; e++) R.cvec[d][e] = R.box[d][e] / R.gdim[d];
+    for (int ox = -1; ox <= 1; ox++)
+      for (int oy = -1; oy <= 1; oy++)
+        for (int oz = -1; oz <= 1; oz++) {
+          int code = (ox + 1) * 9 + (oy + 1) * 3 + (oz + 1);
           for (int e = 0; e < 3; e++)
-            R.shift[code][e] =
-                sx * R.box[0][e] + sy * R.box[1][e] + sz * R.box[2][e];
+            R.offs[code][e] =
+                ox * R.cvec[0][e] + oy * R.cvec[1][e] + oz * R.cvec[2][e];
         }
-    for (int e = 0; e < 3; e++) R.shift[IMDG_CODE_PAD][e] = IMDG_PAD_SHIFT;
+    for (int e = 0; e < 3; e++) R.offs[IMDG_CODE_PAD][e] = IMDG_PAD_SHIFT;
     R.atom_off = slot;
     R.natoms = natoms[r];
     R.nslots = ((natoms[r] + IMDG_BLOCK - 1) / IMDG_BLOCK) * IMDG_BLOCK;
@@ -597,18 +663,20 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
     ALLOC(c->d_mom[k], sizeof(double4) * ns);
     ALLOC(c->d_aid[k], sizeof(int) * ns);
   }
-  ALLOC(c->d_posp, sizeof(double4*) * 2);
   ALLOC(c->d_xnbl, sizeof(double4) * ns);
   ALLOC(c->d_cellid, sizeof(int) * ns);
   ALLOC(c->d_cslot, sizeof(int) * ns);
   ALLOC(c->d_acell, sizeof(int) * ns);
+  ALLOC(c->d_atype, ns);
   ALLOC(c->d_perm, sizeof(int) * ns);
   ALLOC(c->d_cell_cnt, sizeof(int) * c->ncells);
   ALLOC(c->d_cell_beg, sizeof(int) * c->ncells);
   ALLOC(c->d_cell_num, sizeof(int) * c->ncells);
+  ALLOC(c->d_tiles, sizeof(Tile) * c->nblk);
   ALLOC(c->d_nnb, sizeof(int) * ns);
+  ALLOC(c->d_reca, rs * 4 * ns);
+  ALLOC(c->d_recb, rs * 4 * ns);
   ALLOC(c->d_rho, rs * ns);
-  ALLOC(c->d_df, rs * ns);
   ALLOC(c->d_epot, rs * ns);
   ALLOC(c->d_frc, rs * 4 * ns);
   ALLOC(c->d_partA, sizeof(double) * c->nblk);
@@ -663,7 +731,6 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
   Ctl hc;
   memset(&hc, 0, sizeof hc);
   hc.any_rebuild = 1;
-  double4* pp[2] = {c->d_pos[0], c->d_pos[1]};
   hipStream_t st = c->stream;
   HIPCHK(c, hipMemcpyAsync(c->d_rep, c->rep.data(), sizeof(RepDev) * nrep,
                            hipMemcpyHostToDevice, st));
@@ -674,7 +741,6 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
                            hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->d_cell_rep, hcrep.data(),
                            sizeof(int) * c->ncells, hipMemcpyHostToDevice, st));
-  HIPCHK(c, hipMemcpyAsync(c->d_posp, pp, sizeof pp, hipMemcpyHostToDevice, st));
   for (int k = 0; k < 2; k++) {
     HIPCHK(c, hipMemcpyAsync(c->d_pos[k], hpos.data(), sizeof(double4) * ns,
                              hipMemcpyHostToDevice, st));
@@ -688,11 +754,14 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
   HIPCHK(c, hipMemsetAsync(c->d_cell_cnt, 0, sizeof(int) * c->ncells, st));
   HIPCHK(c, hipMemsetAsync(c->d_cell_beg, 0, sizeof(int) * c->ncells, st));
   HIPCHK(c, hipMemsetAsync(c->d_cell_num, 0, sizeof(int) * c->ncells, st));
+  HIPCHK(c, hipMemsetAsync(c->d_tiles, 0, sizeof(Tile) * c->nblk, st));
   HIPCHK(c, hipMemsetAsync(c->d_acell, 0, sizeof(int) * ns, st));
+  HIPCHK(c, hipMemsetAsync(c->d_atype, 0, ns, st));
   HIPCHK(c, hipMemsetAsync(c->d_nnb, 0, sizeof(int) * ns, st));
   HIPCHK(c, hipMemsetAsync(c->d_perm, 0, sizeof(int) * ns, st));
+  HIPCHK(c, hipMemsetAsync(c->d_reca, 0, rs * 4 * ns, st));
+  HIPCHK(c, hipMemsetAsync(c->d_recb, 0, rs * 4 * ns, st));
   HIPCHK(c, hipMemsetAsync(c->d_rho, 0, rs * ns, st));
-  HIPCHK(c, hipMemsetAsync(c->d_df, 0, rs * ns, st));
   HIPCHK(c, hipMemsetAsync(c->d_epot, 0, rs * ns, st));
   HIPCHK(c, hipMemsetAsync(c->d_frc, 0, rs * 4 * ns, st));
   HIPCHK(c, hipMemsetAsync(c->d_ring, 0, sizeof(ThermoRec) * IMDG_RING * nrep,
@@ -750,8 +819,8 @@ int imdg_check_nblist(imdg_ctx* c, int* invalid) {
   if (!c) return IMDG_ERR_ARG;
   int rc = ensure_ready(c);
   if (rc) return rc;
-  launch_check(c->d_rep, c->d_rst, c->d_ctl, c->d_blk_rep, c->d_posp,
-               c->d_xnbl, c->d_maxd, c->nrep, c->nblk, c->stream);
+  launch_check(c->d_rep, c->d_ctl, c->d_blk_rep, c->d_pos[0], c->d_pos[1],
+               c->d_xnbl, c->d_maxd, c->nblk, c->stream);
   HIPCHK(c, hipGetLastError());
   std::vector<double> maxd(c->nblk);
   std::vector<RepState> st(c->nrep);
@@ -777,10 +846,7 @@ int imdg_check_nblist(imdg_ctx* c, int* invalid) {
   return IMDG_OK;
 }
 
-int imdg_make_nblist(imdg_ctx* c) {
-  if (!c) return IMDG_ERR_ARG;
-  int rc = ensure_ready(c);
-  if (rc) return rc;
+static int flag_all_for_rebuild(imdg_ctx* c) {
   std::vector<RepState> st(c->nrep);
   HIPCHK(c, hipMemcpyAsync(st.data(), c->d_rst, sizeof(RepState) * c->nrep,
                            hipMemcpyDeviceToHost, c->stream));
@@ -791,6 +857,15 @@ int imdg_make_nblist(imdg_ctx* c) {
                            hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(&c->d_ctl->any_rebuild, &one, sizeof(int),
                            hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return IMDG_OK;
+}
+
+int imdg_make_nblist(imdg_ctx* c) {
+  if (!c) return IMDG_ERR_ARG;
+  int rc = ensure_ready(c);
+  if (rc) return rc;
+  if ((rc = flag_all_for_rebuild(c))) return rc;
   if ((rc = build_until_fits(c))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   collect_profile(c);
@@ -801,24 +876,10 @@ int imdg_deallocate_nblist(imdg_ctx* c) {
   if (!c) return IMDG_ERR_ARG;
   if (!c->atoms_set) return IMDG_OK;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (auto it = c->bufs.begin(); it != c->bufs.end(); ++it)
-    if (it->p == c->d_nbr) {
-      hipFree(it->p);
-      c->bufs.erase(it);
-      break;
-    }
+  dfree(c, c->d_nbr);
   c->d_nbr = nullptr;
   c->nbr_bytes = 0;
-  std::vector<RepState> st(c->nrep);
-  HIPCHK(c, hipMemcpy(st.data(), c->d_rst, sizeof(RepState) * c->nrep,
-                      hipMemcpyDeviceToHost));
-  for (auto& s : st) s.need_rebuild = 1;
-  int one = 1;
-  HIPCHK(c, hipMemcpy(c->d_rst, st.data(), sizeof(RepState) * c->nrep,
-                      hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(&c->d_ctl->any_rebuild, &one, sizeof(int),
-                      hipMemcpyHostToDevice));
-  return IMDG_OK;
+  return flag_all_for_rebuild(c);
 }
 
 int imdg_run(imdg_ctx* c, long nsteps) {
@@ -861,51 +922,44 @@ int imdg_get_atoms(imdg_ctx* c, double* x, double* p, double* f, double* epot,
                    double* rho, double* df) {
   if (!c || !c->atoms_set) return IMDG_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  Ctl h;
-  int rc = read_ctl(c, &h);
+  SlotView v;
+  int rc = read_slots(c, v, true);
   if (rc) return rc;
   size_t ns = c->nslots;
-  std::vector<double4> hp(ns), hm(ns);
-  std::vector<int> ha(ns);
-  HIPCHK(c, hipMemcpy(hp.data(), c->d_pos[h.pos_cur], sizeof(double4) * ns,
-                      hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(hm.data(), c->d_mom[h.rec_cur], sizeof(double4) * ns,
-                      hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(ha.data(), c->d_aid[h.rec_cur], sizeof(int) * ns,
-                      hipMemcpyDeviceToHost));
-  std::vector<double> hf(4 * ns), he(ns), hr(ns), hd(ns);
+  std::vector<double> hf(4 * ns), he(ns), hr(ns), hb(4 * ns);
   if (c->mixed) {
     std::vector<float> t4(4 * ns), t(ns);
     HIPCHK(c, hipMemcpy(t4.data(), c->d_frc, 4 * ns * 4, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < 4 * ns; k++) hf[k] = t4[k];
+    HIPCHK(c, hipMemcpy(t4.data(), c->d_recb, 4 * ns * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < 4 * ns; k++) hb[k] = t4[k];
     HIPCHK(c, hipMemcpy(t.data(), c->d_epot, ns * 4, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < ns; k++) he[k] = t[k];
     HIPCHK(c, hipMemcpy(t.data(), c->d_rho, ns * 4, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < ns; k++) hr[k] = t[k];
-    HIPCHK(c, hipMemcpy(t.data(), c->d_df, ns * 4, hipMemcpyDeviceToHost));
-    for (size_t k = 0; k < ns; k++) hd[k] = t[k];
   } else {
     HIPCHK(c, hipMemcpy(hf.data(), c->d_frc, 4 * ns * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hb.data(), c->d_recb, 4 * ns * 8, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(he.data(), c->d_epot, ns * 8, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(hr.data(), c->d_rho, ns * 8, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(hd.data(), c->d_df, ns * 8, hipMemcpyDeviceToHost));
   }
+  bool eam = c->tab[1].present;
   for (int r = 0; r < c->nrep; r++) {
     const RepDev& R = c->rep[r];
     for (int k = 0; k < R.natoms; k++) {
       int s = R.atom_off + k;
-      long in = ha[s];
+      long in = v.aid[s];
       if (in < 0 || in >= c->natoms_tot)
         return fail(c, IMDG_ERR_STATE, "slot %d has no atom", s);
       if (x) {
-        x[3 * in] = hp[s].x;
-        x[3 * in + 1] = hp[s].y;
-        x[3 * in + 2] = hp[s].z;
+        x[3 * in] = v.pos[s].x;
+        x[3 * in + 1] = v.pos[s].y;
+        x[3 * in + 2] = v.pos[s].z;
       }
       if (p) {
-        p[3 * in] = hm[s].x;
-        p[3 * in + 1] = hm[s].y;
-        p[3 * in + 2] = hm[s].z;
+        p[3 * in] = v.mom[s].x;
+        p[3 * in + 1] = v.mom[s].y;
+        p[3 * in + 2] = v.mom[s].z;
       }
       if (f) {
         f[3 * in] = hf[4 * s];
@@ -913,8 +967,8 @@ int imdg_get_atoms(imdg_ctx* c, double* x, double* p, double* f, double* epot,
         f[3 * in + 2] = hf[4 * s + 2];
       }
       if (epot) epot[in] = he[s];
-      if (rho) rho[in] = hr[s];
-      if (df) df[in] = hd[s];
+      if (rho) rho[in] = eam ? hr[s] : 0.0;
+      if (df) df[in] = eam ? hb[4 * s + 3] : 0.0;
     }
   }
   return IMDG_OK;
@@ -980,13 +1034,11 @@ int imdg_get_thermo_history(imdg_ctx* c, long first, long count,
 int imdg_get_cells(imdg_ctx* c, int* cell3) {
   if (!c || !c->atoms_set || !cell3) return IMDG_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  Ctl h;
-  int rc = read_ctl(c, &h);
+  SlotView v;
+  int rc = read_slots(c, v, false);
   if (rc) return rc;
   size_t ns = c->nslots;
-  std::vector<int> ha(ns), hc(ns);
-  HIPCHK(c, hipMemcpy(ha.data(), c->d_aid[h.rec_cur], sizeof(int) * ns,
-                      hipMemcpyDeviceToHost));
+  std::vector<int> hc(ns);
   HIPCHK(c, hipMemcpy(hc.data(), c->d_acell, sizeof(int) * ns,
                       hipMemcpyDeviceToHost));
   for (int r = 0; r < c->nrep; r++) {
@@ -994,7 +1046,7 @@ int imdg_get_cells(imdg_ctx* c, int* cell3) {
     for (int k = 0; k < R.natoms; k++) {
       int s = R.atom_off + k;
       int lc = hc[s] - R.cell_off;
-      long in = ha[s];
+      long in = v.aid[s];
       cell3[3 * in + 2] = lc % R.gdim[2];
       cell3[3 * in + 1] = (lc / R.gdim[2]) % R.gdim[1];
       cell3[3 * in + 0] = lc / (R.gdim[1] * R.gdim[2]);
@@ -1006,16 +1058,19 @@ int imdg_get_cells(imdg_ctx* c, int* cell3) {
 int imdg_get_pairs(imdg_ctx* c, long* out, long cap, long* npairs) {
   if (!c || !c->atoms_set) return IMDG_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  Ctl h;
-  int rc = read_ctl(c, &h);
-  if (rc) return rc;
   if (!c->d_nbr) return fail(c, IMDG_ERR_STATE, "no list");
+  SlotView v;
+  int rc = read_slots(c, v, false);
+  if (rc) return rc;
   size_t ns = c->nslots;
-  std::vector<int> ha(ns), hn(ns);
+  std::vector<int> hn(ns), hc(ns);
+  std::vector<Tile> tiles(c->nblk);
   std::vector<uint32_t> hl(c->nbr_bytes / 4);
-  HIPCHK(c, hipMemcpy(ha.data(), c->d_aid[h.rec_cur], sizeof(int) * ns,
-                      hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(hn.data(), c->d_nnb, sizeof(int) * ns,
+                      hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(hc.data(), c->d_acell, sizeof(int) * ns,
+                      hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(tiles.data(), c->d_tiles, sizeof(Tile) * c->nblk,
                       hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(hl.data(), c->d_nbr, c->nbr_bytes, hipMemcpyDeviceToHost));
   long np = 0;
@@ -1023,26 +1078,41 @@ int imdg_get_pairs(imdg_ctx* c, long* out, long cap, long* npairs) {
     const RepDev& R = c->rep[r];
     for (int k = 0; k < R.natoms; k++) {
       int s = R.atom_off + k;
-      long ii = ha[s] - c->in_off[r];
+      const Tile& T = tiles[s / IMDG_BLOCK];
+      long ii = v.aid[s] - c->in_off[r];
+      int lc = hc[s] - R.cell_off;
+      int ci[3] = {lc / (R.gdim[1] * R.gdim[2]), (lc / R.gdim[2]) % R.gdim[1],
+                   lc % R.gdim[2]};
       for (int m = 0; m < hn[s]; m++) {
         size_t at = (((size_t)(s >> 6) * (c->kmax >> 2) + (m >> 2)) * IMDG_WAVE +
                      (s & 63)) * IMDG_GROUP + (m & 3);
         uint32_t e = hl[at];
-        int j = (int)(e & IMDG_IDX_MASK), code = (int)(e >> IMDG_IDX_BITS);
+        int idx = (int)(e & IMDG_IDX_MASK), code = (int)(e >> IMDG_IDX_BITS);
         if (code == IMDG_CODE_PAD) continue;
-        long jj = ha[j] - c->in_off[r];
-        if (ii > jj) continue;  // each unordered pair once, from its lower index
-        int sx = code / 9 - 1, sy = (code / 3) % 3 - 1, sz = code % 3 - 1;
-        if (ii == jj && (sx < 0 || (sx == 0 && (sy < 0 || (sy == 0 && sz < 0)))))
+        int j = idx;
+        if (T.ok) {
+          int q = 0;
+          while (q + 1 < T.nrun && T.run[q + 1].z <= idx) q++;
+          j = T.run[q].x + (idx - T.run[q].z);
+        }
+        long jj = v.aid[j] - c->in_off[r];
+        int o[3] = {code / 9 - 1, (code / 3) % 3 - 1, code % 3 - 1}, sh[3];
+        for (int d = 0; d < 3; d++) {
+          int n = ci[d] + o[d];
+          sh[d] = n < 0 ? -1 : (n >= R.gdim[d] ? 1 : 0);
+        }
+        if (ii > jj) continue;  // each unordered pair once
+        if (ii == jj &&
+            (sh[0] < 0 || (sh[0] == 0 && (sh[1] < 0 || (sh[1] == 0 && sh[2] < 0)))))
           continue;
         if (np < cap) {
-          long* o = out + 6 * np;
-          o[0] = r;
-          o[1] = ii;
-          o[2] = jj;
-          o[3] = sx;
-          o[4] = sy;
-          o[5] = sz;
+          long* ob = out + 6 * np;
+          ob[0] = r;
+          ob[1] = ii;
+          ob[2] = jj;
+          ob[3] = sh[0];
+          ob[4] = sh[1];
+          ob[5] = sh[2];
         }
         np++;
       }
@@ -1079,6 +1149,7 @@ int imdg_get_info(imdg_ctx* c, imdg_info* info) {
   info->steps_done = h.steps_done;
   info->is_short = h.is_short;
   info->nbl_count = st.nbl_count;
+  info->tiles_fallback = h.tiles_fallback;
   std::vector<int> hn(c->nslots);
   HIPCHK(c, hipMemcpy(hn.data(), c->d_nnb, sizeof(int) * c->nslots,
                       hipMemcpyDeviceToHost));

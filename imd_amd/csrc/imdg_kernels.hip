@@ -1,7 +1,7 @@
 // imdg_kernels.hip -- CDNA4 (gfx950) kernels of the neighbour-list pair/EAM
-// path: cell binning and sort, full neighbour list build, host density with
-// embedding, pair + EAM forces fused with the leapfrog step and the skin
-// test, compensated FP64 reductions.
+// path: cell binning and sort, per-block neighbour tiles, full neighbour
+// list build, host density with embedding, pair + EAM forces fused with the
+// leapfrog step and the skin test, compensated FP64 reductions.
 //
 // Reference semantics (paths relative to the reference root):
 //   binning / wrap     src/imd_fix_cells_3d.c:36-201, src/imd_main_3d.c:1972-2065,
@@ -30,13 +30,9 @@ namespace imdg {
 // helpers
 // ---------------------------------------------------------------------------
 
-template <typename R> struct V4;
-template <> struct V4<float> { typedef float4 T; };
-template <> struct V4<double> { typedef double4 T; };
-
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous run of logical blocks (a contiguous, spatially compact run of
-// cell-sorted atoms) so neighbour gathers stay inside that XCD's L2.
+// contiguous run of blocks (a contiguous, spatially compact run of
+// cell-sorted atoms) so neighbour data stays inside that XCD's L2.
 __device__ __forceinline__ int logical_block() {
   int nb = gridDim.x, b = blockIdx.x;
   int q = nb >> 3, rem = nb & 7, x = b & 7, idx = b >> 3;
@@ -76,10 +72,10 @@ __device__ __forceinline__ void kahan_add(double& s, double& c, double x) {
 }
 
 // Block-wide reduction of N values (sum except the indices flagged in
-// maxmask), result valid in thread 0.  256 threads = 4 wavefronts.
+// maxmask), result valid in thread 0.
 template <int N>
 __device__ __forceinline__ void block_reduce(double (&v)[N], unsigned maxmask,
-                                             double* lds /* 4*N */) {
+                                             double* lds /* NWAVE*N */) {
 #pragma unroll
   for (int q = 0; q < N; q++)
     v[q] = ((maxmask >> q) & 1) ? wave_max(v[q]) : wave_sum(v[q]);
@@ -93,7 +89,8 @@ __device__ __forceinline__ void block_reduce(double (&v)[N], unsigned maxmask,
 #pragma unroll
     for (int q = 0; q < N; q++) {
       double a = lds[q];
-      for (int k = 1; k < IMDG_BLOCK / IMDG_WAVE; k++)
+#pragma unroll
+      for (int k = 1; k < IMDG_NWAVE; k++)
         a = ((maxmask >> q) & 1) ? fmax(a, lds[k * N + q]) : a + lds[k * N + q];
       v[q] = a;
     }
@@ -103,59 +100,41 @@ __device__ __forceinline__ void block_reduce(double (&v)[N], unsigned maxmask,
 template <typename R>
 __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
 
-// IMD's 3-point table access on precomputed segment coefficients:
+// Segment index and fraction of IMD's 3-point table access
+// (src/potaccess.h:329-340): argument clamped to [begin, end].
+template <typename R>
+__device__ __forceinline__ int seg_locate(R x, R begin, R end, R invstep,
+                                          int len, R& chi) {
+  R a = rmin(x, end) - begin;
+  a = a < (R)0 ? (R)0 : a;
+  R t = a * invstep;
+  int k = (int)t;
+  chi = t - (R)k;
+  return k < len - 1 ? k : len - 1;
+}
+
 // value p0 + chi (dv + (chi-1)/2 d2v) and twice the argument-derivative
-// 2 istep (dv + (chi - 1/2) d2v), argument clamped to [begin, end].
+// 2 istep (dv + (chi - 1/2) d2v) of IMD's 3-point interpolation
+template <typename R>
+__device__ __forceinline__ R seg_val(const Seg<R>& c, R chi) {
+  return c.p0 + chi * (c.dv + (R)0.5 * (chi - (R)1) * c.d2v);
+}
+template <typename R>
+__device__ __forceinline__ R seg_grad(const Seg<R>& c, R chi, R twoistep) {
+  return twoistep * (c.dv + (chi - (R)0.5) * c.d2v);
+}
+
 template <typename R>
 __device__ __forceinline__ void tab_val_grad(const Seg<R>* seg, int len,
                                              R begin, R end, R invstep,
                                              R twoistep, R x, R& val, R& grad,
                                              int& is_short) {
-  R a = rmin(x, end) - begin;
-  if (a < (R)0) {
-    a = (R)0;
-    is_short = 1;
-  }
-  R t = a * invstep;
-  int k = (int)t;
-  R chi = t - (R)k;
-  k = k < len - 1 ? k : len - 1;
+  R chi;
+  int k = seg_locate(x, begin, end, invstep, len, chi);
+  is_short |= (x < begin);
   Seg<R> c = seg[k];
-  val = c.p0 + chi * (c.dv + (R)0.5 * (chi - (R)1) * c.d2v);
-  grad = twoistep * (c.dv + (chi - (R)0.5) * c.d2v);
-}
-
-template <typename R>
-__device__ __forceinline__ R tab_val(const Seg<R>* seg, int len, R begin,
-                                     R end, R invstep, R x, int& is_short) {
-  R a = rmin(x, end) - begin;
-  if (a < (R)0) {
-    a = (R)0;
-    is_short = 1;
-  }
-  R t = a * invstep;
-  int k = (int)t;
-  R chi = t - (R)k;
-  k = k < len - 1 ? k : len - 1;
-  Seg<R> c = seg[k];
-  return c.p0 + chi * (c.dv + (R)0.5 * (chi - (R)1) * c.d2v);
-}
-
-template <typename R>
-__device__ __forceinline__ R tab_grad(const Seg<R>* seg, int len, R begin,
-                                      R end, R invstep, R twoistep, R x,
-                                      int& is_short) {
-  R a = rmin(x, end) - begin;
-  if (a < (R)0) {
-    a = (R)0;
-    is_short = 1;
-  }
-  R t = a * invstep;
-  int k = (int)t;
-  R chi = t - (R)k;
-  k = k < len - 1 ? k : len - 1;
-  Seg<R> c = seg[k];
-  return twoistep * (c.dv + (chi - (R)0.5) * c.d2v);
+  val = seg_val(c, chi);
+  grad = seg_grad(c, chi, twoistep);
 }
 
 // Entry m of slot i in the grouped list layout (4 entries per lane).
@@ -165,12 +144,39 @@ __device__ __forceinline__ size_t nbr_index(int i, int m, int kmax) {
          (m & 3);
 }
 
+// First uint4 of slot i's list.
+__device__ __forceinline__ const uint4* list_of(const uint32_t* nbr, int i,
+                                                int kmax) {
+  return reinterpret_cast<const uint4*>(nbr) +
+         ((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63);
+}
+
+__device__ __forceinline__ void cell_coords(const RepDev& R, int cell,
+                                            int& cx, int& cy, int& cz) {
+  int lc = cell - R.cell_off;
+  cz = lc % R.gdim[2];
+  int q = lc / R.gdim[2];
+  cy = q % R.gdim[1];
+  cx = q / R.gdim[1];
+}
+
+// Centre of a cell: sum_d (c_d + 1/2) box_d / gdim_d.
+__device__ __forceinline__ void cell_centre(const RepDev& R, int cell,
+                                            double* c) {
+  int cx, cy, cz;
+  cell_coords(R, cell, cx, cy, cz);
+  double fx = cx + 0.5, fy = cy + 0.5, fz = cz + 0.5;
+#pragma unroll
+  for (int e = 0; e < 3; e++)
+    c[e] = fx * R.cvec[0][e] + fy * R.cvec[1][e] + fz * R.cvec[2][e];
+}
+
 // ---------------------------------------------------------------------------
-// rebuild: wrap + bin, scan, place, per-cell order, permute, list
+// rebuild: wrap + bin, scan, place, per-cell order, permute, tiles, list
 // ---------------------------------------------------------------------------
 
 // fix_cells/do_boundaries + cell_coord, for replicas that need a rebuild.
-__global__ void __launch_bounds__(IMDG_BLOCK) k_bin(BuildArgs a) {
+__global__ void __launch_bounds__(256) k_bin(BuildArgs a) {
 #pragma clang fp contract(off)
   if (!a.ctl->any_rebuild || a.ctl->frozen) return;
   double4* pos = a.pos[a.ctl->pos_cur];
@@ -221,7 +227,6 @@ __global__ void __launch_bounds__(1024) k_scan(BuildArgs a) {
   for (int c = b; c < e; c++) s += a.cell_cnt[base + c];
   sums[t] = s;
   __syncthreads();
-  // Hillis-Steele inclusive scan over 1024 partial sums
   for (int o = 1; o < 1024; o <<= 1) {
     int v = (t >= o) ? sums[t - o] : 0;
     __syncthreads();
@@ -238,7 +243,7 @@ __global__ void __launch_bounds__(1024) k_scan(BuildArgs a) {
   }
 }
 
-__global__ void __launch_bounds__(IMDG_BLOCK) k_place(BuildArgs a) {
+__global__ void __launch_bounds__(256) k_place(BuildArgs a) {
   if (!a.ctl->any_rebuild || a.ctl->frozen) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.nslots;
        i += gridDim.x * blockDim.x) {
@@ -251,7 +256,7 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_place(BuildArgs a) {
 }
 
 // Order the atoms of a cell by input index (deterministic list order).
-__global__ void __launch_bounds__(IMDG_BLOCK) k_cellsort(BuildArgs a) {
+__global__ void __launch_bounds__(256) k_cellsort(BuildArgs a) {
   if (!a.ctl->any_rebuild || a.ctl->frozen) return;
   const int* aid = a.aid[a.ctl->rec_cur];
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < a.ncells;
@@ -272,8 +277,9 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_cellsort(BuildArgs a) {
 }
 
 // Gather every slot into the other buffers (new cell-sorted order for the
-// rebuilt replicas, identity for the others), then flip the buffers.
-__global__ void __launch_bounds__(IMDG_BLOCK) k_permute(BuildArgs a) {
+// rebuilt replicas, identity for the others), write the cell-relative
+// records of the moved atoms, then flip the buffers.
+__global__ void __launch_bounds__(256) k_permute(BuildArgs a) {
   if (!a.ctl->any_rebuild || a.ctl->frozen) return;
   int pc = a.ctl->pos_cur, rc = a.ctl->rec_cur;
   const double4* pin = a.pos[pc];
@@ -293,8 +299,19 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_permute(BuildArgs a) {
     mout[i] = min_[old];
     aout[i] = ain[old];
     if (moved) {
-      a.acell[i] = a.cellid[old];
+      int cell = a.cellid[old];
+      a.acell[i] = cell;
+      a.atype[i] = (signed char)p.w;
       a.xnbl[i] = p;
+      double c[3];
+      cell_centre(R, cell, c);
+      if (a.mixed)
+        reinterpret_cast<float4*>(a.rec_a)[i] =
+            make_float4((float)(p.x - c[0]), (float)(p.y - c[1]),
+                        (float)(p.z - c[2]), (float)p.w);
+      else
+        reinterpret_cast<double4*>(a.rec_a)[i] =
+            make_double4(p.x - c[0], p.y - c[1], p.z - c[2], p.w);
     }
   }
   __syncthreads();
@@ -306,6 +323,115 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_permute(BuildArgs a) {
       a.ctl->rec_cur = rc ^ 1;
       a.ctl->done[DC_PERMUTE] = 0;
       __threadfence();
+    }
+  }
+}
+
+// Tiles: for every block of a rebuilt replica, the slot runs of all cells
+// adjacent (27-stencil, periodic wrap, nothing beyond an open face) to the
+// cells of the block's atoms, sorted and merged.
+#define TILE_MAXRANGE 96
+__global__ void __launch_bounds__(64) k_tiles(BuildArgs a) {
+  if (!a.ctl->any_rebuild || a.ctl->frozen) return;
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < a.nblk;
+       b += gridDim.x * blockDim.x) {
+    int r = a.blk_rep[b];
+    if (!a.rst[r].need_rebuild) continue;
+    const RepDev& R = a.rep[r];
+    Tile& T = a.tiles[b];
+    int s0 = b * IMDG_BLOCK;
+    int s1 = min(s0 + IMDG_BLOCK, R.atom_off + R.natoms) - 1;
+    T.nrun = 0;
+    T.natoms = 0;
+    T.ok = 1;
+    if (s1 < s0) continue;
+    int g0 = R.gdim[0], g1 = R.gdim[1], g2 = R.gdim[2];
+    int lc0 = a.acell[s0] - R.cell_off, lc1 = a.acell[s1] - R.cell_off;
+    int rb[TILE_MAXRANGE], re[TILE_MAXRANGE], nr = 0;
+    bool ok = true;
+    // walk the block's cells column by column
+    for (int col = lc0 / g2; col <= lc1 / g2 && ok; col++) {
+      int z0 = (col == lc0 / g2) ? lc0 % g2 : 0;
+      int z1 = (col == lc1 / g2) ? lc1 % g2 : g2 - 1;
+      int cx = col / g1, cy = col % g1;
+      for (int ox = -1; ox <= 1 && ok; ox++) {
+        int nx = cx + ox;
+        if (nx < 0 || nx >= g0) {
+          if (R.pbc[0] != 1) continue;
+          nx = (nx + g0) % g0;
+        }
+        for (int oy = -1; oy <= 1 && ok; oy++) {
+          int ny = cy + oy;
+          if (ny < 0 || ny >= g1) {
+            if (R.pbc[1] != 1) continue;
+            ny = (ny + g1) % g1;
+          }
+          int za = z0 - 1, zb = z1 + 1, p[2], q[2], np = 0;
+          if (R.pbc[2] == 1) {
+            if (zb - za + 1 >= g2) {
+              p[0] = 0; q[0] = g2 - 1; np = 1;
+            } else if (za < 0) {
+              p[0] = za + g2; q[0] = g2 - 1; p[1] = 0; q[1] = zb; np = 2;
+            } else if (zb >= g2) {
+              p[0] = za; q[0] = g2 - 1; p[1] = 0; q[1] = zb - g2; np = 2;
+            } else {
+              p[0] = za; q[0] = zb; np = 1;
+            }
+          } else {
+            p[0] = za < 0 ? 0 : za;
+            q[0] = zb >= g2 ? g2 - 1 : zb;
+            np = 1;
+          }
+          int cbase = R.cell_off + (nx * g1 + ny) * g2;
+          for (int k = 0; k < np; k++) {
+            if (nr >= TILE_MAXRANGE) {
+              ok = false;
+              break;
+            }
+            int cf = cbase + p[k], cl = cbase + q[k];
+            rb[nr] = a.cell_beg[cf];
+            re[nr] = a.cell_beg[cl] + a.cell_num[cl];
+            if (re[nr] > rb[nr]) nr++;
+          }
+        }
+      }
+    }
+    // sort ranges by first slot, merge overlapping or touching ones
+    for (int k = 1; k < nr; k++) {
+      int vb = rb[k], ve = re[k], m = k - 1;
+      while (m >= 0 && rb[m] > vb) {
+        rb[m + 1] = rb[m];
+        re[m + 1] = re[m];
+        m--;
+      }
+      rb[m + 1] = vb;
+      re[m + 1] = ve;
+    }
+    int nrun = 0, tot = 0;
+    for (int k = 0; k < nr && ok; k++) {
+      if (nrun > 0 && rb[k] <= T.run[nrun - 1].x + T.run[nrun - 1].y) {
+        int4& u = T.run[nrun - 1];
+        int end = max(u.x + u.y, re[k]);
+        tot += end - (u.x + u.y);
+        u.y = end - u.x;
+      } else {
+        if (nrun >= IMDG_MAXRUN) {
+          ok = false;
+          break;
+        }
+        T.run[nrun] = make_int4(rb[k], re[k] - rb[k], tot, 0);
+        tot += re[k] - rb[k];
+        nrun++;
+      }
+    }
+    if (!ok || tot > IMDG_TILE_CAP) {
+      T.ok = 0;
+      T.nrun = 0;
+      T.natoms = 0;
+      atomicAdd(&a.ctl->tiles_fallback, 1);
+    } else {
+      T.nrun = nrun;
+      T.natoms = tot;
     }
   }
 }
@@ -340,103 +466,156 @@ __device__ __forceinline__ void image_of(const RepDev& R, double x0, double x1,
   }
 }
 
+// Position of a slot inside the block's tile (runs sorted by first slot).
+__device__ __forceinline__ int tile_index(const int4* runs, int nrun, int j) {
+  int lo = 0, hi = nrun - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (runs[mid].x <= j)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return runs[lo].z + (j - runs[lo].x);
+}
+
+// Slot of a tile position (runs sorted by first tile position).
+__device__ __forceinline__ int tile_slot(const int4* runs, int nrun, int t) {
+  int lo = 0, hi = nrun - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (runs[mid].z <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return runs[lo].x + (t - runs[lo].z);
+}
+
 // Full neighbour list.  For each cell offset o of the 27, the reference
 // evaluates a pair from the atom whose cell has the other in its half
 // stencil (o lexicographically positive: the stencil of
 // src/imd_geom_3d.c:895-899) as (image of the partner) - (owner); the same
 // expression decides membership here, so each unordered pair is in the
 // list of both atoms exactly when the reference has it in its half list.
+// One block per atom block; its tile's FP64 positions are staged in LDS.
 __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
 #pragma clang fp contract(off)
+  __shared__ double4 tpos[IMDG_TILE_CAP];
+  __shared__ int4 truns[IMDG_MAXRUN];
+  __shared__ int tinfo[2];
   if (!a.ctl->any_rebuild || a.ctl->frozen) return;
+  int b = blockIdx.x;
+  int r = a.blk_rep[b];
+  const RepDev& R = a.rep[r];
+  bool flagged = a.rst[r].need_rebuild != 0;
   const double4* pos = a.pos[a.ctl->pos_cur];
   int kmax = a.kmax, seen = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.nslots;
-       i += gridDim.x * blockDim.x) {
-    int r = a.blk_rep[i / IMDG_BLOCK];
-    if (!a.rst[r].need_rebuild) continue;
-    const RepDev& R = a.rep[r];
-    if (i - R.atom_off >= R.natoms) continue;
-    int lc = a.acell[i] - R.cell_off;
-    int g0 = R.gdim[0], g1 = R.gdim[1], g2 = R.gdim[2];
-    int c2 = lc % g2, c1 = (lc / g2) % g1, c0 = lc / (g1 * g2);
-    double4 xi = pos[i];
-    double cs = R.cellsz;
-    int cnt = 0;
-    for (int ox = -1; ox <= 1; ox++) {
-      int n0 = c0 + ox, sx = 0;
-      if (n0 < 0) {
-        if (R.pbc[0] != 1) continue;
-        n0 += g0;
-        sx = -1;
-      } else if (n0 >= g0) {
-        if (R.pbc[0] != 1) continue;
-        n0 -= g0;
-        sx = 1;
-      }
-      for (int oy = -1; oy <= 1; oy++) {
-        int n1 = c1 + oy, sy = 0;
-        if (n1 < 0) {
-          if (R.pbc[1] != 1) continue;
-          n1 += g1;
-          sy = -1;
-        } else if (n1 >= g1) {
-          if (R.pbc[1] != 1) continue;
-          n1 -= g1;
-          sy = 1;
-        }
-        for (int oz = -1; oz <= 1; oz++) {
-          int n2 = c2 + oz, sz = 0;
-          if (n2 < 0) {
-            if (R.pbc[2] != 1) continue;
-            n2 += g2;
-            sz = -1;
-          } else if (n2 >= g2) {
-            if (R.pbc[2] != 1) continue;
-            n2 -= g2;
-            sz = 1;
-          }
-          bool self = (ox == 0 && oy == 0 && oz == 0);
-          bool owner = ox > 0 || (ox == 0 && (oy > 0 || (oy == 0 && oz >= 0)));
-          uint32_t code = (uint32_t)((sx + 1) * 9 + (sy + 1) * 3 + (sz + 1));
-          int nc = R.cell_off + (n0 * g1 + n1) * g2 + n2;
-          int jb = a.cell_beg[nc], je = jb + a.cell_num[nc];
-          for (int j = jb; j < je; j++) {
-            if (self && j == i) continue;
-            double4 xj = pos[j];
-            double d0, d1, d2;
-            if (owner) {
-              double y0, y1, y2;
-              image_of(R, xj.x, xj.y, xj.z, sx, sy, sz, y0, y1, y2);
-              d0 = y0 - xi.x;
-              d1 = y1 - xi.y;
-              d2 = y2 - xi.z;
-            } else {
-              double y0, y1, y2;
-              image_of(R, xi.x, xi.y, xi.z, -sx, -sy, -sz, y0, y1, y2);
-              d0 = y0 - xj.x;
-              d1 = y1 - xj.y;
-              d2 = y2 - xj.z;
-            }
-            double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-            if (r2 < cs) {
-              if (cnt < kmax) a.nbr[nbr_index(i, cnt, kmax)] = (uint32_t)j | (code << IMDG_IDX_BITS);
-              cnt++;
-            }
-          }
-        }
-      }
+  if (flagged) {
+    const Tile& T = a.tiles[b];
+    if (threadIdx.x == 0) {
+      tinfo[0] = T.nrun;
+      tinfo[1] = T.ok;
     }
-    // pad to a whole group with far-away images of the atom itself
-    int pend = (cnt + IMDG_GROUP - 1) & ~(IMDG_GROUP - 1);
-    for (int q = cnt; q < pend && q < kmax; q++)
-      a.nbr[nbr_index(i, q, kmax)] =
-          (uint32_t)i | ((uint32_t)IMDG_CODE_PAD << IMDG_IDX_BITS);
-    a.nnb[i] = cnt < kmax ? cnt : kmax;
-    seen = max(seen, cnt);
+    for (int k = threadIdx.x; k < T.nrun; k += blockDim.x) truns[k] = T.run[k];
+    __syncthreads();
+    int nrun = tinfo[0], tok = tinfo[1];
+    if (tok)
+      for (int k = 0; k < nrun; k++) {
+        int4 u = truns[k];
+        for (int t = threadIdx.x; t < u.y; t += blockDim.x)
+          tpos[u.z + t] = pos[u.x + t];
+      }
+    __syncthreads();
+    int i = b * IMDG_BLOCK + threadIdx.x;
+    if (i - R.atom_off < R.natoms) {
+      int cx, cy, cz;
+      cell_coords(R, a.acell[i], cx, cy, cz);
+      int g0 = R.gdim[0], g1 = R.gdim[1], g2 = R.gdim[2];
+      double4 xi = pos[i];
+      double cs = R.cellsz;
+      int cnt = 0;
+      for (int ox = -1; ox <= 1; ox++) {
+        int n0 = cx + ox, sx = 0;
+        if (n0 < 0) {
+          if (R.pbc[0] != 1) continue;
+          n0 += g0;
+          sx = -1;
+        } else if (n0 >= g0) {
+          if (R.pbc[0] != 1) continue;
+          n0 -= g0;
+          sx = 1;
+        }
+        for (int oy = -1; oy <= 1; oy++) {
+          int n1 = cy + oy, sy = 0;
+          if (n1 < 0) {
+            if (R.pbc[1] != 1) continue;
+            n1 += g1;
+            sy = -1;
+          } else if (n1 >= g1) {
+            if (R.pbc[1] != 1) continue;
+            n1 -= g1;
+            sy = 1;
+          }
+          for (int oz = -1; oz <= 1; oz++) {
+            int n2 = cz + oz, sz = 0;
+            if (n2 < 0) {
+              if (R.pbc[2] != 1) continue;
+              n2 += g2;
+              sz = -1;
+            } else if (n2 >= g2) {
+              if (R.pbc[2] != 1) continue;
+              n2 -= g2;
+              sz = 1;
+            }
+            bool self = (ox == 0 && oy == 0 && oz == 0);
+            bool owner =
+                ox > 0 || (ox == 0 && (oy > 0 || (oy == 0 && oz >= 0)));
+            uint32_t code = (uint32_t)((ox + 1) * 9 + (oy + 1) * 3 + (oz + 1));
+            int nc = R.cell_off + (n0 * g1 + n1) * g2 + n2;
+            int jb = a.cell_beg[nc], je = jb + a.cell_num[nc];
+            int tb = (tok && je > jb) ? tile_index(truns, nrun, jb) : jb;
+            for (int j = jb; j < je; j++) {
+              if (self && j == i) continue;
+              double4 xj = tok ? tpos[tb + (j - jb)] : pos[j];
+              double d0, d1, d2;
+              if (owner) {
+                double y0, y1, y2;
+                image_of(R, xj.x, xj.y, xj.z, sx, sy, sz, y0, y1, y2);
+                d0 = y0 - xi.x;
+                d1 = y1 - xi.y;
+                d2 = y2 - xi.z;
+              } else {
+                double y0, y1, y2;
+                image_of(R, xi.x, xi.y, xi.z, -sx, -sy, -sz, y0, y1, y2);
+                d0 = y0 - xj.x;
+                d1 = y1 - xj.y;
+                d2 = y2 - xj.z;
+              }
+              double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+              if (r2 < cs) {
+                if (cnt < kmax)
+                  a.nbr[nbr_index(i, cnt, kmax)] =
+                      (uint32_t)(tb + (j - jb)) | (code << IMDG_IDX_BITS);
+                cnt++;
+              }
+            }
+          }
+        }
+      }
+      // pad to a whole group with far-away copies of the atom itself
+      int self_idx = tok ? tile_index(truns, nrun, i) : i;
+      int pend = (cnt + IMDG_GROUP - 1) & ~(IMDG_GROUP - 1);
+      for (int q = cnt; q < pend && q < kmax; q++)
+        a.nbr[nbr_index(i, q, kmax)] =
+            (uint32_t)self_idx | ((uint32_t)IMDG_CODE_PAD << IMDG_IDX_BITS);
+      a.nnb[i] = cnt < kmax ? cnt : kmax;
+      seen = cnt;
+    }
   }
-  // block max of the list length, then one atomic per block
-  for (int o = 32; o > 0; o >>= 1) seen = max(seen, __shfl_xor(seen, o, IMDG_WAVE));
+  // block max of the list length, then one atomic per wave
+  for (int o = 32; o > 0; o >>= 1)
+    seen = max(seen, __shfl_xor(seen, o, IMDG_WAVE));
   if ((threadIdx.x % IMDG_WAVE) == 0 && seen > 0) {
     atomicMax(&a.ctl->kmax_seen, seen);
     if (seen > kmax) atomicExch(&a.ctl->overflow, 1);
@@ -469,69 +648,74 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
 // per-step kernels
 // ---------------------------------------------------------------------------
 
-// Image shift vectors of the block's replica, staged in LDS.
-__device__ __forceinline__ void load_shifts(const RepDev& R, double* sh) {
+// Cell-offset vectors of the block's replica in the force precision.
+template <typename R>
+__device__ __forceinline__ void load_offsets(const RepDev& Rp, R* off) {
   for (int k = threadIdx.x; k < 3 * IMDG_NCODES; k += blockDim.x)
-    sh[k] = (&R.shift[0][0])[k];
+    off[k] = (R)((&Rp.offs[0][0])[k]);
 }
 
-// First uint4 of slot i's list (groups of four entries per lane).
-__device__ __forceinline__ const uint4* list_of(const uint32_t* nbr, int i,
-                                                int kmax) {
-  return reinterpret_cast<const uint4*>(nbr) +
-         ((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63);
+// Copy the block's tile of records into LDS (coalesced run copies).
+template <typename R4>
+__device__ __forceinline__ int stage_tile(const Tile* tiles, int b,
+                                          const R4* __restrict__ src, R4* dst,
+                                          int4* runs, int* nrun_out) {
+  const Tile& T = tiles[b];
+  int nrun = T.nrun, ok = T.ok;
+  for (int k = threadIdx.x; k < nrun; k += blockDim.x) runs[k] = T.run[k];
+  __syncthreads();
+  if (ok)
+    for (int k = 0; k < nrun; k++) {
+      int4 u = runs[k];
+      for (int t = threadIdx.x; t < u.y; t += blockDim.x)
+        dst[u.z + t] = src[u.x + t];
+    }
+  __syncthreads();
+  *nrun_out = nrun;
+  return ok;
 }
 
-// Separation of neighbour j (its image given by the entry's code) from i:
-// computed in FP64, then rounded to the force precision.
-template <typename R>
-__device__ __forceinline__ void separation(const double4& xi,
-                                           const double4& xj,
-                                           const double* s, R& dx, R& dy,
-                                           R& dz) {
-  dx = (R)((xj.x + s[0]) - xi.x);
-  dy = (R)((xj.y + s[1]) - xi.y);
-  dz = (R)((xj.z + s[2]) - xi.z);
-}
-
-// Segment index and fraction of IMD's 3-point table access
-// (src/potaccess.h:329-340): argument clamped to [begin, end].
-template <typename R>
-__device__ __forceinline__ int seg_locate(R x, R begin, R end, R invstep,
-                                          int len, R& chi) {
-  R a = rmin(x, end) - begin;
-  a = a < (R)0 ? (R)0 : a;
-  R t = a * invstep;
-  int k = (int)t;
-  chi = t - (R)k;
-  return k < len - 1 ? k : len - 1;
+// Separation of a listed neighbour: (u_j - u_i) + offset(code).
+template <typename R, typename R4>
+__device__ __forceinline__ void separation(const R4& ui, const R4& uj,
+                                           const R* off, uint32_t code,
+                                           R& dx, R& dy, R& dz) {
+  const R* o = off + 3 * code;
+  dx = (uj.x - ui.x) + o[0];
+  dy = (uj.y - ui.y) + o[1];
+  dz = (uj.z - ui.z) + o[2];
 }
 
 // Host density, embedding energy and 2F'(rho) (src/imd_forces_nbl.c:586-611,
-// 1079-1112).  One thread per atom, full list, gather only; four neighbours
-// per step so their loads are in flight together.
+// 1079-1112).  One thread per atom, full list, neighbour records gathered
+// from the block's LDS tile, four neighbours per step.
 template <typename R, bool NT1>
 __global__ void __launch_bounds__(IMDG_BLOCK)
     k_density(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
-  __shared__ double sh[3 * IMDG_NCODES];
-  __shared__ double red[4];
+  typedef typename V4<R>::T R4;
+  __shared__ R4 trec[IMDG_TILE_CAP];
+  __shared__ int4 truns[IMDG_MAXRUN];
+  __shared__ R off[3 * IMDG_NCODES];
+  __shared__ double red[IMDG_NWAVE];
   if (a.ctl->frozen) return;
   int lb = a.xcd_remap ? logical_block() : (int)blockIdx.x;
   int r = a.blk_rep[lb];
   const RepDev& Rp = a.rep[r];
-  load_shifts(Rp, sh);
-  __syncthreads();
+  load_offsets(Rp, off);
+  const R4* __restrict__ reca = a.rec_a;
+  int nrun;
+  bool tok = stage_tile(a.tiles, lb, reca, trec, truns, &nrun) != 0;
+  const R4* __restrict__ src = tok ? (const R4*)trec : reca;
   int i = lb * IMDG_BLOCK + threadIdx.x;
   bool active = (i - Rp.atom_off) < Rp.natoms;
-  const double4* __restrict__ pos = a.pos[a.ctl->pos_cur];
   const Seg<R>* __restrict__ segs = a.seg;
   const TabDev<R>& tr = tabs[1];
   const TabDev<R>& te = tabs[2];
   double emb = 0.0;
   int is_short = 0;
   if (active) {
-    double4 xi = pos[i];
-    int it = (int)xi.w;
+    R4 ui = reca[i];
+    int it = NT1 ? 0 : (int)ui.w;
     int ng = (a.nnb[i] + 3) >> 2;
     R rho = (R)0;
     const uint4* __restrict__ nb = list_of(a.nbr, i, a.kmax);
@@ -542,15 +726,13 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
       for (int g = 0; g < ng; g++) {
         uint4 e4 = nb[(size_t)g * IMDG_WAVE];
         uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
-        double4 xj[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) xj[u] = pos[en[u] & IMDG_IDX_MASK];
         R r2[4], chi[4];
         int k[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
+          R4 uj = src[en[u] & IMDG_IDX_MASK];
           R dx, dy, dz;
-          separation(xi, xj[u], sh + 3 * (en[u] >> IMDG_IDX_BITS), dx, dy, dz);
+          separation(ui, uj, off, en[u] >> IMDG_IDX_BITS, dx, dy, dz);
           r2[u] = dx * dx + dy * dy + dz * dz;
           k[u] = seg_locate(r2[u], b, e, is, len, chi[u]);
         }
@@ -559,9 +741,8 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
         for (int u = 0; u < 4; u++) c[u] = seg[k[u]];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          R v = c[u].p0 + chi[u] * (c[u].dv + (R)0.5 * (chi[u] - (R)1) * c[u].d2v);
           bool in = r2[u] < e;
-          rho += in ? v : (R)0;
+          rho += in ? seg_val(c[u], chi[u]) : (R)0;
           is_short |= (in && r2[u] < b);
         }
       }
@@ -570,19 +751,16 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
       for (int g = 0; g < ng; g++) {
         uint4 e4 = nb[(size_t)g * IMDG_WAVE];
         uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
-        double4 xj[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) xj[u] = pos[en[u] & IMDG_IDX_MASK];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
+          R4 uj = src[en[u] & IMDG_IDX_MASK];
           R dx, dy, dz;
-          separation(xi, xj[u], sh + 3 * (en[u] >> IMDG_IDX_BITS), dx, dy, dz);
+          separation(ui, uj, off, en[u] >> IMDG_IDX_BITS, dx, dy, dz);
           R r2 = dx * dx + dy * dy + dz * dz;
-          int col = it * nt + (int)xj[u].w;
+          int col = it * nt + (int)uj.w;
           R e = tr.end[col], b = tr.begin[col], chi;
           int k = seg_locate(r2, b, e, tr.invstep[col], tr.len[col], chi);
-          Seg<R> c = segs[tr.off[col] + k];
-          R v = c.p0 + chi * (c.dv + (R)0.5 * (chi - (R)1) * c.d2v);
+          R v = seg_val(segs[tr.off[col] + k], chi);
           bool in = r2 < e;
           rho += in ? v : (R)0;
           is_short |= (in && r2 < b);
@@ -594,8 +772,13 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
     int idummy = 0;
     tab_val_grad(segs + te.off[it], te.len[it], te.begin[it], te.end[it],
                  te.invstep[it], te.twoistep[it], rho, F, dF, idummy);
+    R4 rb;
+    rb.x = ui.x;
+    rb.y = ui.y;
+    rb.z = ui.z;
+    rb.w = dF;
+    a.rec_b[i] = rb;
     a.rho[i] = rho;
-    a.df[i] = dF;
     a.epot[i] = F;
     emb = (double)F;
   }
@@ -608,9 +791,8 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
 // Finalisation by the last block of a step kernel: compensated sums of the
 // block partials of every replica, thermostat, skin decision, thermo record.
 template <typename R>
-__device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved,
-                              bool flip) {
-  __shared__ double lred[IMDG_BLOCK / IMDG_WAVE][PB_N + 1][2];
+__device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved) {
+  __shared__ double lred[IMDG_NWAVE][PB_N + 1][2];
   int w = threadIdx.x / IMDG_WAVE, l = threadIdx.x % IMDG_WAVE;
   bool any = false;
   for (int r = 0; r < a.nrep; r++) {
@@ -659,7 +841,7 @@ __device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved,
       for (int q = 0; q <= PB_N; q++) {
         double ss = lred[0][q][0], cc = lred[0][q][1];
 #pragma unroll
-        for (int k = 1; k < IMDG_BLOCK / IMDG_WAVE; k++) {
+        for (int k = 1; k < IMDG_NWAVE; k++) {
           if (q == PB_MAXD) {
             ss = fmax(ss, lred[k][q][0]);
           } else {
@@ -708,15 +890,13 @@ __device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved,
       a.ctl->any_rebuild = any ? 1 : 0;
       a.ctl->steps_done += 1;
     }
-    if (flip) a.ctl->pos_cur ^= 1;
     __threadfence();
   }
 }
 
 template <typename R>
 __device__ __forceinline__ void last_block_finalize(StepArgs<R>& a,
-                                                    bool forces, bool moved,
-                                                    bool flip) {
+                                                    bool forces, bool moved) {
   __shared__ int is_last;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -728,7 +908,7 @@ __device__ __forceinline__ void last_block_finalize(StepArgs<R>& a,
   __syncthreads();
   if (is_last) {
     __threadfence();
-    finalize_step(a, forces, moved, flip);
+    finalize_step(a, forces, moved);
   }
 }
 
@@ -771,39 +951,78 @@ __device__ __forceinline__ double disp2(double4 a, double4 b) {
   return d0 * d0 + d1 * d1 + d2 * d2;
 }
 
+// Integrate slot i, refresh its cell-relative record and return the
+// partials (kinetic terms, displacement).
+template <typename R>
+__device__ __forceinline__ void integrate_slot(StepArgs<R>& a,
+                                               const RepDev& Rp, int r, int i,
+                                               double fx, double fy, double fz,
+                                               double* v) {
+  typedef typename V4<R>::T R4;
+  int pc = a.ctl->pos_cur;
+  double4* pos = a.pos[pc];
+  double4* mom = a.mom[pc];
+  double4 xi = pos[i];
+  double4 p = mom[i];
+  double x0 = xi.x, x1 = xi.y, x2 = xi.z, ke, e1, e2;
+  leapfrog(a.ensemble, a.dt, a.rst[r].eta, fx, fy, fz, p, x0, x1, x2, ke, e1,
+           e2);
+  mom[i] = p;
+  double4 xn = make_double4(x0, x1, x2, xi.w);
+  pos[i] = xn;
+  double c[3];
+  cell_centre(Rp, a.acell[i], c);
+  R4 ra;
+  ra.x = (R)(x0 - c[0]);
+  ra.y = (R)(x1 - c[1]);
+  ra.z = (R)(x2 - c[2]);
+  ra.w = (R)xi.w;
+  a.rec_a[i] = ra;
+  v[PB_KE] = ke;
+  v[PB_E1] = e1;
+  v[PB_E2] = e2;
+  v[PB_MAXD] = disp2(xn, a.xnbl[i]);
+}
+
 // Pair + EAM forces (src/imd_forces_nbl.c:490-584, 1117-1322) as a gather
-// over the full list; with INTEGRATE the leapfrog step, the skin test and
-// the step's reductions follow in the same thread (calc_forces, move_atoms
-// and check_nblist of one main_loop iteration in one launch).
+// over the full list from the block's LDS tile of (u, 2F') records; with
+// INTEGRATE the leapfrog step, the skin test and the step's reductions
+// follow in the same thread (calc_forces, move_atoms and check_nblist of one
+// main_loop iteration in one launch).
 template <typename R, bool NT1, bool INTEGRATE, bool HP, bool HE>
 __global__ void __launch_bounds__(IMDG_BLOCK)
     k_force(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
   typedef typename V4<R>::T R4;
-  __shared__ double sh[3 * IMDG_NCODES];
-  __shared__ double red[4 * PB_N];
+  __shared__ R4 trec[IMDG_TILE_CAP];
+  __shared__ int4 truns[IMDG_MAXRUN];
+  __shared__ R off[3 * IMDG_NCODES];
+  __shared__ double red[IMDG_NWAVE * PB_N];
   if (a.ctl->frozen) return;
   int lb = a.xcd_remap ? logical_block() : (int)blockIdx.x;
   int r = a.blk_rep[lb];
   const RepDev& Rp = a.rep[r];
-  load_shifts(Rp, sh);
-  __syncthreads();
+  load_offsets(Rp, off);
+  // the gathered records (rec_b): (u, 2F') from the density loop with EAM,
+  // a copy of (u, type) without; the integration below rewrites rec_a
+  const R4* __restrict__ recg = a.rec_b;
+  int nrun;
+  bool tok = stage_tile(a.tiles, lb, recg, trec, truns, &nrun) != 0;
+  const R4* __restrict__ src = tok ? (const R4*)trec : recg;
   int i = lb * IMDG_BLOCK + threadIdx.x;
   bool active = (i - Rp.atom_off) < Rp.natoms;
-  int pc = a.ctl->pos_cur;
-  const double4* __restrict__ pos = a.pos[pc];
   const Seg<R>* __restrict__ segs = a.seg;
-  const R* __restrict__ dfa = a.df;
+  const signed char* __restrict__ atype = a.atype;
   const TabDev<R>& tp = tabs[0];
   const TabDev<R>& tr = tabs[1];
   double v[PB_N] = {0, 0, 0, 0, 0, 0};
   int is_short = 0;
+  R fx = 0, fy = 0, fz = 0;
   if (active) {
-    double4 xi = pos[i];
-    int it = (int)xi.w;
+    R4 ui = recg[i];
+    int it = NT1 ? 0 : (int)atype[i];
     int ng = (a.nnb[i] + 3) >> 2;
-    R fx = 0, fy = 0, fz = 0, ep = 0, vir = 0;
-    bool hp = HP, he = HE;
-    R dFi = he ? dfa[i] : (R)0;
+    R ep = 0, vir = 0;
+    R dFi = HE ? ui.w : (R)0;
     const uint4* __restrict__ nb = list_of(a.nbr, i, a.kmax);
     if (NT1) {
       const Seg<R>* __restrict__ ps = segs + tp.off[0];
@@ -816,20 +1035,14 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
       for (int g = 0; g < ng; g++) {
         uint4 e4 = nb[(size_t)g * IMDG_WAVE];
         uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
-        double4 xj[4];
-        R dfj[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          int j = (int)(en[u] & IMDG_IDX_MASK);
-          xj[u] = pos[j];
-          if constexpr (HE) dfj[u] = dfa[j];
-        }
-        R dx[4], dy[4], dz[4], r2[4], cp[4], cr[4];
+        R dx[4], dy[4], dz[4], r2[4], cp[4], cr[4], dfj[4];
         int kp[4], kr[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          separation(xi, xj[u], sh + 3 * (en[u] >> IMDG_IDX_BITS), dx[u],
-                     dy[u], dz[u]);
+          R4 uj = src[en[u] & IMDG_IDX_MASK];
+          separation(ui, uj, off, en[u] >> IMDG_IDX_BITS, dx[u], dy[u],
+                     dz[u]);
+          dfj[u] = uj.w;
           r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
           if constexpr (HP) kp[u] = seg_locate(r2[u], pb, pe, pis, plen, cp[u]);
           if constexpr (HE) kr[u] = seg_locate(r2[u], rb, re, ris, rlen, cr[u]);
@@ -844,18 +1057,15 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
         for (int u = 0; u < 4; u++) {
           R gg = (R)0;
           if constexpr (HP) {
-            R c = cp[u];
-            R pot = sp[u].p0 + c * (sp[u].dv + (R)0.5 * (c - (R)1) * sp[u].d2v);
-            R gp = p2is * (sp[u].dv + (c - (R)0.5) * sp[u].d2v);
             bool inp = r2[u] <= pe;
-            gg += inp ? gp : (R)0;
-            ep += inp ? pot : (R)0;
+            gg += inp ? seg_grad(sp[u], cp[u], p2is) : (R)0;
+            ep += inp ? seg_val(sp[u], cp[u]) : (R)0;
             is_short |= (inp && r2[u] < pb);
           }
           if constexpr (HE) {
-            R rp = r2is * (sr[u].dv + (cr[u] - (R)0.5) * sr[u].d2v);
             bool inr = r2[u] < re;
-            gg += inr ? (R)0.5 * (dFi + dfj[u]) * rp : (R)0;
+            gg += inr ? (R)0.5 * (dFi + dfj[u]) * seg_grad(sr[u], cr[u], r2is)
+                      : (R)0;
             is_short |= (inr && r2[u] < rb);
           }
           fx += dx[u] * gg;
@@ -869,19 +1079,19 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
       for (int g = 0; g < ng; g++) {
         uint4 e4 = nb[(size_t)g * IMDG_WAVE];
         uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
-        double4 xj[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) xj[u] = pos[en[u] & IMDG_IDX_MASK];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          int j = (int)(en[u] & IMDG_IDX_MASK);
+          int jidx = (int)(en[u] & IMDG_IDX_MASK);
+          R4 uj = src[jidx];
           R dx, dy, dz;
-          separation(xi, xj[u], sh + 3 * (en[u] >> IMDG_IDX_BITS), dx, dy, dz);
+          separation(ui, uj, off, en[u] >> IMDG_IDX_BITS, dx, dy, dz);
           R r2 = dx * dx + dy * dy + dz * dz;
-          int jt = (int)xj[u].w;
+          // the list holds tile positions; the runs map them to slots
+          int js = tok ? tile_slot(truns, nrun, jidx) : jidx;
+          int jt = (int)atype[js];
           int col = it * nt + jt, colT = jt * nt + it;
           R g = 0;
-          if (hp && r2 <= tp.end[col]) {
+          if (HP && r2 <= tp.end[col]) {
             R pot, gp;
             tab_val_grad(segs + tp.off[col], tp.len[col], tp.begin[col],
                          tp.end[col], tp.invstep[col], tp.twoistep[col], r2,
@@ -889,19 +1099,22 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
             ep += pot;
             g += gp;
           }
-          if (he && (r2 < tr.end[colT] || r2 < tr.end[col])) {
+          if (HE && (r2 < tr.end[colT] || r2 < tr.end[col])) {
             // rho'_i from column jt*nt+it, rho'_j from it*nt+jt
             // (src/imd_forces_nbl.c:1156-1203)
-            R rpi = tab_grad(segs + tr.off[colT], tr.len[colT],
-                             tr.begin[colT], tr.end[colT], tr.invstep[colT],
-                             tr.twoistep[colT], r2, is_short);
-            R rpj = (colT == col)
-                        ? rpi
-                        : tab_grad(segs + tr.off[col], tr.len[col],
-                                   tr.begin[col], tr.end[col],
-                                   tr.invstep[col], tr.twoistep[col], r2,
-                                   is_short);
-            g += (R)0.5 * (dFi * rpj + dfa[j] * rpi);
+            R chi;
+            int k = seg_locate(r2, tr.begin[colT], tr.end[colT],
+                               tr.invstep[colT], tr.len[colT], chi);
+            is_short |= (r2 < tr.begin[colT]);
+            R rpi = seg_grad(segs[tr.off[colT] + k], chi, tr.twoistep[colT]);
+            R rpj = rpi;
+            if (colT != col) {
+              k = seg_locate(r2, tr.begin[col], tr.end[col], tr.invstep[col],
+                             tr.len[col], chi);
+              is_short |= (r2 < tr.begin[col]);
+              rpj = seg_grad(segs[tr.off[col] + k], chi, tr.twoistep[col]);
+            }
+            g += (R)0.5 * (dFi * rpj + uj.w * rpi);
           }
           fx += dx * g;
           fy += dy * g;
@@ -911,45 +1124,34 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
       }
     }
     // per-atom energy: half of every pair term plus F(rho_i)
-    R ei = (R)0.5 * ep + (he ? a.epot[i] : (R)0);
+    R ei = (R)0.5 * ep + (HE ? a.epot[i] : (R)0);
     a.epot[i] = ei;
     R4 fv;
     fv.x = fx;
     fv.y = fy;
     fv.z = fz;
     fv.w = (R)0;
-    reinterpret_cast<R4*>(a.frc)[i] = fv;
+    a.frc[i] = fv;
     v[PB_EPAIR] = 0.5 * (double)ep;
     v[PB_VIRIAL] = -0.5 * (double)vir;
-    if (INTEGRATE) {
-      double4* mom = a.mom[a.ctl->rec_cur];
-      double4 p = mom[i];
-      double x0 = xi.x, x1 = xi.y, x2 = xi.z, ke, e1, e2;
-      leapfrog(a.ensemble, a.dt, a.rst[r].eta, (double)fx, (double)fy,
-               (double)fz, p, x0, x1, x2, ke, e1, e2);
-      mom[i] = p;
-      double4 xn = make_double4(x0, x1, x2, xi.w);
-      a.pos[pc ^ 1][i] = xn;
-      v[PB_KE] = ke;
-      v[PB_E1] = e1;
-      v[PB_E2] = e2;
-      v[PB_MAXD] = disp2(xn, a.xnbl[i]);
-    }
+    if (INTEGRATE)
+      integrate_slot(a, Rp, r, i, (double)fx, (double)fy, (double)fz, v);
   }
   block_reduce<PB_N>(v, 1u << PB_MAXD, red);
   if (threadIdx.x == 0) {
     double* pbk = a.partB + (size_t)lb * PB_N;
+#pragma unroll
     for (int q = 0; q < PB_N; q++) pbk[q] = v[q];
   }
   if (is_short) atomicOr(&a.ctl->is_short, 1);
-  last_block_finalize(a, true, INTEGRATE, INTEGRATE);
+  last_block_finalize(a, true, INTEGRATE);
 }
 
 // move_atoms + check_nblist on the stored forces (the unfused entry point).
 template <typename R>
 __global__ void __launch_bounds__(IMDG_BLOCK) k_move(StepArgs<R> a) {
   typedef typename V4<R>::T R4;
-  __shared__ double red[4 * PB_N];
+  __shared__ double red[IMDG_NWAVE * PB_N];
   if (a.ctl->frozen) return;
   int lb = a.xcd_remap ? logical_block() : (int)blockIdx.x;
   int r = a.blk_rep[lb];
@@ -958,21 +1160,8 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_move(StepArgs<R> a) {
   bool active = (i - Rp.atom_off) < Rp.natoms;
   double v[PB_N] = {0, 0, 0, 0, 0, 0};
   if (active) {
-    double4* pos = a.pos[a.ctl->pos_cur];
-    double4* mom = a.mom[a.ctl->rec_cur];
-    double4 xi = pos[i];
-    double4 p = mom[i];
-    R4 f = reinterpret_cast<const R4*>(a.frc)[i];
-    double x0 = xi.x, x1 = xi.y, x2 = xi.z, ke, e1, e2;
-    leapfrog(a.ensemble, a.dt, a.rst[r].eta, (double)f.x, (double)f.y,
-             (double)f.z, p, x0, x1, x2, ke, e1, e2);
-    mom[i] = p;
-    double4 xn = make_double4(x0, x1, x2, xi.w);
-    pos[i] = xn;
-    v[PB_KE] = ke;
-    v[PB_E1] = e1;
-    v[PB_E2] = e2;
-    v[PB_MAXD] = disp2(xn, a.xnbl[i]);
+    R4 f = a.frc[i];
+    integrate_slot(a, Rp, r, i, (double)f.x, (double)f.y, (double)f.z, v);
   }
   block_reduce<PB_N>(v, 1u << PB_MAXD, red);
   if (threadIdx.x == 0) {
@@ -983,28 +1172,38 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_move(StepArgs<R> a) {
     pbk[PB_E2] = v[PB_E2];
     pbk[PB_MAXD] = v[PB_MAXD];
   }
-  last_block_finalize(a, false, true, false);
+  last_block_finalize(a, false, true);
 }
 
-// check_nblist alone: max displacement -> rebuild flags.
+// Pair-only systems: the force loop gathers rec_b, which holds a copy of the
+// (u, type) records (rec_a is rewritten by the integration in that loop).
+template <typename R>
+__global__ void __launch_bounds__(256) k_copyrec(StepArgs<R> a, int nslots) {
+  if (a.ctl->frozen) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nslots) a.rec_b[i] = a.rec_a[i];
+}
+
+// check_nblist alone: max displacement per block.
 __global__ void __launch_bounds__(IMDG_BLOCK)
-    k_check(const RepDev* rep, RepState* rst, Ctl* ctl, const int* blk_rep,
-            double4* const* posp, const double4* xnbl, double* maxd,
-            int nrep) {
-  __shared__ double red[4];
+    k_check(const RepDev* rep, Ctl* ctl, const int* blk_rep,
+            const double4* pos0, const double4* pos1, const double4* xnbl,
+            double* maxd) {
+  __shared__ double red[IMDG_NWAVE];
   if (ctl->frozen) return;
   int lb = blockIdx.x;
   int r = blk_rep[lb];
   const RepDev& Rp = rep[r];
   int i = lb * IMDG_BLOCK + threadIdx.x;
+  const double4* pos = ctl->pos_cur ? pos1 : pos0;
   double v[1] = {0.0};
-  if ((i - Rp.atom_off) < Rp.natoms) v[0] = disp2(posp[ctl->pos_cur][i], xnbl[i]);
+  if ((i - Rp.atom_off) < Rp.natoms) v[0] = disp2(pos[i], xnbl[i]);
   block_reduce<1>(v, 1u, red);
   if (threadIdx.x == 0) maxd[lb] = v[0];
 }
 
 // ---------------------------------------------------------------------------
-// explicit instantiations reachable from the host side
+// launchers
 // ---------------------------------------------------------------------------
 
 template <typename R>
@@ -1022,18 +1221,26 @@ template <typename R, bool INTEGRATE>
 void launch_force_i(const StepArgs<R>& a, const TabDev<R>* tabs, bool nt1,
                     int nblk, hipStream_t st) {
   dim3 g(nblk), b(IMDG_BLOCK);
-  if (!nt1)
-    hipLaunchKernelGGL((k_force<R, false, INTEGRATE, true, true>), g, b, 0, st,
-                       a, tabs);
-  else if (a.has_pair && a.has_eam)
+  if (!nt1) {
+    if (a.has_pair && a.has_eam)
+      hipLaunchKernelGGL((k_force<R, false, INTEGRATE, true, true>), g, b, 0,
+                         st, a, tabs);
+    else if (a.has_pair)
+      hipLaunchKernelGGL((k_force<R, false, INTEGRATE, true, false>), g, b, 0,
+                         st, a, tabs);
+    else
+      hipLaunchKernelGGL((k_force<R, false, INTEGRATE, false, true>), g, b, 0,
+                         st, a, tabs);
+  } else if (a.has_pair && a.has_eam) {
     hipLaunchKernelGGL((k_force<R, true, INTEGRATE, true, true>), g, b, 0, st,
                        a, tabs);
-  else if (a.has_pair)
+  } else if (a.has_pair) {
     hipLaunchKernelGGL((k_force<R, true, INTEGRATE, true, false>), g, b, 0, st,
                        a, tabs);
-  else
+  } else {
     hipLaunchKernelGGL((k_force<R, true, INTEGRATE, false, true>), g, b, 0, st,
                        a, tabs);
+  }
 }
 
 template <typename R>
@@ -1065,21 +1272,32 @@ template void launch_force<double>(const StepArgs<double>&,
 template void launch_move<float>(const StepArgs<float>&, int, hipStream_t);
 template void launch_move<double>(const StepArgs<double>&, int, hipStream_t);
 
+template <typename R>
+void launch_copyrec(const StepArgs<R>& a, int nslots, hipStream_t st) {
+  hipLaunchKernelGGL((k_copyrec<R>), dim3((nslots + 255) / 256), dim3(256), 0,
+                     st, a, nslots);
+}
+template void launch_copyrec<float>(const StepArgs<float>&, int, hipStream_t);
+template void launch_copyrec<double>(const StepArgs<double>&, int,
+                                     hipStream_t);
+
 void launch_build(const BuildArgs& b, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_bin, dim3(grid), dim3(IMDG_BLOCK), 0, st, b);
+  hipLaunchKernelGGL(k_bin, dim3(grid), dim3(256), 0, st, b);
   hipLaunchKernelGGL(k_scan, dim3(b.nrep), dim3(1024), 0, st, b);
-  hipLaunchKernelGGL(k_place, dim3(grid), dim3(IMDG_BLOCK), 0, st, b);
-  hipLaunchKernelGGL(k_cellsort, dim3(grid), dim3(IMDG_BLOCK), 0, st, b);
-  hipLaunchKernelGGL(k_permute, dim3(grid), dim3(IMDG_BLOCK), 0, st, b);
-  hipLaunchKernelGGL(k_list, dim3(grid), dim3(IMDG_BLOCK), 0, st, b);
+  hipLaunchKernelGGL(k_place, dim3(grid), dim3(256), 0, st, b);
+  hipLaunchKernelGGL(k_cellsort, dim3(grid), dim3(256), 0, st, b);
+  hipLaunchKernelGGL(k_permute, dim3(grid), dim3(256), 0, st, b);
+  int gt = (b.nblk + 63) / 64;
+  hipLaunchKernelGGL(k_tiles, dim3(gt < 1 ? 1 : gt), dim3(64), 0, st, b);
+  hipLaunchKernelGGL(k_list, dim3(b.nblk), dim3(IMDG_BLOCK), 0, st, b);
 }
 
-void launch_check(const RepDev* rep, RepState* rst, Ctl* ctl,
-                  const int* blk_rep, double4* const* posp,
-                  const double4* xnbl, double* maxd, int nrep, int nblk,
+void launch_check(const RepDev* rep, Ctl* ctl, const int* blk_rep,
+                  const double4* pos0, const double4* pos1,
+                  const double4* xnbl, double* maxd, int nblk,
                   hipStream_t st) {
-  hipLaunchKernelGGL(k_check, dim3(nblk), dim3(IMDG_BLOCK), 0, st, rep, rst,
-                     ctl, blk_rep, posp, xnbl, maxd, nrep);
+  hipLaunchKernelGGL(k_check, dim3(nblk), dim3(IMDG_BLOCK), 0, st, rep, ctl,
+                     blk_rep, pos0, pos1, xnbl, maxd);
 }
 
 }  // namespace imdg

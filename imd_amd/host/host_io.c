@@ -1,0 +1,243 @@
+/*
+ * host_io.c -- configurations, velocity initialisation and thermo output.
+ *
+ * read_atoms: IMD's configuration format (src/imd_io_3d.c:44-520): header
+ *   lines "#F A n t m np nv nd" (numbers of number/type/mass/position/
+ *   velocity/data columns, src/imd_io.c:3369-3497), "#X/#Y/#Z" box vectors
+ *   (src/imd_io.c:3318-3360) and "#E"; ASCII rows
+ *   "number type [mass] x y z [vx vy vz] [data...]"; the type is taken
+ *   modulo ntypes, the mass from the row or the masses key, p = m v, and no
+ *   velocity columns request Maxwell initialisation.
+ * generate_fcc: sites at (x + 0.5) a / 2 for integer x, y, z with even sum
+ *   in box_param unit cells of edge box_unit (src/imd_generate.c:306-486).
+ * maxwell: momenta N(0, sqrt(kT m)) per component from drand48 with the
+ *   polar method, then the mean momentum removed (src/imd_maxwell.c:44-323),
+ *   in input order.
+ * write_eng_file: one line per eng_int steps with time, Epot per atom,
+ *   temperature 2 Ekin / nactive, pressure T/(V/N) + virial/(3V), volume per
+ *   atom and, for NVT, eta * tau_eta (src/imd_io.c:2262-2431).
+ */
+#define _XOPEN_SOURCE 600
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "imd_host.h"
+
+long natoms = 0;
+int nactive = 0;
+double *atom_x = NULL, *atom_p = NULL, *atom_mass = NULL;
+int *atom_type = NULL;
+long *atom_number = NULL;
+double tot_pot_energy = 0.0, tot_kin_energy = 0.0, virial = 0.0, volume = 0.0;
+int nbl_count = 0;
+
+static FILE *eng_file = NULL;
+
+static int grow_atoms(long cap) {
+  atom_x = realloc(atom_x, sizeof(double) * 3 * cap);
+  atom_p = realloc(atom_p, sizeof(double) * 3 * cap);
+  atom_mass = realloc(atom_mass, sizeof(double) * cap);
+  atom_type = realloc(atom_type, sizeof(int) * cap);
+  atom_number = realloc(atom_number, sizeof(long) * cap);
+  return (atom_x && atom_p && atom_mass && atom_type && atom_number) ? 0 : 1;
+}
+
+static double box_volume(void) {
+  double c[3] = {box_y[1] * box_z[2] - box_y[2] * box_z[1],
+                 box_y[2] * box_z[0] - box_y[0] * box_z[2],
+                 box_y[0] * box_z[1] - box_y[1] * box_z[0]};
+  return fabs(box_x[0] * c[0] + box_x[1] * c[1] + box_x[2] * c[2]);
+}
+
+int read_atoms(const char *filename) {
+  char line[4096];
+  int have_format = 0, n_num = 1, n_type = 1, n_mass = 1, n_pos = 3,
+      n_vel = 3, n_data = 0;
+  char fmt = 'A';
+  FILE *f = fopen(filename, "r");
+  if (!f) imd_error("cannot open input file %s", filename);
+  long pos_after_header = 0;
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] != '#') break;
+    if (line[1] == 'F') {
+      char s[16];
+      if (sscanf(line + 2, "%15s %d %d %d %d %d %d", s, &n_num, &n_type,
+                 &n_mass, &n_pos, &n_vel, &n_data) < 7)
+        imd_error("Format line of file %s corrupt", filename);
+      fmt = s[0];
+      have_format = 1;
+    } else if (line[1] == 'X') {
+      if (box_from_header || box_x[0] == 0.0)
+        sscanf(line + 2, "%lf %lf %lf", &box_x[0], &box_x[1], &box_x[2]);
+    } else if (line[1] == 'Y') {
+      if (box_from_header || box_y[1] == 0.0)
+        sscanf(line + 2, "%lf %lf %lf", &box_y[0], &box_y[1], &box_y[2]);
+    } else if (line[1] == 'Z') {
+      if (box_from_header || box_z[2] == 0.0)
+        sscanf(line + 2, "%lf %lf %lf", &box_z[0], &box_z[1], &box_z[2]);
+    } else if (line[1] == 'E') {
+      pos_after_header = ftell(f);
+      break;
+    }
+  }
+  if (!have_format) imd_error("configuration %s has no #F header", filename);
+  if (fmt != 'A')
+    imd_error("configuration format %c not supported (ASCII 'A' only)", fmt);
+  if (n_num != 1 || n_type != 1 || n_pos != 3)
+    imd_error("configuration %s: need number, type and 3 positions", filename);
+  fseek(f, pos_after_header, SEEK_SET);
+  long cap = 1024;
+  natoms = 0;
+  if (grow_atoms(cap)) imd_error("cannot allocate atoms");
+  int n_items = n_num + n_type + n_mass + n_pos + n_vel + n_data;
+  int missing_vel = (n_vel == 0);
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    double d[32];
+    int num, typ;
+    char *s = line, *e;
+    num = (int)strtol(s, &e, 10);
+    if (e == s) continue; /* empty line */
+    s = e;
+    typ = (int)strtol(s, &e, 10);
+    if (e == s) imd_error("incorrect line in configuration file.");
+    s = e;
+    int nd = 0;
+    while (nd < 32) {
+      double v = strtod(s, &e);
+      if (e == s) break;
+      d[nd++] = v;
+      s = e;
+    }
+    int p = nd + 2;
+    if (p < n_items && p != n_items - n_vel)
+      imd_error("incorrect line in configuration file.");
+    if (natoms == cap) {
+      cap *= 2;
+      if (grow_atoms(cap)) imd_error("cannot allocate atoms");
+    }
+    int k = 0;
+    int t = ((typ % ntypes) + ntypes) % ntypes;
+    double m = n_mass ? d[k++] : masses[t];
+    if (!(m > 0)) imd_error("Mass zero or negative.");
+    atom_number[natoms] = num;
+    atom_type[natoms] = t;
+    atom_mass[natoms] = m;
+    for (int c = 0; c < 3; c++) atom_x[3 * natoms + c] = d[k++];
+    if (n_vel && p == n_items) {
+      for (int c = 0; c < 3; c++) atom_p[3 * natoms + c] = d[k++] * m;
+    } else {
+      for (int c = 0; c < 3; c++) atom_p[3 * natoms + c] = 0.0;
+      missing_vel = 1;
+    }
+    natoms++;
+  }
+  fclose(f);
+  if (missing_vel) do_maxwell = 1;
+  nactive = (int)(3 * natoms);
+  volume = box_volume();
+  return 0;
+}
+
+int generate_fcc(void) {
+  int nx = box_param[0], ny = box_param[1], nz = box_param[2];
+  if (nx <= 0 || ny <= 0 || nz <= 0) imd_error("box_param not set");
+  double a = box_unit;
+  memset(box_x, 0, sizeof box_x);
+  memset(box_y, 0, sizeof box_y);
+  memset(box_z, 0, sizeof box_z);
+  box_x[0] = nx * a;
+  box_y[1] = ny * a;
+  box_z[2] = nz * a;
+  long cap = 4L * nx * ny * nz;
+  if (grow_atoms(cap)) imd_error("cannot allocate atoms");
+  natoms = 0;
+  for (int x = 0; x < 2 * nx; x++)
+    for (int y = 0; y < 2 * ny; y++)
+      for (int z = 0; z < 2 * nz; z++) {
+        if ((x + y + z) % 2) continue;
+        atom_x[3 * natoms] = (x + 0.5) * a / 2;
+        atom_x[3 * natoms + 1] = (y + 0.5) * a / 2;
+        atom_x[3 * natoms + 2] = (z + 0.5) * a / 2;
+        atom_type[natoms] = 0;
+        atom_mass[natoms] = masses[0];
+        atom_number[natoms] = natoms;
+        atom_p[3 * natoms] = atom_p[3 * natoms + 1] = atom_p[3 * natoms + 2] = 0;
+        natoms++;
+      }
+  nactive = (int)(3 * natoms);
+  volume = box_volume();
+  do_maxwell = 1;
+  return 0;
+}
+
+/* polar (Box-Muller) method, Knuth vol. 2, 3rd ed., p. 122 */
+static double gaussian(double sigma) {
+  double x, y, r2;
+  do {
+    x = -1 + 2 * drand48();
+    y = -1 + 2 * drand48();
+    r2 = x * x + y * y;
+  } while (r2 > 1.0 || r2 == 0);
+  return sigma * y * sqrt(-2.0 * log(r2) / r2);
+}
+
+void maxwell(double temp) {
+  double tot[3] = {0, 0, 0};
+  srand48(seed);
+  for (long i = 0; i < natoms; i++) {
+    double s = sqrt(temp * atom_mass[i]);
+    for (int c = 0; c < 3; c++) {
+      atom_p[3 * i + c] = gaussian(s);
+      tot[c] += atom_p[3 * i + c];
+    }
+  }
+  for (int c = 0; c < 3; c++) tot[c] = natoms ? tot[c] / natoms : 0.0;
+  for (long i = 0; i < natoms; i++)
+    for (int c = 0; c < 3; c++) atom_p[3 * i + c] -= tot[c];
+}
+
+int write_config(const char *filename) {
+  FILE *f = fopen(filename, "w");
+  if (!f) return 1;
+  fprintf(f, "#F A 1 1 1 3 3 0\n#C number type mass x y z vx vy vz\n");
+  fprintf(f, "#X %.16e %.16e %.16e\n", box_x[0], box_x[1], box_x[2]);
+  fprintf(f, "#Y %.16e %.16e %.16e\n", box_y[0], box_y[1], box_y[2]);
+  fprintf(f, "#Z %.16e %.16e %.16e\n", box_z[0], box_z[1], box_z[2]);
+  fprintf(f, "#E\n");
+  for (long i = 0; i < natoms; i++) {
+    double m = atom_mass[i];
+    fprintf(f, "%ld %d %.16e %.16e %.16e %.16e %.16e %.16e %.16e\n",
+            atom_number[i], atom_type[i], m, atom_x[3 * i], atom_x[3 * i + 1],
+            atom_x[3 * i + 2], atom_p[3 * i] / m, atom_p[3 * i + 1] / m,
+            atom_p[3 * i + 2] / m);
+  }
+  fclose(f);
+  return 0;
+}
+
+int write_eng_file_header(void) {
+  char fname[IMD_STRLEN + 8];
+  snprintf(fname, sizeof fname, "%s.eng", outfilename);
+  eng_file = fopen(fname, "w");
+  if (!eng_file) imd_error("Cannot open properties file %s", fname);
+  fprintf(eng_file, "# time Epot temperature pressure volume%s\n",
+          ensemble == IMDG_NVT ? " eta*tau_eta" : "");
+  return 0;
+}
+
+int write_eng_file(int steps) {
+  if (!eng_file) write_eng_file_header();
+  double Epot = tot_pot_energy / natoms;
+  double Temp = 2.0 * tot_kin_energy / nactive;
+  double vol = volume / natoms;
+  double pressure = Temp / vol + virial / (3 * volume);
+  fprintf(eng_file, "%e %.18e %e %e %e", steps * timestep, Epot, Temp,
+          pressure, vol);
+  if (ensemble == IMDG_NVT) fprintf(eng_file, " %e", eta * tau_eta);
+  fprintf(eng_file, "\n");
+  fflush(eng_file);
+  return 0;
+}

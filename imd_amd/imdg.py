@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(PKG, "libimdg.so")
 
 FP64, MIXED = 0, 1
 NVE, NVT = 0, 1
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_GEOM, ERR_NOTABLE, ERR_STATE = range(7)
 
 SYMBOLS = [
     "imdg_ctx_create", "imdg_ctx_destroy", "imdg_last_error",
@@ -27,7 +28,7 @@ SYMBOLS = [
     "imdg_deallocate_nblist", "imdg_run", "imdg_sync", "imdg_get_atoms",
     "imdg_get_thermo", "imdg_get_thermo_history", "imdg_get_cells",
     "imdg_get_pairs", "imdg_get_grid", "imdg_get_info", "imdg_profile",
-    "imdg_kernel_times",
+    "imdg_kernel_times", "imdg_struct_size",
 ]
 
 
@@ -65,7 +66,8 @@ class Info(C.Structure):
                 ("kmax", C.c_int), ("kmax_seen", C.c_int),
                 ("nbl_count", C.c_long), ("steps_done", C.c_long),
                 ("is_short", C.c_int), ("lanes_per_atom", C.c_int),
-                ("list_bytes", C.c_double), ("mean_k", C.c_double)]
+                ("list_bytes", C.c_double), ("mean_k", C.c_double),
+                ("tiles_fallback", C.c_int)]
 
 
 _lib = None

@@ -115,6 +115,8 @@ typedef struct {
   int    lanes_per_atom;
   double list_bytes;    /* bytes of list storage */
   double mean_k;        /* mean full-list length per real atom */
+  int    tiles_fallback;/* blocks whose neighbour tile did not fit in LDS
+                           (their neighbours are gathered from HBM) */
 } imdg_info;
 
 /* lifecycle */
@@ -192,6 +194,12 @@ int  imdg_get_info(imdg_ctx *ctx, imdg_info *info);
 int  imdg_profile(imdg_ctx *ctx, int enable);
 int  imdg_kernel_times(imdg_ctx *ctx, const char **names, double *ms,
                        long *launches, int n);
+
+/* sizeof() of a public struct by name ("imdg_config", "imdg_table",
+   "imdg_replica", "imdg_thermo", "imdg_info"), -1 for an unknown name: lets
+   a binding in another language check its layouts against the library's.
+   No IMD counterpart (IMD links its force module statically). */
+long imdg_struct_size(const char *name);
 
 #ifdef __cplusplus
 }

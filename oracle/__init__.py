@@ -214,7 +214,7 @@ class OracleSystem:
 
     def scalar(self, which):
         names = dict(epot=0, virial=1, ekin=2, eta=3, cellsz=4, volume=5,
-                     is_short=6, nbl_count=7, natoms=8)
+                     is_short=6, nbl_count=7, natoms=8, virial_abs=9)
         return lib().orc_get_scalar(self._s, names.get(which, which))
 
     def grid(self):

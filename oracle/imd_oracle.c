@@ -284,6 +284,9 @@ struct orc_sys {
   long tbn, tbcap;
   int have_valid_nbl, nbl_count, is_short;
   double tot_pot_energy, virial, tot_kin_energy;
+  /* sum of |terms| of the virial: the scale of its rounding error (a
+     tolerance for reduced-precision results, not part of IMD) */
+  double virial_abs;
 };
 
 static void vec_prod(const double *u, const double *v, double *w) {
@@ -679,6 +682,7 @@ int orc_calc_forces(orc_sys *s, int steps) {
   }
   s->tot_pot_energy = 0.0;
   s->virial = 0.0;
+  s->virial_abs = 0.0;
   for (int q = 0; q < s->nslot; q++) {
     s->sf[3 * q] = s->sf[3 * q + 1] = s->sf[3 * q + 2] = 0.0;
     s->sepot[q] = 0.0;
@@ -713,6 +717,7 @@ int orc_calc_forces(orc_sys *s, int steps) {
         ee += pot;
         s->sepot[js] += pot;
         s->virial -= r2 * grad;
+        s->virial_abs += fabs(r2 * grad);
       }
       if (rt) {
         double rho_h = 0.0;
@@ -778,6 +783,7 @@ int orc_calc_forces(orc_sys *s, int steps) {
           ff[1] += fc[1];
           ff[2] += fc[2];
           s->virial -= d[0] * fc[0] + d[1] * fc[1] + d[2] * fc[2];
+          s->virial_abs += fabs(d[0] * fc[0] + d[1] * fc[1] + d[2] * fc[2]);
         }
       }
       s->sf[3 * ia] += ff[0];
@@ -905,6 +911,7 @@ double orc_get_scalar(const orc_sys *s, int which) {
     case 6: return s->is_short;
     case 7: return s->nbl_count;
     case 8: return s->n;
+    case 9: return s->virial_abs;
     default: return 0.0;
   }
 }

@@ -111,7 +111,8 @@ int  orc_run(orc_sys *s, int nsteps, double *thermo);
 void orc_get_atoms(const orc_sys *s, double *x, double *p, double *f,
                    double *epot, double *rho, double *df);
 /* 0 tot_pot_energy, 1 virial, 2 tot_kin_energy, 3 eta, 4 cellsz (list),
-   5 volume, 6 is_short flag, 7 nbl_count (list builds), 8 natoms */
+   5 volume, 6 is_short flag, 7 nbl_count (list builds), 8 natoms,
+   9 sum of |virial terms| (scale for reduced-precision tolerances) */
 double orc_get_scalar(const orc_sys *s, int which);
 void orc_get_grid(const orc_sys *s, int *gdim3);
 /* Cell coordinates (global, 3 ints per atom) from the last rebuild. */

@@ -39,7 +39,10 @@ def _compare_static(gpu, orc, n, tol_f, tol_s, has_eam=True):
     th = gpu.thermo()
     e_o, v_o = orc.scalar("epot"), orc.scalar("virial")
     assert abs(th["epot"] - e_o) <= tol_s * abs(e_o)
-    assert abs(th["virial"] - v_o) <= max(tol_s * abs(v_o), 1e-9 * n)
+    # the virial is a sum of pair terms of both signs that mostly cancel
+    # (sum |terms| / |sum| is about 1e3 in a crystal); its error is bounded
+    # relative to the sum of |terms|, the scale its rounding works on
+    assert abs(th["virial"] - v_o) <= tol_s * orc.scalar("virial_abs")
 
 
 @pytest.mark.parametrize("precision", [G.FP64, G.MIXED])
