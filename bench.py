@@ -33,11 +33,18 @@ METRIC = "atom-steps/s, 2M-atom EAM: mixed FP32/FP64 vs plain FP64 on 1 GPU"
 
 
 def algorithmic_bytes(kind: str, k: float, mixed: bool) -> float:
-    """Bytes per atom that one launch of the kernel must move (each per-atom
-    array once, list indices int32 read once; neighbour gathers count as
-    on-chip reuse).  Mixed: density 4K+24, force 4K+36, integrate+skin 152;
-    FP64: 4K+48, 4K+72, 148.  The fused force kernel includes the
-    integrate+skin bytes."""
+    """Algorithmic bytes per atom of one launch: every per-atom array the
+    kernel must touch once, the int32 full-list entries once (k = mean list
+    length); neighbour record gathers count as on-chip reuse.
+
+    mixed: density 4k+24 (16 B FP32 xyz+type in, 4 B 2F' out, 4 B energy
+      out); force 4k+36 (16 B record, 4 B 2F', 12 B FP32 force, 4 B
+      energy); integrate + skin test 152 (12 B force, 48 B x read/write,
+      48 B p read/write, 4 B type, 16 B FP32 record out, 24 B x at the last
+      build).
+    FP64: density 4k+48, force 4k+72, integrate + skin test 148.
+    The fused force kernel carries force + integrate: 4k+188 | 4k+220.
+    """
     if kind == "density":
         return 4 * k + (24 if mixed else 48)
     if kind == "force":
@@ -73,6 +80,27 @@ def cpu_baseline(eam, s, steps: int, tau_fs: float, temp_k: float):
                        % (steps, s.n))
 
 
+def copy_probe(mib: int) -> int:
+    """One device-to-device copy of a known size through the HIP runtime
+    (a plain copy kernel): the calibration run for the HBM traffic counters
+    (FETCH_SIZE / WRITE_SIZE of the copy against its known byte count)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(b, a, C.c_size_t(n), 3)  # device to device
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": n, "rc": rc}))
+    return rc
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,9 +114,14 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB "
+                         "(counter calibration) and exit")
     ap.add_argument("--compare", action="store_true",
                     help="also time the other precision (ratio in stderr)")
     args = ap.parse_args(argv)
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
 
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU "

@@ -24,7 +24,7 @@ LIB = os.path.join(PKG, "libimdg.so")
 HOSTLIB = os.path.join(HOST, "libimdhost.so")
 HOSTEXE = os.path.join(HOST, "imd_gpu")
 
-HIP_SOURCES = ["imdg_kernels.hip", "imdg_api.hip"]
+HIP_SOURCES = ["imdg_build.hip", "imdg_step.hip", "imdg_api.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -52,12 +52,15 @@ def _run(cmd, cwd=None):
 
 
 def build_lib(verbose: bool = False) -> str:
-    objs = []
-    for src in HIP_SOURCES:
-        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
-        _run([hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-c",
-              os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+    from concurrent.futures import ThreadPoolExecutor
+    objs = [os.path.join(CSRC, src.replace(".hip", ".o")) for src in HIP_SOURCES]
+    cmds = [[hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-c",
+             os.path.join(CSRC, src), "-o", obj]
+            for src, obj in zip(HIP_SOURCES, objs)]
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out.strip():
+                print(out)
     tmp = LIB + ".tmp"
     _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp,
           *objs])

@@ -17,15 +17,22 @@
 
 namespace imdg {
 template <typename R>
-void launch_density(const StepArgs<R>&, const TabDev<R>*, bool, int,
-                    hipStream_t);
+int step_grid_density(bool nt1, bool lt, size_t lds, int ncu);
 template <typename R>
-void launch_force(const StepArgs<R>&, const TabDev<R>*, bool, bool, int,
-                  hipStream_t);
+int step_grid_force(int mode, bool integrate, bool lt, bool hp, bool he,
+                    size_t lds, int ncu);
 template <typename R>
-void launch_move(const StepArgs<R>&, int, hipStream_t);
+int step_grid_move(int ncu);
 template <typename R>
-void launch_copyrec(const StepArgs<R>&, int, hipStream_t);
+hipError_t launch_density(const StepArgs<R>&, const TabDev<R>*, bool, int,
+                          hipStream_t);
+template <typename R>
+hipError_t launch_force(const StepArgs<R>&, const TabDev<R>*, int, bool,
+                        int, hipStream_t);
+template <typename R>
+hipError_t launch_move(const StepArgs<R>&, int, hipStream_t);
+template <typename R>
+hipError_t launch_copyrec(const StepArgs<R>&, int, hipStream_t);
 void launch_build(const BuildArgs&, int, hipStream_t);
 void launch_check(const RepDev*, Ctl*, const int*, const double4*,
                   const double4*, const double4*, double*, int, hipStream_t);
@@ -71,7 +78,7 @@ struct imdg_ctx {
   std::vector<int> natoms;      // per replica
   std::vector<long> in_off;     // input index of the first atom of a replica
   long natoms_tot = 0;
-  int nslots = 0, nblk = 0, ncells = 0;
+  int nslots = 0, nblk = 0, ncells = 0, nstile = 0;
   int kmax = 0;
   // device memory
   std::vector<DevBuf> bufs;
@@ -86,6 +93,8 @@ struct imdg_ctx {
       *d_perm = nullptr;
   signed char* d_atype = nullptr;
   int *d_cell_cnt = nullptr, *d_cell_beg = nullptr, *d_cell_num = nullptr;
+  int *d_cell_loc = nullptr, *d_tile_sum = nullptr, *d_tile_off = nullptr,
+      *d_tile_rep = nullptr, *d_blkmax = nullptr;
   Tile* d_tiles = nullptr;
   uint32_t* d_nbr = nullptr;
   size_t nbr_bytes = 0;
@@ -95,7 +104,19 @@ struct imdg_ctx {
   double *d_partA = nullptr, *d_partB = nullptr, *d_maxd = nullptr;
   ThermoRec* d_ring = nullptr;
   void* d_tabs = nullptr;        // TabDev<R>[3]
-  void* d_seg = nullptr;         // segments of all tables
+  void* d_knots = nullptr;       // knots of all tables
+  // launch plan of the persistent step kernels
+  int ncu = 0;
+  int nchunk = 0;
+  int part_chunk = 0;
+  int knots_pair = 0, knots_rho = 0;   // first knot of the rho / embed tables
+  int knots_embed = 0;
+  int knots_joint = -1;                // interleaved (pair, rho), one species
+  int force_mode = 0;                  // 0 one species, 1 shared grid, 2 several
+  int lds_den_lo = 0, lds_den_n = 0;   // knots staged by the density loop
+  int lds_frc_lo = 0, lds_frc_n = 0;   // knots staged by the force loop
+  int grid_den = 0, grid_frc[2] = {0, 0}, grid_move = 0;
+  int units_max = 0;
   bool atoms_set = false;
   long target_steps = 0;
   // profiling
@@ -187,46 +208,130 @@ int copy_table(imdg_ctx* c, HostTable& h, const imdg_table* t, int ncols,
   return IMDG_OK;
 }
 
-// Segment coefficients of IMD's 3-point interpolation: rows k, k+1, k+2 of
-// a column give p0, dv = p1 - p0, d2v = p2 - 2 p1 + p0 (src/potaccess.h:
-// 345-349), computed in FP64 and then rounded to R.
+// Knots of IMD's 3-point interpolation: for rows k = 0..len of a column,
+// (p0_k, dv_k) = (t[k], t[k+1] - t[k]); segment k reads knots k and k+1 and
+// takes d2v = dv_k+1 - dv_k = t[k+2] - 2 t[k+1] + t[k] (src/potaccess.h:
+// 345-349).  Differences in FP64, then rounded to R.  Tables are stored
+// pair, rho, embed, so the radial tables form one contiguous knot range.
 template <typename R>
 int upload_tables(imdg_ctx* c) {
-  std::vector<Seg<R>> segs;
+  typedef typename V2<R>::T R2;
+  std::vector<R2> kn;
   TabDev<R> td[3];
   memset(td, 0, sizeof td);
+  int first[4] = {0, 0, 0, 0};
   for (int t = 0; t < 3; t++) {
     HostTable& h = c->tab[t];
+    first[t] = (int)kn.size();
     td[t].ncols = h.present ? h.ncols : 0;
     if (!h.present) continue;
     for (int col = 0; col < h.ncols; col++) {
-      td[t].off[col] = (int)segs.size();
+      td[t].off[col] = (int)kn.size();
       td[t].len[col] = h.len[col];
       td[t].begin[col] = (R)h.begin[col];
       td[t].end[col] = (R)h.end[col];
       td[t].invstep[col] = (R)h.invstep[col];
       td[t].twoistep[col] = (R)(2 * h.invstep[col]);
       int nc = h.ncols;
-      for (int k = 0; k < h.len[col]; k++) {
+      for (int k = 0; k <= h.len[col]; k++) {
         double p0 = h.table[(size_t)k * nc + col];
         double p1 = h.table[(size_t)(k + 1) * nc + col];
-        double p2 = h.table[(size_t)(k + 2) * nc + col];
-        Seg<R> s;
-        s.p0 = (R)p0;
-        s.dv = (R)(p1 - p0);
-        s.d2v = (R)(p2 - 2 * p1 + p0);
-        s.pad = (R)0;
-        segs.push_back(s);
+        R2 q;
+        q.x = (R)p0;
+        q.y = (R)(p1 - p0);
+        kn.push_back(q);
       }
     }
   }
-  int rc = dalloc(c, &c->d_seg, segs.size() * sizeof(Seg<R>));
+  first[3] = (int)kn.size();
+  c->knots_pair = first[0];
+  c->knots_rho = first[1];
+  c->knots_embed = first[2];
+  // one species with pair and rho on the same grid: also an interleaved
+  // table J[k] = (phi p0, phi dv, rho p0, rho dv), at an even knot index so
+  // that it can be read as 16 B (FP32) / 32 B (FP64) records
+  c->knots_joint = -1;
+  const HostTable &hp = c->tab[0], &hr = c->tab[1];
+  bool same = c->cfg.ntypes == 1 && hp.present && hr.present &&
+              hp.begin[0] == hr.begin[0] && hp.end[0] == hr.end[0] &&
+              hp.invstep[0] == hr.invstep[0] && hp.len[0] == hr.len[0];
+  if (const char* e = getenv("IMDG_JOINT_TABLE"))
+    if (atoi(e) == 0) same = false;
+  if (same) {
+    if (kn.size() & 1) kn.push_back(kn.back());
+    c->knots_joint = (int)kn.size();
+    for (int k = 0; k <= hp.len[0]; k++) {
+      kn.push_back(kn[td[0].off[0] + k]);
+      kn.push_back(kn[td[1].off[0] + k]);
+    }
+  }
+  int rc = dalloc(c, &c->d_knots, kn.size() * sizeof(R2));
   if (rc) return rc;
-  HIPCHK(c, hipMemcpy(c->d_seg, segs.data(), segs.size() * sizeof(Seg<R>),
+  HIPCHK(c, hipMemcpy(c->d_knots, kn.data(), kn.size() * sizeof(R2),
                       hipMemcpyHostToDevice));
   rc = dalloc(c, &c->d_tabs, sizeof td);
   if (rc) return rc;
   HIPCHK(c, hipMemcpy(c->d_tabs, td, sizeof td, hipMemcpyHostToDevice));
+  return IMDG_OK;
+}
+
+// Launch plan of the persistent step kernels: which knots each neighbour
+// loop stages in LDS (all knots of the tables it reads, when they fit next
+// to the kernel's static LDS) and the resident grid of each kernel.
+template <typename R>
+int plan_steps(imdg_ctx* c) {
+  typedef typename V2<R>::T R2;
+  if (!c->ncu) {
+    HIPCHK(c, hipDeviceGetAttribute(&c->ncu,
+                                    hipDeviceAttributeMultiprocessorCount,
+                                    c->dev));
+  }
+  bool nt1 = c->cfg.ntypes == 1;
+  bool hp = c->tab[0].present, he = c->tab[1].present;
+  c->force_mode = !nt1 ? 2 : (c->knots_joint >= 0 ? 1 : 0);
+  int mode = c->force_mode;
+  int no_lds = 0;
+  if (const char* e = getenv("IMDG_TABLES_IN_LDS")) no_lds = (atoi(e) == 0);
+  // density: the rho knots
+  c->lds_den_lo = c->knots_rho;
+  c->lds_den_n = he ? c->knots_embed - c->knots_rho : 0;
+  c->grid_den = 0;
+  if (he && !no_lds)
+    c->grid_den = step_grid_density<R>(nt1, true, c->lds_den_n * sizeof(R2),
+                                       c->ncu);
+  if (he && c->grid_den == 0) {
+    c->lds_den_n = 0;
+    c->grid_den = step_grid_density<R>(nt1, false, 0, c->ncu);
+  }
+  // force: pair and rho knots (or the interleaved table)
+  if (mode == 1) {
+    c->lds_frc_lo = c->knots_joint;
+    c->lds_frc_n = 2 * (c->tab[0].len[0] + 1);
+  } else {
+    c->lds_frc_lo = hp ? c->knots_pair : c->knots_rho;
+    c->lds_frc_n = c->knots_embed - c->lds_frc_lo;
+  }
+  for (int integ = 0; integ < 2; integ++) {
+    int g = 0;
+    if (!no_lds)
+      g = step_grid_force<R>(mode, integ, true, hp, he,
+                             c->lds_frc_n * sizeof(R2), c->ncu);
+    if (g == 0) {
+      c->lds_frc_n = 0;
+      g = step_grid_force<R>(mode, integ, false, hp, he, 0, c->ncu);
+      if (integ == 1)  // keep both variants on the same table placement
+        c->grid_frc[0] =
+            step_grid_force<R>(mode, 0, false, hp, he, 0, c->ncu);
+    }
+    c->grid_frc[integ] = g;
+  }
+  c->grid_move = step_grid_move<R>(c->ncu);
+  if (c->grid_frc[0] == 0 || c->grid_frc[1] == 0 || c->grid_move == 0 ||
+      (he && c->grid_den == 0))
+    return fail(c, IMDG_ERR_HIP, "cannot size the persistent step kernels");
+  int gmax = std::max(std::max(c->grid_den, c->grid_move),
+                      std::max(c->grid_frc[0], c->grid_frc[1]));
+  c->units_max = gmax * IMDG_STEP_WAVES;
   return IMDG_OK;
 }
 
@@ -248,10 +353,10 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.atype = c->d_atype;
   a.rec_a = (R4*)c->d_reca;
   a.rec_b = (R4*)c->d_recb;
-  a.tiles = c->d_tiles;
   a.nbr = c->d_nbr;
   a.nnb = c->d_nnb;
-  a.seg = (const Seg<R>*)c->d_seg;
+  a.knots = (const typename V2<R>::T*)c->d_knots;
+  a.joint = c->knots_joint;
   a.rho = (R*)c->d_rho;
   a.epot = (R*)c->d_epot;
   a.frc = (R4*)c->d_frc;
@@ -259,7 +364,9 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.partB = c->d_partB;
   a.ring = c->d_ring;
   a.nrep = c->nrep;
-  a.nblk = c->nblk;
+  a.nchunk = c->nchunk;
+  a.part_chunk = c->part_chunk;
+  a.unitsA = c->grid_den * (IMDG_DENSITY_THREADS / IMDG_WAVE);
   a.kmax = c->kmax;
   a.ntypes = c->cfg.ntypes;
   a.ensemble = c->cfg.ensemble;
@@ -294,6 +401,11 @@ BuildArgs build_args(imdg_ctx* c) {
   b.cell_cnt = c->d_cell_cnt;
   b.cell_beg = c->d_cell_beg;
   b.cell_num = c->d_cell_num;
+  b.cell_loc = c->d_cell_loc;
+  b.tile_sum = c->d_tile_sum;
+  b.tile_off = c->d_tile_off;
+  b.tile_rep = c->d_tile_rep;
+  b.blkmax = c->d_blkmax;
   b.tiles = c->d_tiles;
   b.nbr = c->d_nbr;
   b.nnb = c->d_nnb;
@@ -301,6 +413,7 @@ BuildArgs build_args(imdg_ctx* c) {
   b.nslots = c->nslots;
   b.nblk = c->nblk;
   b.ncells = c->ncells;
+  b.nstile = c->nstile;
   b.kmax = c->kmax;
   return b;
 }
@@ -362,28 +475,36 @@ int enqueue_build(imdg_ctx* c) {
 }
 
 template <typename R>
-void enqueue_forces_t(imdg_ctx* c, bool integrate) {
+hipError_t enqueue_forces_t(imdg_ctx* c, bool integrate) {
   StepArgs<R> a = step_args<R>(c);
   const TabDev<R>* t = (const TabDev<R>*)c->d_tabs;
   bool nt1 = c->cfg.ntypes == 1;
+  hipError_t e;
   {
     // the force loop gathers (u, 2F') records: from the density loop with
     // EAM, a copy of the (u, type) records without
     ProfScope ps(c, KC_DENSITY);
-    if (a.has_eam)
-      launch_density<R>(a, t, nt1, c->nblk, c->stream);
-    else
-      launch_copyrec<R>(a, c->nslots, c->stream);
+    if (a.has_eam) {
+      a.lds_lo = c->lds_den_lo;
+      a.lds_n = c->lds_den_n;
+      e = launch_density<R>(a, t, nt1, c->grid_den, c->stream);
+    } else {
+      e = launch_copyrec<R>(a, c->nslots, c->stream);
+    }
+    if (e != hipSuccess) return e;
   }
   ProfScope ps(c, KC_FORCE);
-  launch_force<R>(a, t, nt1, integrate, c->nblk, c->stream);
+  a.lds_lo = c->lds_frc_lo;
+  a.lds_n = c->lds_frc_n;
+  return launch_force<R>(a, t, c->force_mode, integrate,
+                         c->grid_frc[integrate ? 1 : 0], c->stream);
 }
 
 int enqueue_forces(imdg_ctx* c, bool integrate) {
   if (c->mixed)
-    enqueue_forces_t<float>(c, integrate);
+    HIPCHK(c, enqueue_forces_t<float>(c, integrate));
   else
-    enqueue_forces_t<double>(c, integrate);
+    HIPCHK(c, enqueue_forces_t<double>(c, integrate));
   HIPCHK(c, hipGetLastError());
   return IMDG_OK;
 }
@@ -573,7 +694,7 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
   c->natoms.assign(natoms, natoms + nrep);
   c->in_off.assign(nrep, 0);
   long tot = 0;
-  int slot = 0, cell = 0;
+  int slot = 0, cell = 0, stile = 0;
   double kest = 0.0;
   for (int r = 0; r < nrep; r++) {
     RepDev& R = c->rep[r];
@@ -631,6 +752,9 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
     R.ncells = R.gdim[0] * R.gdim[1] * R.gdim[2];
     R.blk_off = slot / IMDG_BLOCK;
     R.nblk = R.nslots / IMDG_BLOCK;
+    R.stile_off = stile;
+    R.nstile = (R.ncells + IMDG_SCAN_TILE - 1) / IMDG_SCAN_TILE;
+    stile += R.nstile;
     R.nactive = 3 * natoms[r];
     c->in_off[r] = tot;
     tot += natoms[r];
@@ -646,6 +770,7 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
   c->nslots = slot;
   c->nblk = slot / IMDG_BLOCK;
   c->ncells = cell;
+  c->nstile = stile;
   c->kmax = std::max(16, (int)ceil(kest * c->cfg.nbl_size * 1.25));
   c->kmax = (c->kmax + 7) & ~7;
 
@@ -672,6 +797,11 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
   ALLOC(c->d_cell_cnt, sizeof(int) * c->ncells);
   ALLOC(c->d_cell_beg, sizeof(int) * c->ncells);
   ALLOC(c->d_cell_num, sizeof(int) * c->ncells);
+  ALLOC(c->d_cell_loc, sizeof(int) * c->ncells);
+  ALLOC(c->d_tile_sum, sizeof(int) * c->nstile);
+  ALLOC(c->d_tile_off, sizeof(int) * c->nstile);
+  ALLOC(c->d_tile_rep, sizeof(int) * c->nstile);
+  ALLOC(c->d_blkmax, sizeof(int) * c->nblk);
   ALLOC(c->d_tiles, sizeof(Tile) * c->nblk);
   ALLOC(c->d_nnb, sizeof(int) * ns);
   ALLOC(c->d_reca, rs * 4 * ns);
@@ -679,17 +809,25 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
   ALLOC(c->d_rho, rs * ns);
   ALLOC(c->d_epot, rs * ns);
   ALLOC(c->d_frc, rs * 4 * ns);
-  ALLOC(c->d_partA, sizeof(double) * c->nblk);
-  ALLOC(c->d_partB, sizeof(double) * PB_N * c->nblk);
   ALLOC(c->d_maxd, sizeof(double) * c->nblk);
   ALLOC(c->d_ring, sizeof(ThermoRec) * IMDG_RING * nrep);
-#undef ALLOC
   if ((rc = c->mixed ? upload_tables<float>(c) : upload_tables<double>(c)))
     return rc;
+  c->nchunk = c->nslots / IMDG_CHUNK;
+  c->part_chunk = nrep > 1 ? 1 : 0;
+  if ((rc = c->mixed ? plan_steps<float>(c) : plan_steps<double>(c)))
+    return rc;
+  {
+    size_t units = std::max(c->nchunk, c->units_max);
+    ALLOC(c->d_partA, sizeof(double) * units);
+    ALLOC(c->d_partB, sizeof(double) * PB_N * units);
+  }
+#undef ALLOC
 
   // host staging in slot order
   std::vector<double4> hpos(ns), hmom(ns);
-  std::vector<int> haid(ns, -1), hblk(c->nblk), hcrep(c->ncells);
+  std::vector<int> haid(ns, -1), hblk(c->nblk), hcrep(c->ncells),
+      htrep(c->nstile);
   for (size_t k = 0; k < ns; k++) {
     hpos[k] = make_double4(0, 0, 0, -1.0);
     hmom[k] = make_double4(0, 0, 0, 1.0);
@@ -698,6 +836,7 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
     RepDev& R = c->rep[r];
     for (int b = 0; b < R.nblk; b++) hblk[R.blk_off + b] = r;
     for (int q = 0; q < R.ncells; q++) hcrep[R.cell_off + q] = r;
+    for (int q = 0; q < R.nstile; q++) htrep[R.stile_off + q] = r;
     for (int k = 0; k < R.natoms; k++) {
       long in = c->in_off[r] + k;
       int t = type[in];
@@ -741,6 +880,8 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
                            hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->d_cell_rep, hcrep.data(),
                            sizeof(int) * c->ncells, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->d_tile_rep, htrep.data(),
+                           sizeof(int) * c->nstile, hipMemcpyHostToDevice, st));
   for (int k = 0; k < 2; k++) {
     HIPCHK(c, hipMemcpyAsync(c->d_pos[k], hpos.data(), sizeof(double4) * ns,
                              hipMemcpyHostToDevice, st));
@@ -804,9 +945,11 @@ int imdg_move_atoms(imdg_ctx* c) {
   {
     ProfScope ps(c, KC_MOVE);
     if (c->mixed)
-      launch_move<float>(step_args<float>(c), c->nblk, c->stream);
+      HIPCHK(c, launch_move<float>(step_args<float>(c), c->grid_move,
+                                   c->stream));
     else
-      launch_move<double>(step_args<double>(c), c->nblk, c->stream);
+      HIPCHK(c, launch_move<double>(step_args<double>(c), c->grid_move,
+                                    c->stream));
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1064,13 +1207,10 @@ int imdg_get_pairs(imdg_ctx* c, long* out, long cap, long* npairs) {
   if (rc) return rc;
   size_t ns = c->nslots;
   std::vector<int> hn(ns), hc(ns);
-  std::vector<Tile> tiles(c->nblk);
   std::vector<uint32_t> hl(c->nbr_bytes / 4);
   HIPCHK(c, hipMemcpy(hn.data(), c->d_nnb, sizeof(int) * ns,
                       hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(hc.data(), c->d_acell, sizeof(int) * ns,
-                      hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(tiles.data(), c->d_tiles, sizeof(Tile) * c->nblk,
                       hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(hl.data(), c->d_nbr, c->nbr_bytes, hipMemcpyDeviceToHost));
   long np = 0;
@@ -1078,7 +1218,6 @@ int imdg_get_pairs(imdg_ctx* c, long* out, long cap, long* npairs) {
     const RepDev& R = c->rep[r];
     for (int k = 0; k < R.natoms; k++) {
       int s = R.atom_off + k;
-      const Tile& T = tiles[s / IMDG_BLOCK];
       long ii = v.aid[s] - c->in_off[r];
       int lc = hc[s] - R.cell_off;
       int ci[3] = {lc / (R.gdim[1] * R.gdim[2]), (lc / R.gdim[2]) % R.gdim[1],
@@ -1090,11 +1229,6 @@ int imdg_get_pairs(imdg_ctx* c, long* out, long cap, long* npairs) {
         int idx = (int)(e & IMDG_IDX_MASK), code = (int)(e >> IMDG_IDX_BITS);
         if (code == IMDG_CODE_PAD) continue;
         int j = idx;
-        if (T.ok) {
-          int q = 0;
-          while (q + 1 < T.nrun && T.run[q + 1].z <= idx) q++;
-          j = T.run[q].x + (idx - T.run[q].z);
-        }
         long jj = v.aid[j] - c->in_off[r];
         int o[3] = {code / 9 - 1, (code / 3) % 3 - 1, code % 3 - 1}, sh[3];
         for (int d = 0; d < 3; d++) {

@@ -1,0 +1,697 @@
+// imdg_build.hip -- CDNA4 (gfx950) kernels of the list rebuild: wrap and
+// cell binning, per-replica scan, cell sort by input index, permutation of
+// the state into cell order, per-block neighbour tiles and the full
+// neighbour list.  Every kernel of the chain is launched every step and
+// returns at its first instructions unless the skin test of the previous
+// step (or the host) asked for a rebuild, so the fused loop never waits for
+// the host.
+//
+// Reference semantics (paths relative to the reference root):
+//   binning / wrap     src/imd_fix_cells_3d.c:36-201, src/imd_main_3d.c:1972-2065,
+//                      cell_coord src/imd_geom_3d.c:1054-1074
+//   list               make_nblist src/imd_forces_nbl.c:136-273 (half list over
+//                      the 14-cell stencil of src/imd_geom_3d.c:895-899 with
+//                      buffer-cell images src/imd_comm_force_3d.c:222-397);
+//                      built here as a full list whose r^2 is decided in the
+//                      same direction and operation order as the reference, so
+//                      the pair set is identical
+//   skin test          check_nblist src/imd_forces_nbl.c:2007-2037
+#include "imdg_device.h"
+
+namespace imdg {
+
+// Blocks of the per-slot rebuild kernels cover IMDG_BLOCK slots of one
+// replica, so the replica's geometry is wave-uniform (scalar loads).
+__device__ __forceinline__ int block_replica(const int* blk_rep) {
+  return __builtin_amdgcn_readfirstlane(blk_rep[blockIdx.x]);
+}
+
+// fix_cells/do_boundaries + cell_coord, for replicas that need a rebuild.
+__global__ void __launch_bounds__(IMDG_BLOCK) k_bin(BuildArgs a) {
+#pragma clang fp contract(off)
+  if (!a.ctl->any_rebuild || a.ctl->frozen) return;
+  int r = block_replica(a.blk_rep);
+  if (!a.rst[r].need_rebuild) return;
+  const RepDev& R = a.rep[r];
+  int i = blockIdx.x * IMDG_BLOCK + threadIdx.x;
+  if (i - R.atom_off >= R.natoms) return;
+  double4* pos = a.pos[a.ctl->pos_cur];
+  double4 p = pos[i];
+  double x[3] = {p.x, p.y, p.z};
+  for (int d = 0; d < 3; d++) {
+    if (R.pbc[d] != 1) continue;
+    double f = -floor(x[0] * R.tbox[d][0] + x[1] * R.tbox[d][1] +
+                      x[2] * R.tbox[d][2]);
+    x[0] += f * R.box[d][0];
+    x[1] += f * R.box[d][1];
+    x[2] += f * R.box[d][2];
+  }
+  pos[i] = make_double4(x[0], x[1], x[2], p.w);
+  int c[3];
+  for (int d = 0; d < 3; d++) {
+    c[d] = (int)(R.gdim[d] * (x[0] * R.tbox[d][0] + x[1] * R.tbox[d][1] +
+                              x[2] * R.tbox[d][2]));
+    if (c[d] >= R.gdim[d])
+      c[d] = R.gdim[d] - 1;
+    else if (c[d] < 0)
+      c[d] = 0;
+  }
+  int cell = R.cell_off + (c[0] * R.gdim[1] + c[1]) * R.gdim[2] + c[2];
+  a.cellid[i] = cell;
+  a.cslot[i] = atomicAdd(&a.cell_cnt[cell], 1);
+}
+
+// Exclusive scan of the cell counts, two levels in one launch: every block
+// scans one tile of IMDG_SCAN_TILE cells of a replica (cell_loc: offset
+// inside the tile, tile_sum: the tile's total); the last block to finish
+// scans the tile totals of every rebuilt replica (tile_off: first slot of
+// the tile's first cell).
+__global__ void __launch_bounds__(IMDG_SCAN_TILE) k_scan(BuildArgs a) {
+  __shared__ int wsum[IMDG_SCAN_TILE / IMDG_WAVE];
+  __shared__ int is_last;
+  if (!a.ctl->any_rebuild || a.ctl->frozen) return;
+  int t = blockIdx.x;
+  int r = __builtin_amdgcn_readfirstlane(a.tile_rep[t]);
+  const RepDev& R = a.rep[r];
+  int lane = threadIdx.x % IMDG_WAVE, w = threadIdx.x / IMDG_WAVE;
+  if (a.rst[r].need_rebuild) {
+    int lc = (t - R.stile_off) * IMDG_SCAN_TILE + threadIdx.x;
+    int cell = R.cell_off + lc;
+    int n = lc < R.ncells ? a.cell_cnt[cell] : 0;
+    // inclusive scan inside the wave, then over the wave totals
+    int v = n;
+#pragma unroll
+    for (int o = 1; o < IMDG_WAVE; o <<= 1) {
+      int y = __shfl_up(v, o, IMDG_WAVE);
+      if (lane >= o) v += y;
+    }
+    if (lane == IMDG_WAVE - 1) wsum[w] = v;
+    __syncthreads();
+    if (w == 0) {
+      int s = lane < IMDG_SCAN_TILE / IMDG_WAVE ? wsum[lane] : 0;
+#pragma unroll
+      for (int o = 1; o < IMDG_WAVE; o <<= 1) {
+        int y = __shfl_up(s, o, IMDG_WAVE);
+        if (lane >= o) s += y;
+      }
+      if (lane < IMDG_SCAN_TILE / IMDG_WAVE) wsum[lane] = s;
+    }
+    __syncthreads();
+    int before = (w > 0 ? wsum[w - 1] : 0) + v - n;
+    if (lc < R.ncells) a.cell_loc[cell] = before;
+    if (threadIdx.x == IMDG_SCAN_TILE - 1)
+      a.tile_sum[t] = wsum[IMDG_SCAN_TILE / IMDG_WAVE - 1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned d = atomicAdd(&a.ctl->done[DC_SCAN], 1u);
+    is_last = (d == gridDim.x - 1);
+    if (is_last) a.ctl->done[DC_SCAN] = 0;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  // one wavefront per replica: exclusive scan of its tile totals
+  for (int q = w; q < a.nrep; q += IMDG_SCAN_TILE / IMDG_WAVE) {
+    if (!a.rst[q].need_rebuild) continue;
+    const RepDev& Q = a.rep[q];
+    int run = Q.atom_off;
+    for (int k0 = 0; k0 < Q.nstile; k0 += IMDG_WAVE) {
+      int k = k0 + lane;
+      int n = k < Q.nstile ? __atomic_load_n(&a.tile_sum[Q.stile_off + k],
+                                              __ATOMIC_RELAXED)
+                           : 0;
+      int v = n;
+#pragma unroll
+      for (int o = 1; o < IMDG_WAVE; o <<= 1) {
+        int y = __shfl_up(v, o, IMDG_WAVE);
+        if (lane >= o) v += y;
+      }
+      if (k < Q.nstile) a.tile_off[Q.stile_off + k] = run + v - n;
+      run += __shfl(v, IMDG_WAVE - 1, IMDG_WAVE);
+    }
+  }
+}
+
+// First slot of a cell of a rebuilt replica (after k_scan).
+__device__ __forceinline__ int cell_first_slot(const BuildArgs& a,
+                                               const RepDev& R, int cell) {
+  int lc = cell - R.cell_off;
+  return a.tile_off[R.stile_off + lc / IMDG_SCAN_TILE] + a.cell_loc[cell];
+}
+
+__global__ void __launch_bounds__(IMDG_BLOCK) k_place(BuildArgs a) {
+  if (!a.ctl->any_rebuild || a.ctl->frozen) return;
+  int r = block_replica(a.blk_rep);
+  if (!a.rst[r].need_rebuild) return;
+  const RepDev& R = a.rep[r];
+  int i = blockIdx.x * IMDG_BLOCK + threadIdx.x;
+  if (i - R.atom_off >= R.natoms) return;
+  int cell = a.cellid[i];
+  a.perm[cell_first_slot(a, R, cell) + a.cslot[i]] = i;
+}
+
+// Per cell: first slot and count, counter reset, and the atoms of the cell
+// ordered by input index (deterministic list order).
+__global__ void __launch_bounds__(256) k_cellsort(BuildArgs a) {
+  if (!a.ctl->any_rebuild || a.ctl->frozen) return;
+  const int* aid = a.aid[a.ctl->rec_cur];
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < a.ncells;
+       c += gridDim.x * blockDim.x) {
+    int r = a.cell_rep[c];
+    if (!a.rst[r].need_rebuild) continue;
+    const RepDev& R = a.rep[r];
+    int b = cell_first_slot(a, R, c), n = a.cell_cnt[c];
+    a.cell_beg[c] = b;
+    a.cell_num[c] = n;
+    a.cell_cnt[c] = 0;
+    int* p = a.perm + b;
+    for (int k = 1; k < n; k++) {
+      int v = p[k], key = aid[v], m = k - 1;
+      while (m >= 0 && aid[p[m]] > key) {
+        p[m + 1] = p[m];
+        m--;
+      }
+      p[m + 1] = v;
+    }
+  }
+}
+
+// Gather every slot into the other buffers (new cell-sorted order for the
+// rebuilt replicas, identity for the others) and write the cell-relative
+// records of the moved atoms.  The buffers are flipped by k_list_done once
+// the list fits (on an overflow the whole chain runs again from the old
+// buffers).
+__global__ void __launch_bounds__(IMDG_BLOCK) k_permute(BuildArgs a) {
+  if (!a.ctl->any_rebuild || a.ctl->frozen) return;
+  int pc = a.ctl->pos_cur, rc = a.ctl->rec_cur;
+  const double4* pin = a.pos[pc];
+  double4* pout = a.pos[pc ^ 1];
+  const double4* min_ = a.mom[rc];
+  double4* mout = a.mom[rc ^ 1];
+  const int* ain = a.aid[rc];
+  int* aout = a.aid[rc ^ 1];
+  int r = block_replica(a.blk_rep);
+  const RepDev& R = a.rep[r];
+  int i = blockIdx.x * IMDG_BLOCK + threadIdx.x;
+  bool moved = a.rst[r].need_rebuild && (i - R.atom_off < R.natoms);
+  int old = moved ? a.perm[i] : i;
+  double4 p = pin[old];
+  pout[i] = p;
+  mout[i] = min_[old];
+  aout[i] = ain[old];
+  if (moved) {
+    int cell = a.cellid[old];
+    a.acell[i] = cell;
+    a.atype[i] = (signed char)p.w;
+    a.xnbl[i] = p;
+    double c[3];
+    cell_centre(R, cell, c);
+    if (a.mixed)
+      reinterpret_cast<float4*>(a.rec_a)[i] =
+          make_float4((float)(p.x - c[0]), (float)(p.y - c[1]),
+                      (float)(p.z - c[2]), (float)p.w);
+    else
+      reinterpret_cast<double4*>(a.rec_a)[i] =
+          make_double4(p.x - c[0], p.y - c[1], p.z - c[2], p.w);
+  }
+}
+
+// Tiles: for every block of a rebuilt replica, the slot runs of all cells
+// adjacent (27-stencil, periodic wrap, nothing beyond an open face) to the
+// cells of the block's atoms, sorted and merged.  One wavefront per block:
+// lane l handles neighbour column l % 9 of the block's column l / 9 (at
+// most 7 columns; more falls back to a tile-less list), producing at most
+// two slot ranges (a z range split by the periodic wrap); lane 0 sorts and
+// merges them.
+#define TILE_MAXRANGE 126
+#define TILE_MAXCOL 7
+#define TILE_WAVES 4
+
+__global__ void __launch_bounds__(TILE_WAVES * IMDG_WAVE) k_tiles(BuildArgs a) {
+  __shared__ int srb[TILE_WAVES][TILE_MAXRANGE], sre[TILE_WAVES][TILE_MAXRANGE];
+  if (!a.ctl->any_rebuild || a.ctl->frozen) return;
+  int w = threadIdx.x / IMDG_WAVE, l = threadIdx.x % IMDG_WAVE;
+  int b = blockIdx.x * TILE_WAVES + w;
+  if (b >= a.nblk) return;  // whole wavefront; no block barrier below
+  int r = __builtin_amdgcn_readfirstlane(a.blk_rep[b]);
+  if (!a.rst[r].need_rebuild) return;
+  const RepDev& R = a.rep[r];
+  Tile& T = a.tiles[b];
+  int s0 = b * IMDG_BLOCK;
+  int s1 = min(s0 + IMDG_BLOCK, R.atom_off + R.natoms) - 1;
+  if (s1 < s0) {
+    if (l == 0) {
+      T.nrun = 0;
+      T.natoms = 0;
+      T.ok = 1;
+    }
+    return;
+  }
+  int g0 = R.gdim[0], g1 = R.gdim[1], g2 = R.gdim[2];
+  int lc0 = a.acell[s0] - R.cell_off, lc1 = a.acell[s1] - R.cell_off;
+  int col0 = lc0 / g2, col1 = lc1 / g2, ncol = col1 - col0 + 1;
+  bool ok = ncol <= TILE_MAXCOL;
+  int nmine = 0, mb[2] = {0, 0}, me[2] = {0, 0};
+  if (ok && l < 9 * ncol) {
+    int col = col0 + l / 9;
+    int z0 = (col == col0) ? lc0 % g2 : 0;
+    int z1 = (col == col1) ? lc1 % g2 : g2 - 1;
+    int cx = col / g1, cy = col % g1;
+    int ox = (l % 9) / 3 - 1, oy = (l % 9) % 3 - 1;
+    int nx = cx + ox, ny = cy + oy;
+    bool valid = true;
+    if (nx < 0 || nx >= g0) {
+      if (R.pbc[0] != 1)
+        valid = false;
+      else
+        nx = (nx + g0) % g0;
+    }
+    if (ny < 0 || ny >= g1) {
+      if (R.pbc[1] != 1)
+        valid = false;
+      else
+        ny = (ny + g1) % g1;
+    }
+    if (valid) {
+      int za = z0 - 1, zb = z1 + 1, p[2], q[2], np = 0;
+      if (R.pbc[2] == 1) {
+        if (zb - za + 1 >= g2) {
+          p[0] = 0; q[0] = g2 - 1; np = 1;
+        } else if (za < 0) {
+          p[0] = za + g2; q[0] = g2 - 1; p[1] = 0; q[1] = zb; np = 2;
+        } else if (zb >= g2) {
+          p[0] = za; q[0] = g2 - 1; p[1] = 0; q[1] = zb - g2; np = 2;
+        } else {
+          p[0] = za; q[0] = zb; np = 1;
+        }
+      } else {
+        p[0] = za < 0 ? 0 : za;
+        q[0] = zb >= g2 ? g2 - 1 : zb;
+        np = 1;
+      }
+      int cbase = R.cell_off + (nx * g1 + ny) * g2;
+      for (int k = 0; k < np; k++) {
+        int cf = cbase + p[k], cl = cbase + q[k];
+        int rb = a.cell_beg[cf], re = a.cell_beg[cl] + a.cell_num[cl];
+        if (re > rb) {
+          mb[nmine] = rb;
+          me[nmine] = re;
+          nmine++;
+        }
+      }
+    }
+  }
+  // compact the ranges of all lanes into the wavefront's LDS rows
+  int pre = nmine;
+#pragma unroll
+  for (int o = 1; o < IMDG_WAVE; o <<= 1) {
+    int y = __shfl_up(pre, o, IMDG_WAVE);
+    if (l >= o) pre += y;
+  }
+  int nr = __shfl(pre, IMDG_WAVE - 1, IMDG_WAVE);
+  pre -= nmine;
+  for (int k = 0; k < nmine; k++) {
+    srb[w][pre + k] = mb[k];
+    sre[w][pre + k] = me[k];
+  }
+  wave_lds_sync();
+  if (l != 0) return;
+  int* rb = srb[w];
+  int* re = sre[w];
+  // sort ranges by first slot, merge overlapping or touching ones
+  for (int k = 1; k < nr; k++) {
+    int vb = rb[k], ve = re[k], m = k - 1;
+    while (m >= 0 && rb[m] > vb) {
+      rb[m + 1] = rb[m];
+      re[m + 1] = re[m];
+      m--;
+    }
+    rb[m + 1] = vb;
+    re[m + 1] = ve;
+  }
+  int nrun = 0, tot = 0;
+  for (int k = 0; k < nr && ok; k++) {
+    if (nrun > 0 && rb[k] <= T.run[nrun - 1].x + T.run[nrun - 1].y) {
+      int4& u = T.run[nrun - 1];
+      int end = max(u.x + u.y, re[k]);
+      tot += end - (u.x + u.y);
+      u.y = end - u.x;
+    } else {
+      if (nrun >= IMDG_MAXRUN) {
+        ok = false;
+        break;
+      }
+      T.run[nrun] = make_int4(rb[k], re[k] - rb[k], tot, 0);
+      tot += re[k] - rb[k];
+      nrun++;
+    }
+  }
+  if (!ok || tot > IMDG_TILE_CAP) {
+    T.ok = 0;
+    T.nrun = 0;
+    T.natoms = 0;
+    atomicAdd(&a.ctl->tiles_fallback, 1);
+  } else {
+    T.ok = 1;
+    T.nrun = nrun;
+    T.natoms = tot;
+  }
+}
+
+// Image of x shifted by s box vectors, added z, then y, then x: the order in
+// which send_cells builds buffer-cell copies (copy_cell adds the box vector,
+// src/imd_comm_force_3d.c:754-756).
+__device__ __forceinline__ void image_of(const RepDev& R, double x0, double x1,
+                                        double x2, int sx, int sy, int sz,
+                                        double& y0, double& y1, double& y2) {
+#pragma clang fp contract(off)
+  y0 = x0;
+  y1 = x1;
+  y2 = x2;
+  if (sz) {
+    double s = (double)sz;
+    y0 = y0 + s * R.box[2][0];
+    y1 = y1 + s * R.box[2][1];
+    y2 = y2 + s * R.box[2][2];
+  }
+  if (sy) {
+    double s = (double)sy;
+    y0 = y0 + s * R.box[1][0];
+    y1 = y1 + s * R.box[1][1];
+    y2 = y2 + s * R.box[1][2];
+  }
+  if (sx) {
+    double s = (double)sx;
+    y0 = y0 + s * R.box[0][0];
+    y1 = y1 + s * R.box[0][1];
+    y2 = y2 + s * R.box[0][2];
+  }
+}
+
+// Position of a slot inside the block's tile (runs sorted by first slot).
+__device__ __forceinline__ int tile_index(const int4* runs, int nrun, int j) {
+  int lo = 0, hi = nrun - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (runs[mid].x <= j)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return runs[lo].z + (j - runs[lo].x);
+}
+
+// FP32 screening band of the list build: a candidate whose FP32 r^2 from
+// the cell-relative records lies within this relative distance of the list
+// radius is decided by the reference's FP64 expression instead.  The FP32
+// r^2 is off by less than 2e-6 relative (records and offsets rounded once,
+// |u| below a cell diagonal), so everything outside the band is decided
+// exactly as the FP64 expression would decide it.
+#define IMDG_SCREEN_BAND 1.0e-4
+
+// The reference's decision for one candidate: r^2 of (image of the partner)
+// - (owner) in FP64 with separate roundings, owner being the atom whose
+// half stencil holds the other's cell.
+__device__ __forceinline__ bool exact_pair(const RepDev& R,
+                                           const double4* __restrict__ pos,
+                                           int i, int j, int sx, int sy,
+                                           int sz, bool owner) {
+#pragma clang fp contract(off)
+  double4 xi = pos[i], xj = pos[j];
+  double d0, d1, d2;
+  if (owner) {
+    double y0, y1, y2;
+    image_of(R, xj.x, xj.y, xj.z, sx, sy, sz, y0, y1, y2);
+    d0 = y0 - xi.x;
+    d1 = y1 - xi.y;
+    d2 = y2 - xi.z;
+  } else {
+    double y0, y1, y2;
+    image_of(R, xi.x, xi.y, xi.z, -sx, -sy, -sz, y0, y1, y2);
+    d0 = y0 - xj.x;
+    d1 = y1 - xj.y;
+    d2 = y2 - xj.z;
+  }
+  double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+  return r2 < R.cellsz;
+}
+
+__device__ __forceinline__ float4 rec32(const BuildArgs& a, int j) {
+  if (a.mixed) return reinterpret_cast<const float4*>(a.rec_a)[j];
+  double4 d = reinterpret_cast<const double4*>(a.rec_a)[j];
+  return make_float4((float)d.x, (float)d.y, (float)d.z, (float)d.w);
+}
+
+// Full neighbour list.  For each cell offset o of the 27, the reference
+// evaluates a pair from the atom whose cell has the other in its half
+// stencil (o lexicographically positive: the stencil of
+// src/imd_geom_3d.c:895-899) as (image of the partner) - (owner); the same
+// expression decides membership here (directly, or after the FP32 screen
+// for candidates far from the list radius), so each unordered pair is in
+// the list of both atoms exactly when the reference has it in its half
+// list.  One block per 128-slot block; the FP32 records of its tile are
+// staged in LDS; entries are written four at a time.
+__global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
+  __shared__ float4 trec[IMDG_TILE_CAP];
+  __shared__ int4 truns[IMDG_MAXRUN];
+  __shared__ float off[3 * IMDG_NCODES];
+  __shared__ int tinfo[2];
+  if (!a.ctl->any_rebuild || a.ctl->frozen) return;
+  int b = blockIdx.x;
+  int r = a.blk_rep[b];
+  const RepDev& R = a.rep[r];
+  bool flagged = a.rst[r].need_rebuild != 0;
+  // the permuted positions (k_list_done flips the buffers afterwards)
+  const double4* pos = a.pos[a.ctl->pos_cur ^ 1];
+  int kmax = a.kmax, seen = 0;
+  if (flagged) {
+    const Tile& T = a.tiles[b];
+    if (threadIdx.x == 0) {
+      tinfo[0] = T.nrun;
+      tinfo[1] = T.ok;
+    }
+    for (int k = threadIdx.x; k < T.nrun; k += blockDim.x) truns[k] = T.run[k];
+    for (int k = threadIdx.x; k < 3 * IMDG_NCODES; k += blockDim.x)
+      off[k] = (float)((&R.offs[0][0])[k]);
+    __syncthreads();
+    int nrun = tinfo[0], tok = tinfo[1];
+    if (tok) {
+      int tot = T.natoms;
+      // all of a thread's loads in flight together: 8 records per pass
+      int k = 0;
+      for (int t0 = threadIdx.x; t0 < tot; t0 += 8 * IMDG_BLOCK) {
+        float4 v[8];
+        int t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          t[u] = t0 + u * IMDG_BLOCK;
+          if (t[u] < tot) {
+            while (t[u] >= truns[k].z + truns[k].y) k++;
+            v[u] = rec32(a, truns[k].x + (t[u] - truns[k].z));
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if (t[u] < tot) trec[t[u]] = v[u];
+      }
+    }
+    __syncthreads();
+    int i = b * IMDG_BLOCK + threadIdx.x;
+    if (i - R.atom_off < R.natoms) {
+      int cx, cy, cz;
+      cell_coords(R, a.acell[i], cx, cy, cz);
+      int g0 = R.gdim[0], g1 = R.gdim[1], g2 = R.gdim[2];
+      float4 ui = rec32(a, i);
+      float cs = (float)R.cellsz;
+      float lo = cs * (float)(1.0 - IMDG_SCREEN_BAND);
+      float hi = cs * (float)(1.0 + IMDG_SCREEN_BAND);
+      uint4* lst = reinterpret_cast<uint4*>(a.nbr) +
+                   ((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63);
+      uint4 q = make_uint4(0u, 0u, 0u, 0u);
+      int cnt = 0;
+      for (int ox = -1; ox <= 1; ox++) {
+        int n0 = cx + ox, sx = 0;
+        if (n0 < 0) {
+          if (R.pbc[0] != 1) continue;
+          n0 += g0;
+          sx = -1;
+        } else if (n0 >= g0) {
+          if (R.pbc[0] != 1) continue;
+          n0 -= g0;
+          sx = 1;
+        }
+        for (int oy = -1; oy <= 1; oy++) {
+          int n1 = cy + oy, sy = 0;
+          if (n1 < 0) {
+            if (R.pbc[1] != 1) continue;
+            n1 += g1;
+            sy = -1;
+          } else if (n1 >= g1) {
+            if (R.pbc[1] != 1) continue;
+            n1 -= g1;
+            sy = 1;
+          }
+          for (int oz = -1; oz <= 1; oz++) {
+            int n2 = cz + oz, sz = 0;
+            if (n2 < 0) {
+              if (R.pbc[2] != 1) continue;
+              n2 += g2;
+              sz = -1;
+            } else if (n2 >= g2) {
+              if (R.pbc[2] != 1) continue;
+              n2 -= g2;
+              sz = 1;
+            }
+            bool self = (ox == 0 && oy == 0 && oz == 0);
+            bool owner =
+                ox > 0 || (ox == 0 && (oy > 0 || (oy == 0 && oz >= 0)));
+            uint32_t code = (uint32_t)((ox + 1) * 9 + (oy + 1) * 3 + (oz + 1));
+            float o0 = off[3 * code], o1 = off[3 * code + 1],
+                  o2 = off[3 * code + 2];
+            int nc = R.cell_off + (n0 * g1 + n1) * g2 + n2;
+            int jb = a.cell_beg[nc], je = jb + a.cell_num[nc];
+            int tb = (tok && je > jb) ? tile_index(truns, nrun, jb) : jb;
+            for (int j = jb; j < je; j++) {
+              if (self && j == i) continue;
+              float4 uj = tok ? trec[tb + (j - jb)] : rec32(a, j);
+              float d0 = (uj.x - ui.x) + o0;
+              float d1 = (uj.y - ui.y) + o1;
+              float d2 = (uj.z - ui.z) + o2;
+              float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+              bool in = r2 < lo;
+              if (!in && r2 < hi) in = exact_pair(R, pos, i, j, sx, sy, sz, owner);
+              if (in) {
+                uint32_t e = (uint32_t)j | (code << IMDG_IDX_BITS);
+                int s = cnt & 3;
+                q.x = s == 0 ? e : q.x;
+                q.y = s == 1 ? e : q.y;
+                q.z = s == 2 ? e : q.z;
+                q.w = s == 3 ? e : q.w;
+                cnt++;
+                if (s == 3 && cnt <= kmax)
+                  lst[(size_t)((cnt - 1) >> 2) * IMDG_WAVE] = q;
+              }
+            }
+          }
+        }
+      }
+      // pad to a multiple of IMDG_LIST_PAD entries with far-away copies of
+      // the atom itself (kmax is a multiple of IMDG_LIST_PAD, so the padded
+      // list fits whenever the list does)
+      if (cnt <= kmax) {
+        uint32_t pad = (uint32_t)i | ((uint32_t)IMDG_CODE_PAD << IMDG_IDX_BITS);
+        int s = cnt & 3;
+        if (s != 0) {
+          if (s <= 1) q.y = pad;
+          if (s <= 2) q.z = pad;
+          q.w = pad;
+          lst[(size_t)(cnt >> 2) * IMDG_WAVE] = q;
+        }
+        int g = (cnt + 3) >> 2, gend = ((cnt + IMDG_LIST_PAD - 1) /
+                                        IMDG_LIST_PAD) * (IMDG_LIST_PAD / 4);
+        for (; g < gend; g++)
+          lst[(size_t)g * IMDG_WAVE] = make_uint4(pad, pad, pad, pad);
+      }
+      a.nnb[i] = cnt < kmax ? cnt : kmax;
+      seen = cnt;
+    }
+  }
+  // block max of the list length (k_list_done reduces the blocks)
+  __shared__ int wmax[IMDG_NWAVE];
+  for (int o = 32; o > 0; o >>= 1)
+    seen = max(seen, __shfl_xor(seen, o, IMDG_WAVE));
+  if ((threadIdx.x % IMDG_WAVE) == 0) wmax[threadIdx.x / IMDG_WAVE] = seen;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = wmax[0];
+    for (int k = 1; k < IMDG_NWAVE; k++) m = max(m, wmax[k]);
+    a.blkmax[b] = m;
+  }
+}
+
+// End of a rebuild (one block): the longest list; on an overflow freeze the
+// state (the host grows the list and runs the chain again from the old
+// buffers), otherwise clear the rebuild flags and flip the buffers that
+// k_permute filled.
+__global__ void __launch_bounds__(1024) k_list_done(BuildArgs a) {
+  __shared__ int wmax[1024 / IMDG_WAVE];
+  if (!a.ctl->any_rebuild || a.ctl->frozen) return;
+  int m = 0;
+  for (int b = threadIdx.x; b < a.nblk; b += blockDim.x)
+    if (a.rst[a.blk_rep[b]].need_rebuild) m = max(m, a.blkmax[b]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, IMDG_WAVE));
+  if ((threadIdx.x % IMDG_WAVE) == 0) wmax[threadIdx.x / IMDG_WAVE] = m;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int k = 1; k < (int)(blockDim.x / IMDG_WAVE); k++) m = max(m, wmax[k]);
+  m = max(m, wmax[0]);
+  if (m > a.ctl->kmax_seen) a.ctl->kmax_seen = m;
+  if (m > a.kmax) {
+    // keep the rebuild flags: the host grows the list and builds again
+    a.ctl->overflow = 1;
+    a.ctl->frozen = 1;
+  } else {
+    for (int q = 0; q < a.nrep; q++)
+      if (a.rst[q].need_rebuild) {
+        a.rst[q].rebuilt = 1;
+        a.rst[q].nbl_count += 1;
+        a.rst[q].need_rebuild = 0;
+      }
+    a.ctl->pos_cur ^= 1;
+    a.ctl->rec_cur ^= 1;
+    a.ctl->any_rebuild = 0;
+  }
+}
+
+// check_nblist alone: max displacement^2 per block of IMDG_BLOCK slots.
+__global__ void __launch_bounds__(IMDG_BLOCK)
+    k_check(const RepDev* rep, Ctl* ctl, const int* blk_rep,
+            const double4* pos0, const double4* pos1, const double4* xnbl,
+            double* maxd) {
+  __shared__ double red[IMDG_NWAVE];
+  if (ctl->frozen) return;
+  int lb = blockIdx.x;
+  int r = blk_rep[lb];
+  const RepDev& Rp = rep[r];
+  int i = lb * IMDG_BLOCK + threadIdx.x;
+  const double4* pos = ctl->pos_cur ? pos1 : pos0;
+  double v = 0.0;
+  if ((i - Rp.atom_off) < Rp.natoms) v = disp2(pos[i], xnbl[i]);
+  v = wave_max(v);
+  if ((threadIdx.x % IMDG_WAVE) == 0) red[threadIdx.x / IMDG_WAVE] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int k = 1; k < IMDG_NWAVE; k++) m = fmax(m, red[k]);
+    maxd[lb] = m;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+void launch_build(const BuildArgs& b, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_bin, dim3(b.nblk), dim3(IMDG_BLOCK), 0, st, b);
+  hipLaunchKernelGGL(k_scan, dim3(b.nstile), dim3(IMDG_SCAN_TILE), 0, st, b);
+  hipLaunchKernelGGL(k_place, dim3(b.nblk), dim3(IMDG_BLOCK), 0, st, b);
+  hipLaunchKernelGGL(k_cellsort, dim3(grid), dim3(256), 0, st, b);
+  hipLaunchKernelGGL(k_permute, dim3(b.nblk), dim3(IMDG_BLOCK), 0, st, b);
+  int gt = (b.nblk + TILE_WAVES - 1) / TILE_WAVES;
+  hipLaunchKernelGGL(k_tiles, dim3(gt < 1 ? 1 : gt), dim3(TILE_WAVES * IMDG_WAVE),
+                     0, st, b);
+  hipLaunchKernelGGL(k_list, dim3(b.nblk), dim3(IMDG_BLOCK), 0, st, b);
+  hipLaunchKernelGGL(k_list_done, dim3(1), dim3(1024), 0, st, b);
+}
+
+void launch_check(const RepDev* rep, Ctl* ctl, const int* blk_rep,
+                  const double4* pos0, const double4* pos1,
+                  const double4* xnbl, double* maxd, int nblk,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_check, dim3(nblk), dim3(IMDG_BLOCK), 0, st, rep, ctl,
+                     blk_rep, pos0, pos1, xnbl, maxd);
+}
+
+}  // namespace imdg

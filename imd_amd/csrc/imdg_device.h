@@ -1,0 +1,129 @@
+// imdg_device.h -- device helpers shared by the rebuild kernels
+// (imdg_build.hip) and the per-step kernels (imdg_step.hip): wavefront and
+// compensated reductions, IMD's 3-point table access on knot pairs, the
+// grouped neighbour-list layout, cell geometry.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "imdg_internal.h"
+
+namespace imdg {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, IMDG_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, IMDG_WAVE));
+  return v;
+}
+
+// Error-free sum of two doubles (Knuth TwoSum).
+__device__ __forceinline__ void two_sum(double a, double b, double& s,
+                                        double& e) {
+#pragma clang fp contract(off)
+  s = a + b;
+  double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Neumaier-compensated accumulation.
+__device__ __forceinline__ void kahan_add(double& s, double& c, double x) {
+#pragma clang fp contract(off)
+  double t = s + x;
+  if (fabs(s) >= fabs(x))
+    c += (s - t) + x;
+  else
+    c += (x - t) + s;
+  s = t;
+}
+
+template <typename R>
+__device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
+
+// Segment index and fraction of IMD's 3-point table access
+// (src/potaccess.h:329-340): argument clamped to [begin, end], segment
+// k = floor((x - begin) / step) capped at len - 1.
+template <typename R>
+__device__ __forceinline__ int seg_locate(R x, R begin, R end, R invstep,
+                                          int len, R& chi) {
+  R a = rmin(x, end) - begin;
+  a = a < (R)0 ? (R)0 : a;
+  R t = a * invstep;
+  int k = (int)t;
+  chi = t - (R)k;
+  return k < len - 1 ? k : len - 1;
+}
+
+// IMD's 3-point interpolation on segment k from the knots (p0_k, dv_k) and
+// (p0_k+1, dv_k+1), with dv_k = t[k+1] - t[k] and the second difference
+// d2v = dv_k+1 - dv_k = t[k+2] - 2 t[k+1] + t[k] (src/potaccess.h:345-353):
+// value p0 + chi (dv + (chi - 1)/2 d2v), and twice the derivative with
+// respect to the tabulated argument, 2 istep (dv + (chi - 1/2) d2v).
+template <typename R, typename R2>
+__device__ __forceinline__ R knot_val(R2 a, R2 b, R chi) {
+  R d2v = b.y - a.y;
+  return a.x + chi * (a.y + (R)0.5 * (chi - (R)1) * d2v);
+}
+template <typename R, typename R2>
+__device__ __forceinline__ R knot_grad(R2 a, R2 b, R chi, R twoistep) {
+  R d2v = b.y - a.y;
+  return twoistep * (a.y + (chi - (R)0.5) * d2v);
+}
+
+// Entry m of slot i in the grouped list layout (4 entries per lane).
+__device__ __forceinline__ size_t nbr_index(int i, int m, int kmax) {
+  return (((size_t)(i >> 6) * (kmax >> 2) + (m >> 2)) * IMDG_WAVE + (i & 63)) *
+             IMDG_GROUP +
+         (m & 3);
+}
+
+// First uint4 of slot i's list; group g is at + g * IMDG_WAVE.
+__device__ __forceinline__ const uint4* list_of(const uint32_t* nbr, int i,
+                                                int kmax) {
+  return reinterpret_cast<const uint4*>(nbr) +
+         ((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63);
+}
+
+__device__ __forceinline__ void cell_coords(const RepDev& R, int cell,
+                                            int& cx, int& cy, int& cz) {
+  int lc = cell - R.cell_off;
+  cz = lc % R.gdim[2];
+  int q = lc / R.gdim[2];
+  cy = q % R.gdim[1];
+  cx = q / R.gdim[1];
+}
+
+// Centre of a cell: sum_d (c_d + 1/2) box_d / gdim_d.
+__device__ __forceinline__ void cell_centre(const RepDev& R, int cell,
+                                            double* c) {
+  int cx, cy, cz;
+  cell_coords(R, cell, cx, cy, cz);
+  double fx = cx + 0.5, fy = cy + 0.5, fz = cz + 0.5;
+#pragma unroll
+  for (int e = 0; e < 3; e++)
+    c[e] = fx * R.cvec[0][e] + fy * R.cvec[1][e] + fz * R.cvec[2][e];
+}
+
+// Squared displacement since the last list build (check_nblist's test,
+// src/imd_forces_nbl.c:2016-2024), FP64 with separate roundings.
+__device__ __forceinline__ double disp2(double4 a, double4 b) {
+#pragma clang fp contract(off)
+  double d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z;
+  return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+// Make the LDS stores of this wavefront visible to its own later loads
+// (no block barrier: the waves of a block run independent schedules).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+}  // namespace imdg

@@ -1,14 +1,13 @@
 // imdg_internal.h -- device-side data layout of the MI355X neighbour-list
-// pair/EAM path.  Shared by the kernels (imdg_kernels.hip) and the host side
-// of the C ABI (imdg_api.hip).
+// pair/EAM path.  Shared by the kernels (imdg_build.hip, imdg_step.hip) and
+// the host side of the C ABI (imdg_api.hip).
 //
 // Layout in HBM (one context, all replicas concatenated):
 //  * atoms live in "slots"; every replica owns a run of slots padded to a
-//    multiple of IMDG_BLOCK so a thread block never spans two replicas;
-//    padding slots carry type -1 and take part in nothing;
+//    multiple of IMDG_BLOCK; padding slots take part in nothing;
 //  * at every list rebuild the atoms of a replica are sorted by cell (z
-//    fastest, then y, then x; inside a cell by input index), so a block of
-//    IMDG_BLOCK consecutive slots is a short run of cells along z;
+//    fastest, then y, then x; inside a cell by input index), so a chunk of
+//    64 consecutive slots (one wavefront) is a short run of cells along z;
 //  * pos[2] double4 (x, y, z, type), mom[2] double4 (px, py, pz, mass),
 //    aid[2] input index: the FP64 state; the second copy is the target of
 //    the rebuild permutation only;
@@ -19,27 +18,29 @@
 //    the coordinates are small, so FP32 keeps ~1e-7 A; the separation of a
 //    listed pair is (u_j - u_i) + offset(cell offset), offset being
 //    o . (box / grid) for o in {-1,0,1}^3 (periodic images included);
-//  * tiles: for every block, the slot runs of all cells adjacent to the
-//    block's cells (runs of consecutive cells along z are contiguous slot
-//    ranges); the force loops copy the block's tile into LDS with coalesced
-//    loads and gather neighbours from LDS;
-//  * nbr: uint32 full neighbour list, entry = index | code << 27 where code
-//    is the cell offset (ox+1)*9+(oy+1)*3+(oz+1) and index the neighbour's
-//    position in the block's LDS tile (global slot if the tile did not fit
-//    into IMDG_TILE_CAP records); padded per atom to a multiple of 4 with
-//    entries pointing at the atom itself through the far-away code
-//    IMDG_CODE_PAD; stored four per lane: entries 4c..4c+3 of slot i form
-//    the uint4 at ((i / 64) * (kmax / 4) + c) * 64 + i % 64;
-//  * tables as per-segment coefficients (p0, dv, d2v) of IMD's 3-point
-//    interpolation, precomputed in FP64 and rounded to the force precision.
+//  * nbr: uint32 full neighbour list, entry = slot | code << 27 where code
+//    is the cell offset (ox+1)*9+(oy+1)*3+(oz+1); padded per atom to a
+//    multiple of 8 with entries pointing at the atom itself through the
+//    far-away code IMDG_CODE_PAD; stored four per lane: entries 4c..4c+3 of
+//    slot i form the uint4 at ((i / 64) * (kmax / 4) + c) * 64 + i % 64;
+//  * tiles: for every block of IMDG_BLOCK slots, the slot runs of all
+//    cells adjacent to the block's cells, staged into LDS by the list build;
+//  * tables as knots (p0_k, dv_k = t[k+1] - t[k]), k = 0..len, of IMD's
+//    3-point interpolation, differences taken in FP64 and then rounded to
+//    the force precision; the per-step kernels copy the knots of the
+//    tables their neighbour loop reads into LDS when they fit.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define IMDG_BLOCK 128
+#define IMDG_BLOCK 128       // replica padding, rebuild blocks
 #define IMDG_WAVE 64
 #define IMDG_NWAVE (IMDG_BLOCK / IMDG_WAVE)
+#define IMDG_CHUNK 64        // slots per wavefront step of the step kernels
+#define IMDG_STEP_THREADS 1024   // largest block of the step kernels
+#define IMDG_STEP_WAVES (IMDG_STEP_THREADS / IMDG_WAVE)
+#define IMDG_DENSITY_THREADS 512 // density loop: several blocks per CU
 #define IMDG_MAXTYPES 8
 #define IMDG_MAXCOLS (IMDG_MAXTYPES * IMDG_MAXTYPES)
 #define IMDG_RING 4096
@@ -50,6 +51,7 @@
 #define IMDG_NCODES 28
 #define IMDG_PAD_SHIFT 1.0e6
 #define IMDG_GROUP 4          // list entries per lane and load
+#define IMDG_LIST_PAD 8       // lists padded to whole pairs of groups
 #define IMDG_TILE_CAP 2048    // records of one block's LDS tile
 #define IMDG_MAXRUN 40        // slot runs of one tile
 
@@ -66,7 +68,10 @@
 #define DC_PERMUTE 0
 #define DC_LIST 1
 #define DC_FINAL 2
+#define DC_SCAN 3
 #define DC_N 4
+
+#define IMDG_SCAN_TILE 1024  // cells per block of the cell-count scan
 
 struct RepDev {
   double box[3][3];       // rows: box_x, box_y, box_z
@@ -87,6 +92,8 @@ struct RepDev {
   int ncells;
   int blk_off;            // first block
   int nblk;
+  int stile_off;          // first tile of the cell-count scan
+  int nstile;
   int nactive;            // 3 * natoms
 };
 
@@ -130,31 +137,43 @@ struct Tile {
 
 // Table of one interpolated function set on the device.
 template <typename R>
-struct Seg {
-  R p0, dv, d2v, pad;
-};
-
-template <typename R>
 struct TabDev {
   int ncols;
-  int off[IMDG_MAXCOLS];
+  int off[IMDG_MAXCOLS];  // first knot of the column in the knot array
   int len[IMDG_MAXCOLS];  // usable segments (rows of the IMD column)
   R begin[IMDG_MAXCOLS], end[IMDG_MAXCOLS], invstep[IMDG_MAXCOLS];
   R twoistep[IMDG_MAXCOLS];
 };
 
+// Column parameters as the NT > 1 neighbour loops read them from LDS.
+template <typename R>
+struct ColPar {
+  R begin, end, invstep, twoistep;
+  int off, len;
+};
+
 template <typename R> struct V4;
 template <> struct V4<float> { typedef float4 T; };
 template <> struct V4<double> { typedef double4 T; };
+template <typename R> struct V2;
+template <> struct V2<float> { typedef float2 T; };
+template <> struct V2<double> { typedef double2 T; };
 
 // Everything the per-step kernels need, passed by value.
+//
+// The per-step kernels are persistent: IMDG_STEP_THREADS-thread blocks, a
+// grid sized to the resident capacity, every wavefront walking its share of
+// the 64-slot chunks.  Reductions go to one partial per wavefront
+// (part_chunk = 0, a single replica) or one per chunk (part_chunk = 1,
+// several replicas); the last block to finish sums them in a fixed order.
 template <typename R>
 struct StepArgs {
   typedef typename V4<R>::T R4;
+  typedef typename V2<R>::T R2;
   const RepDev* rep;
   RepState* rst;
   Ctl* ctl;
-  const int* blk_rep;     // block -> replica
+  const int* blk_rep;     // IMDG_BLOCK-slot block -> replica
   double4* pos[2];
   double4* mom[2];
   double4* xnbl;
@@ -162,24 +181,28 @@ struct StepArgs {
   const signed char* atype;  // type of every slot (current order)
   R4* rec_a;              // (u, type)
   R4* rec_b;              // (u, 2F')
-  const Tile* tiles;
   const uint32_t* nbr;
   const int* nnb;
-  const Seg<R>* seg;      // segment coefficients of all tables
+  const R2* knots;        // knots of all tables
   R* rho;
   R* epot;
   R4* frc;                // (fx, fy, fz, 0)
-  double* partA;          // [nblk] embedding energy partials
-  double* partB;          // [nblk * PB_N]
+  double* partA;          // embedding energy partials
+  double* partB;          // [unit * PB_N]
   ThermoRec* ring;        // [IMDG_RING * nrep]
   int nrep;
-  int nblk;
+  int nchunk;             // 64-slot chunks of all replicas
+  int part_chunk;         // partials per chunk (1) or per wavefront (0)
+  int unitsA;             // wavefront partials written by the density loop
   int kmax;
   int ntypes;
   int ensemble;
   double dt;
   int has_pair, has_eam;
-  int xcd_remap;          // map blocks to XCD-contiguous atom runs
+  int xcd_remap;          // XCD-local chunk ranges
+  int lds_lo, lds_n;      // knots staged in LDS by this launch (n = 0: none)
+  int joint;              // first knot of the interleaved single-species
+                          // (pair, rho) table, -1 if there is none
 };
 
 struct BuildArgs {
@@ -200,14 +223,20 @@ struct BuildArgs {
   signed char* atype;     // type of slot after the permutation
   int* perm;              // new slot -> old slot
   int* cell_cnt;          // atomic counters (zero outside a rebuild)
+  int* cell_loc;          // first slot of a cell relative to its scan tile
+  int* tile_sum;          // atoms per scan tile
+  int* tile_off;          // first slot of a scan tile
+  const int* tile_rep;    // scan tile -> replica
   int* cell_beg;
   int* cell_num;
   Tile* tiles;
   uint32_t* nbr;
   int* nnb;
+  int* blkmax;            // longest list of every block (k_list)
   int nrep;
   int nslots;
   int nblk;
   int ncells;
+  int nstile;
   int kmax;
 };

@@ -1,0 +1,1028 @@
+// imdg_step.hip -- CDNA4 (gfx950) per-step kernels of the neighbour-list
+// pair/EAM path: host density with embedding, pair + EAM forces fused with
+// the leapfrog step and the skin test, the unfused move, compensated FP64
+// reductions finished by the last block of each step.
+//
+// Execution model: persistent kernels.  A grid of IMDG_STEP_THREADS-thread
+// blocks sized to what the chip holds at once copies the knots of the
+// tables its neighbour loop reads into LDS (when they fit), then every
+// wavefront walks its share of the 64-slot chunks: lane = atom, neighbour
+// records gathered from HBM/L2 through the grouped full list (four entries
+// per lane and load, the next groups prefetched), table knots from LDS.
+// There is no block barrier after the staging, so waves never wait for
+// each other.
+//
+// Reference semantics (paths relative to the reference root):
+//   density/embedding  src/imd_forces_nbl.c:586-611, 1079-1112
+//   pair + EAM force   src/imd_forces_nbl.c:490-584, 1117-1322
+//   tables             PAIR_INT2/VAL_FUNC2/DERIV_FUNC2 src/potaccess.h:323-354,
+//                      465-495, 591-621
+//   integrators        move_atoms_nve src/imd_integrate.c:122-358,
+//                      move_atoms_nvt src/imd_integrate.c:949-1141
+//   skin test          check_nblist src/imd_forces_nbl.c:2007-2037
+#include "imdg_device.h"
+
+namespace imdg {
+
+// Threads per block of the force loop: its 4-neighbour load phases need
+// more than the 128 VGPRs a 1024-thread block leaves a wave, and with the
+// tables in LDS one block per CU is resident anyway.
+template <typename R>
+struct ForceThreads {
+  static constexpr int value = sizeof(R) == 4 ? 768 : 512;
+};
+
+// Neighbour-loop variants of the force kernel.
+enum {
+  FM_NT1 = 0,  // one species
+  FM_SG = 1,   // one species, pair and rho tables on the same grid: one
+               // segment lookup and one interleaved knot table
+  FM_NTN = 2   // several species
+};
+
+// ---------------------------------------------------------------------------
+// schedule, offsets, partials
+// ---------------------------------------------------------------------------
+
+// Chunks of one wavefront: c, c + stride, ... below c1.  Blocks are
+// dispatched round-robin over the 8 XCDs; with xcd_remap the waves of XCD x
+// walk the x-th eighth of the chunks, so the neighbour records they gather
+// stay in that XCD's L2.  unit: the wavefront's partial slot.
+struct Sched {
+  int c, c1, stride, unit;
+};
+
+__device__ __forceinline__ Sched make_sched(int nchunk, int xcd_remap) {
+  Sched s;
+  int nwb = blockDim.x / IMDG_WAVE, w = threadIdx.x / IMDG_WAVE;
+  s.unit = blockIdx.x * nwb + w;
+  if (xcd_remap && (gridDim.x & 7) == 0) {
+    int x = blockIdx.x & 7, bx = blockIdx.x >> 3;
+    int per = (nchunk + 7) >> 3;
+    int c0 = min(x * per, nchunk);
+    s.c1 = min(c0 + per, nchunk);
+    s.stride = (gridDim.x >> 3) * nwb;
+    s.c = c0 + bx * nwb + w;
+  } else {
+    s.c1 = nchunk;
+    s.stride = gridDim.x * nwb;
+    s.c = s.unit;
+  }
+  return s;
+}
+
+// Cell-offset vectors of a replica, in the force precision, into the
+// wavefront's own LDS row.
+template <typename R>
+__device__ __forceinline__ void wave_offsets(const RepDev& Rp, R* off) {
+  int l = threadIdx.x & (IMDG_WAVE - 1);
+  for (int k = l; k < 3 * IMDG_NCODES; k += IMDG_WAVE)
+    off[k] = (R)((&Rp.offs[0][0])[k]);
+  wave_lds_sync();
+}
+
+// Wavefront reduction of the step partials (sum; max for PB_MAXD).
+__device__ __forceinline__ void wave_reduce_part(double (&v)[PB_N]) {
+#pragma unroll
+  for (int q = 0; q < PB_N; q++)
+    v[q] = (q == PB_MAXD) ? wave_max(v[q]) : wave_sum(v[q]);
+}
+
+__device__ __forceinline__ void add_part(double (&acc)[PB_N],
+                                         const double (&v)[PB_N]) {
+#pragma unroll
+  for (int q = 0; q < PB_N; q++)
+    acc[q] = (q == PB_MAXD) ? fmax(acc[q], v[q]) : acc[q] + v[q];
+}
+
+// Partials of one chunk (part_chunk) or accumulation into the wavefront's
+// running sums.
+__device__ __forceinline__ void put_part(double* partB, int part_chunk,
+                                         int c, double (&v)[PB_N],
+                                         double (&acc)[PB_N]) {
+  if (part_chunk) {
+    wave_reduce_part(v);
+    if ((threadIdx.x & (IMDG_WAVE - 1)) == 0) {
+#pragma unroll
+      for (int q = 0; q < PB_N; q++) partB[(size_t)c * PB_N + q] = v[q];
+    }
+  } else {
+    add_part(acc, v);
+  }
+}
+
+__device__ __forceinline__ void flush_part(double* partB, int part_chunk,
+                                           int unit, double (&acc)[PB_N]) {
+  if (part_chunk) return;
+  wave_reduce_part(acc);
+  if ((threadIdx.x & (IMDG_WAVE - 1)) == 0) {
+#pragma unroll
+    for (int q = 0; q < PB_N; q++) partB[(size_t)unit * PB_N + q] = acc[q];
+  }
+}
+
+__device__ __forceinline__ void flag_short(Ctl* ctl, int is_short) {
+  if (__ballot(is_short) != 0 && (threadIdx.x & (IMDG_WAVE - 1)) == 0)
+    atomicOr(&ctl->is_short, 1);
+}
+
+// Knots of the launch's LDS range (block-wide, before any wave loop).
+template <typename R2>
+__device__ __forceinline__ void stage_knots(const R2* __restrict__ g, int lo,
+                                            int n, R2* lds) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) lds[k] = g[lo + k];
+}
+
+template <typename R>
+__device__ __forceinline__ void stage_colpar(const TabDev<R>& t, int klo,
+                                             ColPar<R>* cp) {
+  for (int k = threadIdx.x; k < t.ncols; k += blockDim.x) {
+    ColPar<R> p;
+    p.begin = t.begin[k];
+    p.end = t.end[k];
+    p.invstep = t.invstep[k];
+    p.twoistep = t.twoistep[k];
+    p.off = t.off[k] - klo;
+    p.len = t.len[k];
+    cp[k] = p;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// finalisation (last block of a step kernel)
+// ---------------------------------------------------------------------------
+
+// Compensated sums of the partials of every replica (fixed order, so the
+// result does not depend on which block finishes last), thermostat, skin
+// decision, thermo record.
+template <typename R>
+__device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved) {
+  __shared__ double lred[IMDG_STEP_WAVES][PB_N + 1][2];
+  int nw = blockDim.x / IMDG_WAVE;
+  int w = threadIdx.x / IMDG_WAVE, l = threadIdx.x % IMDG_WAVE;
+  int unitsB = gridDim.x * nw;
+  bool any = false;
+  for (int r = 0; r < a.nrep; r++) {
+    const RepDev& Rp = a.rep[r];
+    int b0, b1, a0, a1;
+    if (a.part_chunk) {
+      b0 = a0 = Rp.blk_off * (IMDG_BLOCK / IMDG_CHUNK);
+      b1 = a1 = (Rp.blk_off + Rp.nblk) * (IMDG_BLOCK / IMDG_CHUNK);
+    } else {
+      b0 = a0 = 0;
+      b1 = unitsB;
+      a1 = a.unitsA;
+    }
+    double s[PB_N + 1], c[PB_N + 1];
+#pragma unroll
+    for (int q = 0; q <= PB_N; q++) s[q] = c[q] = 0.0;
+    for (int u = b0 + threadIdx.x; u < b1; u += blockDim.x) {
+      const double* pb = a.partB + (size_t)u * PB_N;
+#pragma unroll
+      for (int q = 0; q < PB_N; q++) {
+        if (q == PB_MAXD)
+          s[q] = fmax(s[q], pb[q]);
+        else
+          kahan_add(s[q], c[q], pb[q]);
+      }
+    }
+    if (forces && a.has_eam)
+      for (int u = a0 + threadIdx.x; u < a1; u += blockDim.x)
+        kahan_add(s[PB_N], c[PB_N], a.partA[u]);
+    // combine across the block in a fixed order (TwoSum keeps the error)
+#pragma unroll
+    for (int q = 0; q <= PB_N; q++) {
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        double so = __shfl_xor(s[q], o, IMDG_WAVE);
+        double co = __shfl_xor(c[q], o, IMDG_WAVE);
+        if (q == PB_MAXD) {
+          s[q] = fmax(s[q], so);
+        } else {
+          double t, e;
+          two_sum(s[q], so, t, e);
+          s[q] = t;
+          c[q] = c[q] + co + e;
+        }
+      }
+      if (l == 0) {
+        lred[w][q][0] = s[q];
+        lred[w][q][1] = c[q];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot[PB_N + 1];
+#pragma unroll
+      for (int q = 0; q <= PB_N; q++) {
+        double ss = lred[0][q][0], cc = lred[0][q][1];
+        for (int k = 1; k < nw; k++) {
+          if (q == PB_MAXD) {
+            ss = fmax(ss, lred[k][q][0]);
+          } else {
+            double t, e;
+            two_sum(ss, lred[k][q][0], t, e);
+            ss = t;
+            cc = cc + lred[k][q][1] + e;
+          }
+        }
+        tot[q] = (q == PB_MAXD) ? ss : ss + cc;
+      }
+      RepState& st = a.rst[r];
+      if (forces) {
+        st.epot = tot[PB_EPAIR] + tot[PB_N];
+        st.virial = tot[PB_VIRIAL];
+      }
+      if (moved) {
+        double ekin;
+        if (a.ensemble == 0) {
+          ekin = tot[PB_KE];
+        } else {
+          ekin = (tot[PB_E1] + tot[PB_E2]) / 4.0;
+          double ttt = Rp.nactive * Rp.temperature;
+          st.eta += a.dt * (tot[PB_E2] / ttt - 1.0) * Rp.isq_tau_eta;
+        }
+        int need = tot[PB_MAXD] > Rp.skin2;
+        st.need_rebuild = need;
+        any |= need;
+        long step = a.ctl->steps_done;
+        ThermoRec& tr = a.ring[(size_t)(step % IMDG_RING) * a.nrep + r];
+        tr.epot = st.epot;
+        tr.ekin = ekin;
+        tr.virial = st.virial;
+        tr.eta = st.eta;
+        tr.max_disp2 = tot[PB_MAXD];
+        tr.step = step;
+        tr.rebuilt = st.rebuilt;
+        tr.natoms = Rp.natoms;
+        st.rebuilt = 0;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (moved) {
+      a.ctl->any_rebuild = any ? 1 : 0;
+      a.ctl->steps_done += 1;
+    }
+    __threadfence();
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void last_block_finalize(StepArgs<R>& a,
+                                                    bool forces, bool moved) {
+  __shared__ int is_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned t = atomicAdd(&a.ctl->done[DC_FINAL], 1u);
+    is_last = (t == gridDim.x - 1);
+    if (is_last) a.ctl->done[DC_FINAL] = 0;
+  }
+  __syncthreads();
+  if (is_last) {
+    __threadfence();
+    finalize_step(a, forces, moved);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// integration
+// ---------------------------------------------------------------------------
+
+// Leapfrog on momenta (src/imd_integrate.c:213-217, 329-358 and
+// 1020-1056), FP64, separately rounded like the reference.
+__device__ __forceinline__ void leapfrog(int ensemble, double dt, double eta,
+                                         double fx, double fy, double fz,
+                                         double4& p, double& x0, double& x1,
+                                         double& x2, double& ke, double& e1,
+                                         double& e2) {
+#pragma clang fp contract(off)
+  double m = p.w;
+  if (ensemble == 0) {
+    double k1 = p.x * p.x + p.y * p.y + p.z * p.z;
+    p.x += dt * fx;
+    p.y += dt * fy;
+    p.z += dt * fz;
+    double k2 = p.x * p.x + p.y * p.y + p.z * p.z;
+    ke = (k1 + k2) / (4 * m);
+    e1 = e2 = 0.0;
+  } else {
+    double reib = 1.0 - eta * dt / 2.0;
+    double einsd = 1.0 / (1.0 + eta * dt / 2.0);
+    e1 = (p.x * p.x + p.y * p.y + p.z * p.z) / m;
+    p.x = (p.x * reib + dt * fx) * einsd;
+    p.y = (p.y * reib + dt * fy) * einsd;
+    p.z = (p.z * reib + dt * fz) * einsd;
+    e2 = (p.x * p.x + p.y * p.y + p.z * p.z) / m;
+    ke = 0.0;
+  }
+  double tmp = dt / m;
+  x0 += tmp * p.x;
+  x1 += tmp * p.y;
+  x2 += tmp * p.z;
+}
+
+// Integrate slot i, refresh its cell-relative record and return the
+// partials (kinetic terms, displacement).
+template <typename R>
+__device__ __forceinline__ void integrate_slot(StepArgs<R>& a,
+                                               const RepDev& Rp, int r, int i,
+                                               double fx, double fy, double fz,
+                                               double* v) {
+  typedef typename V4<R>::T R4;
+  int pc = a.ctl->pos_cur;
+  double4* pos = a.pos[pc];
+  double4* mom = a.mom[pc];
+  double4 xi = pos[i];
+  double4 p = mom[i];
+  double x0 = xi.x, x1 = xi.y, x2 = xi.z, ke, e1, e2;
+  leapfrog(a.ensemble, a.dt, a.rst[r].eta, fx, fy, fz, p, x0, x1, x2, ke, e1,
+           e2);
+  mom[i] = p;
+  double4 xn = make_double4(x0, x1, x2, xi.w);
+  pos[i] = xn;
+  double c[3];
+  cell_centre(Rp, a.acell[i], c);
+  R4 ra;
+  ra.x = (R)(x0 - c[0]);
+  ra.y = (R)(x1 - c[1]);
+  ra.z = (R)(x2 - c[2]);
+  ra.w = (R)xi.w;
+  a.rec_a[i] = ra;
+  v[PB_KE] = ke;
+  v[PB_E1] = e1;
+  v[PB_E2] = e2;
+  v[PB_MAXD] = disp2(xn, a.xnbl[i]);
+}
+
+
+// ---------------------------------------------------------------------------
+// neighbour-list walk
+// ---------------------------------------------------------------------------
+
+// Walk a slot's list in pairs of groups (lists are padded to a multiple of
+// 8 entries, so every list has an even number of groups).  Groups A = 2h
+// and B = 2h+1 sit in two registers of their own; each group's body issues
+// its record gathers, then (through pf/slot) the load of the group two
+// ahead into the register it has just consumed, then computes.  No register
+// changes role between iterations, so nothing waits for the prefetch
+// before the next group's gathers need it.  The initial values are
+// selects rather than plain loads: a phi of loads would be folded into one
+// load next to its use, which undoes the prefetch.
+template <typename F>
+__device__ __forceinline__ void list_walk(const uint4* nb, int nnb, F body) {
+  const int npair = (nnb + 7) >> 3;
+  const int glast = npair > 0 ? 2 * npair - 1 : 0;
+  uint4 A = nb[0];
+  uint4 B = nb[(size_t)min(1, glast) * IMDG_WAVE];
+  A = make_uint4(npair > 0 ? A.x : 0u, npair > 0 ? A.y : 0u,
+                 npair > 0 ? A.z : 0u, npair > 0 ? A.w : 0u);
+  B = make_uint4(npair > 0 ? B.x : 0u, npair > 0 ? B.y : 0u,
+                 npair > 0 ? B.z : 0u, npair > 0 ? B.w : 0u);
+  for (int h = 0; h < npair; h++) {
+    body(A, nb + (size_t)min(2 * h + 2, glast) * IMDG_WAVE, A);
+    body(B, nb + (size_t)min(2 * h + 3, glast) * IMDG_WAVE, B);
+  }
+}
+
+// Single-species column of the density loop.
+template <typename R>
+struct DensityCol {
+  int kb, len;
+  R begin, end, invstep;
+};
+
+// Single-species columns of the force loop (pair, rho).
+template <typename R>
+struct ForceCols {
+  int pk, plen;
+  R pb, pe, pis, p2is;
+  int rk, rlen;
+  R rb, re, ris, r2is;
+};
+
+// Four neighbours of the density loop: records and offsets, the prefetch,
+// r^2 and segments, all knot loads together, accumulation with mask
+// multiplies (no branches around the loads).
+template <typename R, bool NT1, typename R4, typename R2>
+__device__ __forceinline__ void density_group(
+    uint4 e4, const uint4* pf, uint4& slot, const R4* __restrict__ rec,
+    const R* off, const R4& ui, int it, int nt, const R2* __restrict__ K,
+    const ColPar<R>* cpr, const DensityCol<R>& c0, R& rho, int& is_short) {
+  uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
+  R4 uj[4];
+  R o[4][3];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    uj[u] = rec[en[u] & IMDG_IDX_MASK];
+    const R* op = off + 3 * (en[u] >> IMDG_IDX_BITS);
+    o[u][0] = op[0];
+    o[u][1] = op[1];
+    o[u][2] = op[2];
+  }
+  // issued after the gathers: loads complete in order, and the gathers
+  // must not wait for this (HBM) one
+  __builtin_amdgcn_sched_barrier(0);
+  slot = *pf;
+  R r2[4], chi[4], cb[4], ce[4];
+  int kk[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    R dx = (uj[u].x - ui.x) + o[u][0];
+    R dy = (uj[u].y - ui.y) + o[u][1];
+    R dz = (uj[u].z - ui.z) + o[u][2];
+    r2[u] = dx * dx + dy * dy + dz * dz;
+    if (NT1) {
+      kk[u] = c0.kb + seg_locate(r2[u], c0.begin, c0.end, c0.invstep, c0.len,
+                                 chi[u]);
+      cb[u] = c0.begin;
+      ce[u] = c0.end;
+    } else {
+      ColPar<R> cp = cpr[it * nt + (int)uj[u].w];
+      kk[u] = cp.off +
+              seg_locate(r2[u], cp.begin, cp.end, cp.invstep, cp.len, chi[u]);
+      cb[u] = cp.begin;
+      ce[u] = cp.end;
+    }
+  }
+  R2 p[4], q[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    p[u] = K[kk[u]];
+    q[u] = K[kk[u] + 1];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    bool in = r2[u] < ce[u];
+    rho = fma(in ? (R)1 : (R)0, knot_val<R>(p[u], q[u], chi[u]), rho);
+    is_short |= (in && r2[u] < cb[u]);
+  }
+}
+
+// Four neighbours of the force loop (same phases as density_group).
+template <typename R, bool NT1, bool HP, bool HE, typename R4, typename R2>
+__device__ __forceinline__ void force_group(
+    uint4 e4, const uint4* pf, uint4& slot, const R4* __restrict__ rec,
+    const signed char* __restrict__ atype, const R* off, const R4& ui, R dFi,
+    int it, int nt, const R2* __restrict__ K, const ColPar<R>* cpp,
+    const ColPar<R>* cpr, const ForceCols<R>& c, R& fx, R& fy, R& fz, R& ep,
+    R& vir, int& is_short) {
+  uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
+  R4 uj[4];
+  int jt[4];
+  R o[4][3];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    uj[u] = rec[en[u] & IMDG_IDX_MASK];
+    jt[u] = NT1 ? 0 : (int)atype[en[u] & IMDG_IDX_MASK];
+    const R* op = off + 3 * (en[u] >> IMDG_IDX_BITS);
+    o[u][0] = op[0];
+    o[u][1] = op[1];
+    o[u][2] = op[2];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  slot = *pf;
+  R dx[4], dy[4], dz[4], r2[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    dx[u] = (uj[u].x - ui.x) + o[u][0];
+    dy[u] = (uj[u].y - ui.y) + o[u][1];
+    dz[u] = (uj[u].z - ui.z) + o[u][2];
+    r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+  }
+  R gg[4];
+  if (NT1) {
+    R cp[4], cr[4];
+    int kp[4], kr[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (HP) kp[u] = c.pk + seg_locate(r2[u], c.pb, c.pe, c.pis, c.plen, cp[u]);
+      if (HE) kr[u] = c.rk + seg_locate(r2[u], c.rb, c.re, c.ris, c.rlen, cr[u]);
+    }
+    R2 p0[4], p1[4], q0[4], q1[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (HP) {
+        p0[u] = K[kp[u]];
+        p1[u] = K[kp[u] + 1];
+      }
+      if (HE) {
+        q0[u] = K[kr[u]];
+        q1[u] = K[kr[u] + 1];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      R g1 = (R)0;
+      if (HP) {
+        bool in = r2[u] <= c.pe;
+        R m = in ? (R)1 : (R)0;
+        g1 = m * knot_grad<R>(p0[u], p1[u], cp[u], c.p2is);
+        ep = fma(m, knot_val<R>(p0[u], p1[u], cp[u]), ep);
+        is_short |= (in && r2[u] < c.pb);
+      }
+      if (HE) {
+        bool in = r2[u] < c.re;
+        R m = in ? (R)0.5 : (R)0;
+        g1 = fma(m * (dFi + uj[u].w), knot_grad<R>(q0[u], q1[u], cr[u], c.r2is),
+                 g1);
+        is_short |= (in && r2[u] < c.rb);
+      }
+      gg[u] = g1;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int col = it * nt + jt[u], colT = jt[u] * nt + it;
+      R g1 = (R)0;
+      if (HP) {
+        ColPar<R> cp = cpp[col];
+        if (r2[u] <= cp.end) {
+          R chi;
+          int k = seg_locate(r2[u], cp.begin, cp.end, cp.invstep, cp.len, chi);
+          R2 p = K[cp.off + k], q = K[cp.off + k + 1];
+          g1 += knot_grad<R>(p, q, chi, cp.twoistep);
+          ep += knot_val<R>(p, q, chi);
+          is_short |= (r2[u] < cp.begin);
+        }
+      }
+      if (HE) {
+        // rho'_i from column jt*nt+it, rho'_j from it*nt+jt
+        // (src/imd_forces_nbl.c:1156-1203)
+        ColPar<R> ci = cpr[colT], cj = cpr[col];
+        if (r2[u] < ci.end || r2[u] < cj.end) {
+          R chi;
+          int k = seg_locate(r2[u], ci.begin, ci.end, ci.invstep, ci.len, chi);
+          is_short |= (r2[u] < ci.begin);
+          R rpi = knot_grad<R>(K[ci.off + k], K[ci.off + k + 1], chi,
+                               ci.twoistep);
+          R rpj = rpi;
+          if (colT != col) {
+            k = seg_locate(r2[u], cj.begin, cj.end, cj.invstep, cj.len, chi);
+            is_short |= (r2[u] < cj.begin);
+            rpj = knot_grad<R>(K[cj.off + k], K[cj.off + k + 1], chi,
+                               cj.twoistep);
+          }
+          g1 += (R)0.5 * (dFi * rpj + uj[u].w * rpi);
+        }
+      }
+      gg[u] = g1;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    fx = fma(dx[u], gg[u], fx);
+    fy = fma(dy[u], gg[u], fy);
+    fz = fma(dz[u], gg[u], fz);
+    vir = fma(r2[u], gg[u], vir);
+  }
+}
+
+// Four neighbours of the single-species force loop when the pair and rho
+// tables share their grid: one segment per neighbour, knots J[k] =
+// (phi p0, phi dv, rho p0, rho dv) read as two 16 B LDS loads.  The common
+// factor 2 istep of both gradients is applied once per atom by the caller
+// (sums s* are in units of 1 / (2 istep)); hFi = F'(rho_i) = 2F'/2.
+template <typename R, typename R4>
+__device__ __forceinline__ void force_group_sg(
+    uint4 e4, const uint4* pf, uint4& slot, const R4* __restrict__ rec,
+    const R* off, const R4& ui, R hFi, const R4* __restrict__ J, int jb,
+    int len, R begin, R end, R invstep, R& sx, R& sy, R& sz, R& ep, R& svir,
+    R& r2min) {
+  uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
+  R4 uj[4];
+  R o[4][3];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    uj[u] = rec[en[u] & IMDG_IDX_MASK];
+    const R* op = off + 3 * (en[u] >> IMDG_IDX_BITS);
+    o[u][0] = op[0];
+    o[u][1] = op[1];
+    o[u][2] = op[2];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  slot = *pf;
+  R dx[4], dy[4], dz[4], r2[4], chi[4];
+  int kk[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    dx[u] = (uj[u].x - ui.x) + o[u][0];
+    dy[u] = (uj[u].y - ui.y) + o[u][1];
+    dz[u] = (uj[u].z - ui.z) + o[u][2];
+    r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+    kk[u] = jb + seg_locate(r2[u], begin, end, invstep, len, chi[u]);
+  }
+  R4 j0[4], j1[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    j0[u] = J[kk[u]];
+    j1[u] = J[kk[u] + 1];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    R c = chi[u] - (R)0.5;
+    R d2p = j1[u].y - j0[u].y;
+    R d2r = j1[u].w - j0[u].w;
+    R gp = fma(c, d2p, j0[u].y);
+    R gr = fma(c, d2r, j0[u].w);
+    R vp = fma(chi[u], fma((R)0.5 * (chi[u] - (R)1), d2p, j0[u].y), j0[u].x);
+    // pair: r2 <= end (PAIR_INT2 guard); rho: r2 < end (VAL/DERIV_FUNC2)
+    bool inp = r2[u] <= end, inr = r2[u] < end;
+    R h = fma((R)0.5, uj[u].w, hFi);
+    R g = fma(h, inr ? gr : (R)0, inp ? gp : (R)0);
+    ep += inp ? vp : (R)0;
+    sx = fma(dx[u], g, sx);
+    sy = fma(dy[u], g, sy);
+    sz = fma(dz[u], g, sz);
+    svir = fma(r2[u], g, svir);
+    r2min = r2[u] < r2min ? r2[u] : r2min;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// density loop
+// ---------------------------------------------------------------------------
+
+// Host density, embedding energy and 2F'(rho) (src/imd_forces_nbl.c:586-611,
+// 1079-1112): rho_i = sum_j rho_{type(i) type(j)}(r_ij^2) over the full
+// list, then F and 2F' from the embedding column of type(i).  Writes the
+// (u, 2F') records the force loop gathers.
+template <typename R, bool NT1, bool LT>
+__global__ void __launch_bounds__(IMDG_DENSITY_THREADS)
+    k_density(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
+  typedef typename V4<R>::T R4;
+  typedef typename V2<R>::T R2;
+  extern __shared__ __align__(16) unsigned char dyn_lds[];
+  __shared__ R woff[IMDG_STEP_WAVES][3 * IMDG_NCODES];
+  __shared__ ColPar<R> cpr[IMDG_MAXCOLS];
+  if (a.ctl->frozen) return;
+  const TabDev<R>& tr = tabs[1];
+  const TabDev<R>& te = tabs[2];
+  const int klo = LT ? a.lds_lo : 0;
+  if (LT) stage_knots(a.knots, a.lds_lo, a.lds_n, (R2*)dyn_lds);
+  if (!NT1) stage_colpar(tr, klo, cpr);
+  __syncthreads();
+  const R2* __restrict__ K = LT ? (const R2*)dyn_lds : a.knots;
+  const R4* __restrict__ rec = a.rec_a;
+  const int nt = a.ntypes;
+  // single-species column
+  const int kb0 = tr.off[0] - klo, len0 = tr.len[0];
+  const R b0 = tr.begin[0], e0 = tr.end[0], is0 = tr.invstep[0];
+
+  Sched S = make_sched(a.nchunk, a.xcd_remap);
+  const int lane = threadIdx.x & (IMDG_WAVE - 1);
+  R* off = woff[threadIdx.x / IMDG_WAVE];
+  int cur = -1, is_short = 0;
+  double acc = 0.0;
+  for (int c = S.c; c < S.c1; c += S.stride) {
+    int r = a.blk_rep[c / (IMDG_BLOCK / IMDG_CHUNK)];
+    const RepDev& Rp = a.rep[r];
+    if (r != cur) {
+      wave_offsets(Rp, off);
+      cur = r;
+    }
+    int i = c * IMDG_CHUNK + lane;
+    double emb = 0.0;
+    if (i - Rp.atom_off < Rp.natoms) {
+      R4 ui = rec[i];
+      int it = NT1 ? 0 : (int)ui.w;
+      const uint4* nb = list_of(a.nbr, i, a.kmax);
+      R rho = (R)0;
+      DensityCol<R> col0 = {kb0, len0, b0, e0, is0};
+      list_walk(nb, a.nnb[i], [&](uint4 e4, const uint4* pf, uint4& slot) {
+        density_group<R, NT1>(e4, pf, slot, rec, off, ui, it, nt, K, cpr,
+                              col0, rho, is_short);
+      });
+      // embedding: PAIR_INT2 on the embedding column of the type, argument
+      // rho (knots in HBM; one lookup per atom)
+      const R2* __restrict__ ke = a.knots + te.off[it];
+      R chi;
+      int k = seg_locate(rho, te.begin[it], te.end[it], te.invstep[it],
+                         te.len[it], chi);
+      R2 p = ke[k], q = ke[k + 1];
+      R F = knot_val<R>(p, q, chi);
+      R dF = knot_grad<R>(p, q, chi, te.twoistep[it]);
+      R4 rb;
+      rb.x = ui.x;
+      rb.y = ui.y;
+      rb.z = ui.z;
+      rb.w = dF;
+      a.rec_b[i] = rb;
+      a.rho[i] = rho;
+      a.epot[i] = F;
+      emb = (double)F;
+    }
+    if (a.part_chunk) {
+      double s = wave_sum(emb);
+      if (lane == 0) a.partA[c] = s;
+    } else {
+      acc += emb;
+    }
+  }
+  if (!a.part_chunk) {
+    double s = wave_sum(acc);
+    if (lane == 0) a.partA[S.unit] = s;
+  }
+  flag_short(a.ctl, is_short);
+}
+
+// ---------------------------------------------------------------------------
+// force loop (+ integration)
+// ---------------------------------------------------------------------------
+
+// Pair + EAM forces (src/imd_forces_nbl.c:490-584, 1117-1322) as a gather
+// over the full list of (u, 2F') records; with INTEGRATE the leapfrog step,
+// the skin test and the step's reductions follow in the same thread
+// (calc_forces, move_atoms and check_nblist of one main_loop iteration in
+// one launch).
+template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT>
+__global__ void __launch_bounds__(ForceThreads<R>::value)
+    k_force(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
+  constexpr bool NT1 = MODE != FM_NTN;
+  typedef typename V4<R>::T R4;
+  typedef typename V2<R>::T R2;
+  extern __shared__ __align__(16) unsigned char dyn_lds[];
+  __shared__ R woff[IMDG_STEP_WAVES][3 * IMDG_NCODES];
+  __shared__ ColPar<R> cpp[IMDG_MAXCOLS], cpr[IMDG_MAXCOLS];
+  if (a.ctl->frozen) return;
+  const TabDev<R>& tp = tabs[0];
+  const TabDev<R>& tr = tabs[1];
+  const int klo = LT ? a.lds_lo : 0;
+  if (LT) stage_knots(a.knots, a.lds_lo, a.lds_n, (R2*)dyn_lds);
+  if (!NT1) {
+    if (HP) stage_colpar(tp, klo, cpp);
+    if (HE) stage_colpar(tr, klo, cpr);
+  }
+  __syncthreads();
+  const R2* __restrict__ K = LT ? (const R2*)dyn_lds : a.knots;
+  // the gathered records (rec_b): (u, 2F') from the density loop with EAM,
+  // a copy of (u, type) without; the integration below rewrites rec_a
+  const R4* __restrict__ rec = a.rec_b;
+  const signed char* __restrict__ atype = a.atype;
+  const int nt = a.ntypes;
+  // single-species columns
+  const int pk0 = HP ? tp.off[0] - klo : 0, plen = HP ? tp.len[0] : 0;
+  const R pb = HP ? tp.begin[0] : (R)0, pe = HP ? tp.end[0] : (R)0,
+          pis = HP ? tp.invstep[0] : (R)0, p2is = HP ? tp.twoistep[0] : (R)0;
+  const int rk0 = HE ? tr.off[0] - klo : 0, rlen = HE ? tr.len[0] : 0;
+  const R rb = HE ? tr.begin[0] : (R)0, re = HE ? tr.end[0] : (R)0,
+          ris = HE ? tr.invstep[0] : (R)0, r2is = HE ? tr.twoistep[0] : (R)0;
+
+  Sched S = make_sched(a.nchunk, a.xcd_remap);
+  const int lane = threadIdx.x & (IMDG_WAVE - 1);
+  R* off = woff[threadIdx.x / IMDG_WAVE];
+  int cur = -1, is_short = 0;
+  double acc[PB_N] = {0, 0, 0, 0, 0, 0};
+  for (int c = S.c; c < S.c1; c += S.stride) {
+    int r = a.blk_rep[c / (IMDG_BLOCK / IMDG_CHUNK)];
+    const RepDev& Rp = a.rep[r];
+    if (r != cur) {
+      wave_offsets(Rp, off);
+      cur = r;
+    }
+    int i = c * IMDG_CHUNK + lane;
+    double v[PB_N] = {0, 0, 0, 0, 0, 0};
+    if (i - Rp.atom_off < Rp.natoms) {
+      R4 ui = rec[i];
+      int it = NT1 ? 0 : (int)atype[i];
+      R dFi = HE ? ui.w : (R)0;
+      const uint4* nb = list_of(a.nbr, i, a.kmax);
+      R fx = 0, fy = 0, fz = 0, ep = 0, vir = 0;
+      if (MODE == FM_SG) {
+        // one grid: sums in units of 1/(2 istep), scaled once per atom
+        R r2min = (R)3.0e38;
+        const R4* __restrict__ J =
+            reinterpret_cast<const R4*>(K + (a.joint - klo));
+        list_walk(nb, a.nnb[i], [&](uint4 e4, const uint4* pf, uint4& slot) {
+          force_group_sg<R>(e4, pf, slot, rec, off, ui, (R)0.5 * dFi, J, 0,
+                            plen, pb, pe, pis, fx, fy, fz, ep, vir, r2min);
+        });
+        fx *= p2is;
+        fy *= p2is;
+        fz *= p2is;
+        vir *= p2is;
+        is_short |= (r2min < pb);
+      } else {
+        ForceCols<R> fc = {pk0, plen, pb, pe, pis, p2is,
+                           rk0, rlen, rb, re, ris, r2is};
+        list_walk(nb, a.nnb[i], [&](uint4 e4, const uint4* pf, uint4& slot) {
+          force_group<R, NT1, HP, HE>(e4, pf, slot, rec, atype, off, ui, dFi,
+                                      it, nt, K, cpp, cpr, fc, fx, fy, fz, ep,
+                                      vir, is_short);
+        });
+      }
+      // per-atom energy: half of every pair term plus F(rho_i)
+      R ei = (R)0.5 * ep + (HE ? a.epot[i] : (R)0);
+      a.epot[i] = ei;
+      R4 fv;
+      fv.x = fx;
+      fv.y = fy;
+      fv.z = fz;
+      fv.w = (R)0;
+      a.frc[i] = fv;
+      v[PB_EPAIR] = 0.5 * (double)ep;
+      v[PB_VIRIAL] = -0.5 * (double)vir;
+      if (INTEGRATE)
+        integrate_slot(a, Rp, r, i, (double)fx, (double)fy, (double)fz, v);
+    }
+    put_part(a.partB, a.part_chunk, c, v, acc);
+  }
+  flush_part(a.partB, a.part_chunk, S.unit, acc);
+  flag_short(a.ctl, is_short);
+  last_block_finalize(a, true, INTEGRATE);
+}
+
+// move_atoms + check_nblist on the stored forces (the unfused entry point).
+template <typename R>
+__global__ void __launch_bounds__(IMDG_STEP_THREADS) k_move(StepArgs<R> a) {
+  typedef typename V4<R>::T R4;
+  if (a.ctl->frozen) return;
+  Sched S = make_sched(a.nchunk, a.xcd_remap);
+  const int lane = threadIdx.x & (IMDG_WAVE - 1);
+  double acc[PB_N] = {0, 0, 0, 0, 0, 0};
+  for (int c = S.c; c < S.c1; c += S.stride) {
+    int r = a.blk_rep[c / (IMDG_BLOCK / IMDG_CHUNK)];
+    const RepDev& Rp = a.rep[r];
+    int i = c * IMDG_CHUNK + lane;
+    double v[PB_N] = {0, 0, 0, 0, 0, 0};
+    if (i - Rp.atom_off < Rp.natoms) {
+      R4 f = a.frc[i];
+      integrate_slot(a, Rp, r, i, (double)f.x, (double)f.y, (double)f.z, v);
+    }
+    put_part(a.partB, a.part_chunk, c, v, acc);
+  }
+  flush_part(a.partB, a.part_chunk, S.unit, acc);
+  last_block_finalize(a, false, true);
+}
+
+// Pair-only systems: the force loop gathers rec_b, which holds a copy of the
+// (u, type) records (rec_a is rewritten by the integration in that loop).
+template <typename R>
+__global__ void __launch_bounds__(256) k_copyrec(StepArgs<R> a, int nslots) {
+  if (a.ctl->frozen) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nslots) a.rec_b[i] = a.rec_a[i];
+}
+
+// ---------------------------------------------------------------------------
+// launch plans and launchers
+// ---------------------------------------------------------------------------
+
+template <typename R, bool NT1, bool LT>
+const void* density_fn() {
+  return reinterpret_cast<const void*>(&k_density<R, NT1, LT>);
+}
+
+template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT>
+const void* force_fn3() {
+  return reinterpret_cast<const void*>(
+      &k_force<R, MODE, INTEGRATE, HP, HE, LT>);
+}
+
+template <typename R, int MODE, bool INTEGRATE, bool LT>
+const void* force_fn2(bool hp, bool he) {
+  if (MODE == FM_SG) return force_fn3<R, MODE, INTEGRATE, true, true, LT>();
+  if (hp && he) return force_fn3<R, MODE, INTEGRATE, true, true, LT>();
+  if (hp) return force_fn3<R, MODE, INTEGRATE, true, false, LT>();
+  return force_fn3<R, MODE, INTEGRATE, false, true, LT>();
+}
+
+template <typename R, int MODE>
+const void* force_fn1(bool integrate, bool lt, bool hp, bool he) {
+  if (integrate)
+    return lt ? force_fn2<R, MODE, true, true>(hp, he)
+              : force_fn2<R, MODE, true, false>(hp, he);
+  return lt ? force_fn2<R, MODE, false, true>(hp, he)
+            : force_fn2<R, MODE, false, false>(hp, he);
+}
+
+// mode: 0 one species, 1 one species on a shared grid (needs hp and he),
+// 2 several species
+template <typename R>
+const void* force_fn(int mode, bool integrate, bool lt, bool hp, bool he) {
+  if (mode == FM_SG) return force_fn1<R, FM_SG>(integrate, lt, true, true);
+  if (mode == FM_NT1) return force_fn1<R, FM_NT1>(integrate, lt, hp, he);
+  return force_fn1<R, FM_NTN>(integrate, lt, hp, he);
+}
+
+template <typename R>
+const void* density_fn(bool nt1, bool lt) {
+  if (nt1) return lt ? density_fn<R, true, true>() : density_fn<R, true, false>();
+  return lt ? density_fn<R, false, true>() : density_fn<R, false, false>();
+}
+
+// Persistent grid of a kernel: resident blocks per CU (from the occupancy
+// calculator with the launch's dynamic LDS) times the CU count, rounded
+// down to a multiple of the 8 XCDs; 0 if the kernel cannot be launched with
+// that much dynamic LDS.
+static int persistent_grid(const void* fn, int threads, size_t lds, int ncu) {
+  hipFuncAttributes fa;
+  if (hipFuncGetAttributes(&fa, fn) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  int dev = 0, maxlds = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                            dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  if (fa.sharedSizeBytes + lds > (size_t)maxlds) return 0;
+  if (lds > 0 &&
+      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, lds) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  if (per < 1) return 0;
+  int g = per * ncu;
+  if (g >= 8) g &= ~7;
+  return g;
+}
+
+template <typename R>
+int step_grid_density(bool nt1, bool lt, size_t lds, int ncu) {
+  return persistent_grid(density_fn<R>(nt1, lt), IMDG_DENSITY_THREADS, lds,
+                         ncu);
+}
+
+template <typename R>
+int step_grid_force(int mode, bool integrate, bool lt, bool hp, bool he,
+                    size_t lds, int ncu) {
+  return persistent_grid(force_fn<R>(mode, integrate, lt, hp, he),
+                         ForceThreads<R>::value, lds, ncu);
+}
+
+template <typename R>
+int step_grid_move(int ncu) {
+  return persistent_grid(reinterpret_cast<const void*>(&k_move<R>),
+                         IMDG_STEP_THREADS, 0, ncu);
+}
+
+template <typename R>
+hipError_t launch_density(const StepArgs<R>& a, const TabDev<R>* tabs,
+                          bool nt1, int grid, hipStream_t st) {
+  bool lt = a.lds_n > 0;
+  size_t lds = (size_t)a.lds_n * sizeof(typename V2<R>::T);
+  struct {
+    StepArgs<R> a;
+    const TabDev<R>* t;
+  } pk = {a, tabs};
+  void* kargs[] = {&pk.a, &pk.t};
+  return hipLaunchKernel(density_fn<R>(nt1, lt), dim3(grid),
+                         dim3(IMDG_DENSITY_THREADS), kargs, lds, st);
+}
+
+template <typename R>
+hipError_t launch_force(const StepArgs<R>& a, const TabDev<R>* tabs,
+                        int mode, bool integrate, int grid, hipStream_t st) {
+  bool lt = a.lds_n > 0;
+  size_t lds = (size_t)a.lds_n * sizeof(typename V2<R>::T);
+  StepArgs<R> aa = a;
+  const TabDev<R>* tt = tabs;
+  void* kargs[] = {&aa, &tt};
+  return hipLaunchKernel(
+      force_fn<R>(mode, integrate, lt, a.has_pair != 0, a.has_eam != 0),
+      dim3(grid), dim3(ForceThreads<R>::value), kargs, lds, st);
+}
+
+template <typename R>
+hipError_t launch_move(const StepArgs<R>& a, int grid, hipStream_t st) {
+  StepArgs<R> aa = a;
+  void* kargs[] = {&aa};
+  return hipLaunchKernel(reinterpret_cast<const void*>(&k_move<R>),
+                         dim3(grid), dim3(IMDG_STEP_THREADS), kargs, 0, st);
+}
+
+template <typename R>
+hipError_t launch_copyrec(const StepArgs<R>& a, int nslots, hipStream_t st) {
+  hipLaunchKernelGGL((k_copyrec<R>), dim3((nslots + 255) / 256), dim3(256), 0,
+                     st, a, nslots);
+  return hipGetLastError();
+}
+
+#define IMDG_INSTANTIATE(R)                                                   \
+  template int step_grid_density<R>(bool, bool, size_t, int);                \
+  template int step_grid_force<R>(int, bool, bool, bool, bool, size_t, int);  \
+  template int step_grid_move<R>(int);                                        \
+  template hipError_t launch_density<R>(const StepArgs<R>&,                   \
+                                        const TabDev<R>*, bool, int,          \
+                                        hipStream_t);                         \
+  template hipError_t launch_force<R>(const StepArgs<R>&, const TabDev<R>*,   \
+                                      int, bool, int, hipStream_t);           \
+  template hipError_t launch_move<R>(const StepArgs<R>&, int, hipStream_t);   \
+  template hipError_t launch_copyrec<R>(const StepArgs<R>&, int, hipStream_t);
+
+IMDG_INSTANTIATE(float)
+IMDG_INSTANTIATE(double)
+
+}  // namespace imdg
