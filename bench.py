@@ -25,7 +25,6 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from imd_amd import imdg as G  # noqa: E402
 from imd_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -101,6 +100,207 @@ def copy_probe(mib: int) -> int:
     return rc
 
 
+def _pmc_pass(counter: str, outdir: str, args_list, timeout: int = 240):
+    """One rocprofv3 counter pass over a child bench process; returns
+    {kernel name: mean counter value per dispatch} or None."""
+    import csv
+    import shutil
+    import subprocess
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", outdir,
+           "-o", "run", "--", sys.executable, "-u",
+           os.path.join(ROOT, "bench.py")] + args_list
+    try:
+        subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                       timeout=timeout, check=False)
+    except subprocess.TimeoutExpired:
+        return None
+    path = os.path.join(outdir, "run_counter_collection.csv")
+    if not os.path.exists(path):
+        return None
+    acc = {}
+    for row in csv.DictReader(open(path)):
+        if row["Counter_Name"] != counter:
+            continue
+        k = row["Kernel_Name"]
+        acc.setdefault(k, []).append(float(row["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def measure_traffic(args, kernel_key: str = "k_force"):
+    """HBM bytes per launch of the force kernel from FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch, one rocprofv3 pass each), each scaled by
+    what the counter reads for a 256 MiB device-to-device copy."""
+    import tempfile
+    base = tempfile.mkdtemp(prefix="imdg_pmc_")
+    run = ["--steps", "5", "--warmup", "2", "--no-cpu-baseline",
+           "--no-profile", "--no-traffic", "--precision", args.precision,
+           "--cells", str(args.cells), "--seed", str(args.seed)]
+    probe = ["--copy-probe", "256"]
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        k = _pmc_pass(counter, os.path.join(base, counter), run)
+        c = _pmc_pass(counter, os.path.join(base, "copy_" + counter), probe,
+                      timeout=120)
+        if not k or not c:
+            return None
+        kv = [v for name, v in k.items() if kernel_key in name]
+        cv = [v for name, v in c.items() if "copyBuffer" in name]
+        if not kv or not cv or cv[0] <= 0:
+            return None
+        res[counter] = kv[0] * (256 * 1024 / cv[0]) * 1024  # bytes
+    return dict(bytes=res["FETCH_SIZE"] + res["WRITE_SIZE"],
+                fetch_bytes=res["FETCH_SIZE"], write_bytes=res["WRITE_SIZE"])
+
+
+def drift_run(args):
+    """NVE energy drift: the 256,000-atom EAM crystal (40^3 cells, 600 K,
+    displacements N(0, 0.02 A), seed 1001, dt 1 fs, skin 0.4 A) for
+    args.drift steps in each precision.  E_tot per atom = (Epot + Ekin) / N
+    every 10 steps; drift = least-squares slope x 1000 steps / |<E_tot>|."""
+    from imd_amd import imdg as G
+    eam = W.johnson_eam()
+    s = W.eam_crystal(n_cells=40, temp_k=600.0, seed=1001, displace=0.02)
+    out = {"metric": "NVE total-energy drift per 1000 steps (relative)",
+           "config": {"workload": "fcc EAM crystal 40^3 cells (%d atoms), "
+                                  "600 K, dt 1 fs, skin 0.4 A, NVE" % s.n,
+                      "steps": args.drift, "sample_every": 10}}
+    for prec, name in ((G.MIXED, "mixed"), (G.FP64, "fp64")):
+        ctx = G.eam_context(eam, [s], precision=prec, ensemble=G.NVE,
+                            timestep=W.FS, nbl_margin=0.4)
+        e, done, t0 = [], 0, time.perf_counter()
+        while done < args.drift:
+            n = min(1000, args.drift - done)
+            ctx.run(n)
+            h = ctx.thermo_history(done, n)[:, 0, :]
+            for k in range(0, n, 10):
+                e.append((done + k, (h[k, 1] + h[k, 2]) / s.n))
+            done += n
+        el = time.perf_counter() - t0
+        ctx.close()
+        t = np.array([v[0] for v in e], dtype=np.float64)
+        y = np.array([v[1] for v in e], dtype=np.float64)
+        slope = np.polyfit(t, y, 1)[0]
+        out[name] = {"drift_per_1000_steps": float(slope * 1000 / abs(y.mean())),
+                     "etot_per_atom_mean": float(y.mean()),
+                     "etot_per_atom_span": float(y.max() - y.min()),
+                     "atom_steps_per_s": s.n * args.drift / el}
+    print(json.dumps(out))
+    return 0
+
+
+def precision_errors(G, eam, s):
+    """Per-quantity error of the mixed path against the FP64 path on the
+    same configuration (one force evaluation each): RMS relative error and
+    max absolute error over atoms for forces, rho, 2F'(rho) and per-atom
+    energy; relative error of the total energy and of the virial (the
+    configurational part of the pressure)."""
+    res = {}
+    for prec in (G.FP64, G.MIXED):
+        ctx = G.eam_context(eam, [s], precision=prec, ensemble=G.NVE,
+                            timestep=W.FS, nbl_margin=0.4)
+        ctx.calc_forces()
+        res[prec] = (ctx.atoms(), ctx.thermo())
+        ctx.close()
+    (a64, t64), (a32, t32) = res[G.FP64], res[G.MIXED]
+    out = {}
+    for key, name in (("f", "force"), ("rho", "rho"), ("df", "2F'(rho)"),
+                      ("epot", "energy per atom")):
+        ref = np.asarray(a64[key], dtype=np.float64)
+        d = np.asarray(a32[key], dtype=np.float64) - ref
+        scale = np.sqrt(np.mean(ref * ref))
+        out[name] = {"rms_rel": float(np.sqrt(np.mean(d * d)) / scale),
+                     "max_abs": float(np.max(np.abs(d))),
+                     "rms_ref": float(scale)}
+    out["total energy"] = {"rel": abs(t32["epot"] - t64["epot"]) / abs(t64["epot"])}
+    out["virial"] = {"rel": abs(t32["virial"] - t64["virial"]) / abs(t64["virial"])}
+    return out
+
+
+CONFIGS = {
+    "c0": "4,000-atom fcc LJ crystal (10^3 cells, a = 1.5496 sigma), LJ table "
+          "r^2 0.25-6.25 with 2,000 points, kT 0.7, NVE, dt 0.002, skin 0.3, FP64",
+    "c1": "256,000-atom fcc EAM crystal (40^3 cells), Johnson-form tables "
+          "5,000 points r_c 5.0 A, 600 K, NVE, dt 1 fs, skin 0.4 A",
+    "c3": "64 independent fcc EAM replicas, N log-uniform in [500, 50,000], "
+          "0-1% vacancies, 300-1,200 K, NVE, dt 1 fs, skin 0.4 A, mixed",
+    "c4": "8,000,000-atom binary fcc slab (200x200x50 cells, 70/30 types, "
+          "open z with 20 A vacuum), two-species Johnson EAM r_c 5.5 A, "
+          "NVT 500 K, dt 1 fs, skin 0.4 A, mixed",
+}
+
+
+def config_run(args):
+    """Throughput of one of the other configurations (one JSON line; the
+    default run measures the 2M-atom metric workload)."""
+    from imd_amd import imdg as G
+    name = args.config
+    runs = []
+    if name == "c0":
+        s = W.lj_crystal()
+        systems = [s]
+        make = lambda prec: G.pair_context(W.lj_table(), systems, precision=prec,
+                                           ensemble=G.NVE, timestep=0.002,
+                                           nbl_margin=0.3)
+        runs = [("fp64", G.FP64), ("mixed", G.MIXED)]
+    elif name == "c1":
+        eam = W.johnson_eam()
+        systems = [W.eam_crystal(n_cells=40, temp_k=600.0, seed=1001)]
+        make = lambda prec: G.eam_context(eam, systems, precision=prec,
+                                          ensemble=G.NVE, timestep=W.FS,
+                                          nbl_margin=0.4)
+        runs = [("mixed", G.MIXED), ("fp64", G.FP64)]
+    elif name == "c3":
+        eam = W.johnson_eam()
+        systems = W.replica_set(64, seed=1003)
+        make = lambda prec: G.eam_context(eam, systems, precision=prec,
+                                          ensemble=G.NVE, timestep=W.FS,
+                                          nbl_margin=0.4)
+        runs = [("mixed", G.MIXED), ("fp64", G.FP64)]
+    elif name == "c4":
+        eam = W.johnson_eam((W.CU, W.NI), rc=5.5)
+        systems = [W.alloy_slab()]
+        isq = 1.0 / (args.tau_fs * W.FS) ** 2
+
+        def make(prec):
+            ctx = G.eam_context(eam, systems, precision=prec, ensemble=G.NVT,
+                                timestep=W.FS, nbl_margin=0.4)
+            ctx.set_nvt(0, W.kT(500.0), 0.0, isq)
+            return ctx
+        runs = [("mixed", G.MIXED)]
+    else:
+        raise SystemExit("unknown config %s" % name)
+    natoms = sum(x.n for x in systems)
+    res = {}
+    for label, prec in runs:
+        ctx = make(prec)
+        ctx.run(args.warmup)
+        nb0 = ctx.info()["nbl_count"]
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.run(args.steps, sync=False)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        info = ctx.info()
+        ctx.close()
+        res[label] = {"atom_steps_per_s": natoms * args.steps / el,
+                      "ms_per_step": 1e3 * el / args.steps,
+                      "mean_neighbours": info["mean_k"],
+                      "list_rebuilds_timed": int(info["nbl_count"] - nb0)}
+    out = {"metric": "atom-steps/s", "config": {"workload": CONFIGS[name],
+                                                 "name": name,
+                                                 "natoms": natoms,
+                                                 "replicas": len(systems)},
+           "steps": args.steps, "warmup": args.warmup, "results": res}
+    if "mixed" in res and "fp64" in res:
+        out["mixed_over_fp64"] = (res["mixed"]["atom_steps_per_s"]
+                                  / res["fp64"]["atom_steps_per_s"])
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,24 +314,45 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB "
                          "(counter calibration) and exit")
-    ap.add_argument("--compare", action="store_true",
-                    help="also time the other precision (ratio in stderr)")
+    ap.add_argument("--drift", type=int, default=0, metavar="STEPS",
+                    help="only measure the NVE energy drift over STEPS steps "
+                         "(both precisions) and exit")
+    ap.add_argument("--config", choices=sorted(CONFIGS), default=None,
+                    help="measure one of the other configurations instead "
+                         "of the 2M-atom metric workload")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the timed run of the other precision and the "
+                         "per-quantity errors of mixed vs FP64")
     args = ap.parse_args(argv)
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.drift:
+        return drift_run(args)
+    if args.config:
+        return config_run(args)
 
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU "
-              "path (the halo-exchange design is described in DESIGN.md); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: this build runs the path on "
+              "a single GPU (several GPUs are a design, not code); run with "
+              "--gpus 1", file=sys.stderr)
         return 2
 
     mixed = args.precision == "mixed"
     eam, s = make_workload(args.cells, args.seed, args.temp)
     isq = 1.0 / (args.tau_fs * W.FS) ** 2
+
+    # the counter passes run as child processes before this process touches
+    # the GPU (a process that has initialised the GPU starts no programs)
+    traffic = None
+    if not args.no_traffic and not args.no_profile:
+        traffic = measure_traffic(args)
+
+    from imd_amd import imdg as G
 
     def run_one(prec_mixed, profile):
         ctx = G.eam_context(eam, [s], precision=G.MIXED if prec_mixed else G.FP64,
@@ -179,15 +400,25 @@ def main(argv=None):
         if "rebuild" in kt and kt["rebuild"][1] > 0:
             roofline["rebuild_ms_per_step"] = round(kt["rebuild"][0] / kt["rebuild"][1], 4)
 
+    if roofline is not None and traffic is not None:
+        # HBM bytes per launch: calibrated FETCH_SIZE + WRITE_SIZE
+        roofline["traffic"] = int(traffic["bytes"])
+        roofline["traffic_over_algorithmic"] = round(
+            traffic["bytes"] / roofline["bytes_per_launch"], 3)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(eam, s, args.cpu_steps, args.tau_fs, args.temp)
 
-    if args.compare:
+    compare = None
+    if not args.no_compare:
         el2, _, _, _, _ = run_one(not mixed, False)
         other = natoms * args.steps / el2
-        ratio = value / other if mixed else other / value
-        print("mixed/fp64 atom-steps/s ratio: %.3f" % ratio, file=sys.stderr)
+        compare = {"other_precision": "fp64" if mixed else "mixed",
+                   "other_value": round(other, 1),
+                   "mixed_over_fp64": round(value / other if mixed
+                                            else other / value, 3),
+                   "errors_mixed_vs_fp64": precision_errors(G, eam, s)}
 
     out = {
         "metric": METRIC,
@@ -214,6 +445,8 @@ def main(argv=None):
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if compare is not None:
+        out["precision_compare"] = compare
     print(json.dumps(out))
     return 0
 

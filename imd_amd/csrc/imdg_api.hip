@@ -117,6 +117,10 @@ struct imdg_ctx {
   int lds_frc_lo = 0, lds_frc_n = 0;   // knots staged by the force loop
   int grid_den = 0, grid_frc[2] = {0, 0}, grid_move = 0;
   int units_max = 0;
+  // one captured step (rebuild chain + density + force), replayed by
+  // imdg_run when graphs are on (IMDG_GRAPH=1) and profiling is off
+  bool use_graph = false;
+  hipGraphExec_t step_graph = nullptr;
   bool atoms_set = false;
   long target_steps = 0;
   // profiling
@@ -146,6 +150,12 @@ int fail(imdg_ctx* c, int code, const char* fmt, ...) {
       return fail(ctx, IMDG_ERR_HIP, "%s: %s (%s:%d)", #call,                \
                   hipGetErrorString(e_), __FILE__, __LINE__);                \
   } while (0)
+
+// Drop the captured step (its kernel arguments hold buffer pointers).
+void drop_graph(imdg_ctx* c) {
+  if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
+  c->step_graph = nullptr;
+}
 
 int dalloc(imdg_ctx* c, void** p, size_t bytes) {
   *p = nullptr;
@@ -529,6 +539,7 @@ int fix_overflow(imdg_ctx* c, int* fixed) {
   if (!h.frozen) return IMDG_OK;
   int newk = std::max(c->kmax + 8, (int)ceil(h.kmax_seen * 1.2));
   newk = (newk + 7) & ~7;
+  drop_graph(c);
   dfree(c, c->d_nbr);
   c->d_nbr = nullptr;
   c->kmax = newk;
@@ -629,6 +640,8 @@ int imdg_ctx_create(const imdg_config* cfg, imdg_ctx** out) {
   c->mixed = cfg->precision == IMDG_MIXED;
   const char* remap = getenv("IMDG_XCD_REMAP");
   c->xcd_remap = remap ? atoi(remap) != 0 : 1;
+  const char* graph = getenv("IMDG_GRAPH");
+  c->use_graph = graph ? atoi(graph) != 0 : false;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 ||
       cfg->device < 0 || cfg->device >= ndev) {
@@ -651,6 +664,7 @@ int imdg_ctx_destroy(imdg_ctx* c) {
   hipSetDevice(c->dev);
   if (c->stream) hipStreamSynchronize(c->stream);
   collect_profile(c);
+  drop_graph(c);
   for (auto e : c->event_pool) hipEventDestroy(e);
   free_all(c);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -687,6 +701,7 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
   if (nrep < 1 || !reps || !natoms || !x || !type || !mass)
     return fail(c, IMDG_ERR_ARG, "bad set_atoms arguments");
   hipSetDevice(c->dev);
+  drop_graph(c);
   free_all(c);
   c->atoms_set = false;
   c->nrep = nrep;
@@ -1019,19 +1034,45 @@ int imdg_deallocate_nblist(imdg_ctx* c) {
   if (!c) return IMDG_ERR_ARG;
   if (!c->atoms_set) return IMDG_OK;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graph(c);
   dfree(c, c->d_nbr);
   c->d_nbr = nullptr;
   c->nbr_bytes = 0;
   return flag_all_for_rebuild(c);
 }
 
+// Capture one fused step into a graph: the launches only, no memory
+// operations (nothing in the step clears memory with a memset).
+static int capture_step(imdg_ctx* c) {
+  hipGraph_t g = nullptr;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_build(c);
+  if (!rc) rc = enqueue_forces(c, true);
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HIPCHK(c, e);
+  e = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIPCHK(c, e);
+  return IMDG_OK;
+}
+
 int imdg_run(imdg_ctx* c, long nsteps) {
   if (!c || nsteps < 0) return IMDG_ERR_ARG;
   int rc = ensure_ready(c);
   if (rc) return rc;
-  for (long s = 0; s < nsteps; s++) {
-    if ((rc = enqueue_build(c))) return rc;
-    if ((rc = enqueue_forces(c, true))) return rc;
+  if (c->use_graph && !c->profiling && nsteps > 0) {
+    if (!c->step_graph && (rc = capture_step(c))) return rc;
+    for (long s = 0; s < nsteps; s++)
+      HIPCHK(c, hipGraphLaunch(c->step_graph, c->stream));
+  } else {
+    for (long s = 0; s < nsteps; s++) {
+      if ((rc = enqueue_build(c))) return rc;
+      if ((rc = enqueue_forces(c, true))) return rc;
+    }
   }
   c->target_steps += nsteps;
   return IMDG_OK;

@@ -27,38 +27,65 @@ __device__ __forceinline__ int block_replica(const int* blk_rep) {
 }
 
 // fix_cells/do_boundaries + cell_coord, for replicas that need a rebuild.
+// The atoms of a block are counted per cell in an LDS hash table first, so
+// the global counters see one atomic per distinct cell of the block (the
+// slot of an atom inside its cell is base + its rank in the block; the
+// order inside a cell is fixed later by k_cellsort).
+#define BIN_HASH 256
+
 __global__ void __launch_bounds__(IMDG_BLOCK) k_bin(BuildArgs a) {
 #pragma clang fp contract(off)
+  __shared__ int hkey[BIN_HASH], hcnt[BIN_HASH], hbase[BIN_HASH];
   if (!a.ctl->any_rebuild || a.ctl->frozen) return;
   int r = block_replica(a.blk_rep);
   if (!a.rst[r].need_rebuild) return;
   const RepDev& R = a.rep[r];
+  for (int e = threadIdx.x; e < BIN_HASH; e += IMDG_BLOCK) {
+    hkey[e] = -1;
+    hcnt[e] = 0;
+  }
+  __syncthreads();
   int i = blockIdx.x * IMDG_BLOCK + threadIdx.x;
-  if (i - R.atom_off >= R.natoms) return;
-  double4* pos = a.pos[a.ctl->pos_cur];
-  double4 p = pos[i];
-  double x[3] = {p.x, p.y, p.z};
-  for (int d = 0; d < 3; d++) {
-    if (R.pbc[d] != 1) continue;
-    double f = -floor(x[0] * R.tbox[d][0] + x[1] * R.tbox[d][1] +
-                      x[2] * R.tbox[d][2]);
-    x[0] += f * R.box[d][0];
-    x[1] += f * R.box[d][1];
-    x[2] += f * R.box[d][2];
+  bool active = i - R.atom_off < R.natoms;
+  int h = -1, rank = 0;
+  if (active) {
+    double4* pos = a.pos[a.ctl->pos_cur];
+    double4 p = pos[i];
+    double x[3] = {p.x, p.y, p.z};
+    for (int d = 0; d < 3; d++) {
+      if (R.pbc[d] != 1) continue;
+      double f = -floor(x[0] * R.tbox[d][0] + x[1] * R.tbox[d][1] +
+                        x[2] * R.tbox[d][2]);
+      x[0] += f * R.box[d][0];
+      x[1] += f * R.box[d][1];
+      x[2] += f * R.box[d][2];
+    }
+    pos[i] = make_double4(x[0], x[1], x[2], p.w);
+    int c[3];
+    for (int d = 0; d < 3; d++) {
+      c[d] = (int)(R.gdim[d] * (x[0] * R.tbox[d][0] + x[1] * R.tbox[d][1] +
+                                x[2] * R.tbox[d][2]));
+      if (c[d] >= R.gdim[d])
+        c[d] = R.gdim[d] - 1;
+      else if (c[d] < 0)
+        c[d] = 0;
+    }
+    int cell = R.cell_off + (c[0] * R.gdim[1] + c[1]) * R.gdim[2] + c[2];
+    a.cellid[i] = cell;
+    // at most IMDG_BLOCK distinct keys in BIN_HASH entries: probing ends
+    h = (int)(((unsigned)cell * 2654435761u) >> 24) & (BIN_HASH - 1);
+    for (;;) {
+      int old = atomicCAS(&hkey[h], -1, cell);
+      if (old == -1 || old == cell) break;
+      h = (h + 1) & (BIN_HASH - 1);
+    }
+    rank = atomicAdd(&hcnt[h], 1);
   }
-  pos[i] = make_double4(x[0], x[1], x[2], p.w);
-  int c[3];
-  for (int d = 0; d < 3; d++) {
-    c[d] = (int)(R.gdim[d] * (x[0] * R.tbox[d][0] + x[1] * R.tbox[d][1] +
-                              x[2] * R.tbox[d][2]));
-    if (c[d] >= R.gdim[d])
-      c[d] = R.gdim[d] - 1;
-    else if (c[d] < 0)
-      c[d] = 0;
-  }
-  int cell = R.cell_off + (c[0] * R.gdim[1] + c[1]) * R.gdim[2] + c[2];
-  a.cellid[i] = cell;
-  a.cslot[i] = atomicAdd(&a.cell_cnt[cell], 1);
+  __syncthreads();
+  for (int e = threadIdx.x; e < BIN_HASH; e += IMDG_BLOCK)
+    if (hkey[e] >= 0) hbase[e] = atomicAdd(&a.cell_cnt[hkey[e]], hcnt[e]);
+  __syncthreads();
+  if (active) a.cslot[i] = hbase[h] + rank;
 }
 
 // Exclusive scan of the cell counts, two levels in one launch: every block
@@ -153,27 +180,46 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_place(BuildArgs a) {
 }
 
 // Per cell: first slot and count, counter reset, and the atoms of the cell
-// ordered by input index (deterministic list order).
+// ordered by input index (deterministic list order).  32 lanes per cell:
+// each lane takes one atom and writes it at its rank among the cell's input
+// indices (unique); a cell with more atoms is sorted by its first lane.
+#define SORT_LANES 32
+
 __global__ void __launch_bounds__(256) k_cellsort(BuildArgs a) {
   if (!a.ctl->any_rebuild || a.ctl->frozen) return;
   const int* aid = a.aid[a.ctl->rec_cur];
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < a.ncells;
-       c += gridDim.x * blockDim.x) {
+  int seg = threadIdx.x / SORT_LANES, l = threadIdx.x % SORT_LANES;
+  int base = (threadIdx.x % IMDG_WAVE) - l;  // first lane of the segment
+  for (int c = blockIdx.x * (blockDim.x / SORT_LANES) + seg; c < a.ncells;
+       c += gridDim.x * (blockDim.x / SORT_LANES)) {
     int r = a.cell_rep[c];
-    if (!a.rst[r].need_rebuild) continue;
+    if (!a.rst[r].need_rebuild) continue;  // the whole segment skips
     const RepDev& R = a.rep[r];
     int b = cell_first_slot(a, R, c), n = a.cell_cnt[c];
-    a.cell_beg[c] = b;
-    a.cell_num[c] = n;
-    a.cell_cnt[c] = 0;
     int* p = a.perm + b;
-    for (int k = 1; k < n; k++) {
-      int v = p[k], key = aid[v], m = k - 1;
-      while (m >= 0 && aid[p[m]] > key) {
-        p[m + 1] = p[m];
-        m--;
+    if (l == 0) {
+      a.cell_beg[c] = b;
+      a.cell_num[c] = n;
+      a.cell_cnt[c] = 0;
+    }
+    if (n <= SORT_LANES) {
+      int v = l < n ? p[l] : 0;
+      int key = l < n ? aid[v] : 0x7fffffff;
+      int rank = 0;
+      for (int m = 0; m < n; m++) {
+        int km = __shfl(key, base + m, IMDG_WAVE);
+        rank += km < key;
       }
-      p[m + 1] = v;
+      if (l < n) p[rank] = v;
+    } else if (l == 0) {
+      for (int k = 1; k < n; k++) {
+        int v = p[k], key = aid[v], m = k - 1;
+        while (m >= 0 && aid[p[m]] > key) {
+          p[m + 1] = p[m];
+          m--;
+        }
+        p[m + 1] = v;
+      }
     }
   }
 }
@@ -231,6 +277,7 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_permute(BuildArgs a) {
 
 __global__ void __launch_bounds__(TILE_WAVES * IMDG_WAVE) k_tiles(BuildArgs a) {
   __shared__ int srb[TILE_WAVES][TILE_MAXRANGE], sre[TILE_WAVES][TILE_MAXRANGE];
+  __shared__ int4 srun[TILE_WAVES][IMDG_MAXRUN];
   if (!a.ctl->any_rebuild || a.ctl->frozen) return;
   int w = threadIdx.x / IMDG_WAVE, l = threadIdx.x % IMDG_WAVE;
   int b = blockIdx.x * TILE_WAVES + w;
@@ -332,22 +379,26 @@ __global__ void __launch_bounds__(TILE_WAVES * IMDG_WAVE) k_tiles(BuildArgs a) {
     re[m + 1] = ve;
   }
   int nrun = 0, tot = 0;
+  int4* run = srun[w];
   for (int k = 0; k < nr && ok; k++) {
-    if (nrun > 0 && rb[k] <= T.run[nrun - 1].x + T.run[nrun - 1].y) {
-      int4& u = T.run[nrun - 1];
+    if (nrun > 0 && rb[k] <= run[nrun - 1].x + run[nrun - 1].y) {
+      int4 u = run[nrun - 1];
       int end = max(u.x + u.y, re[k]);
       tot += end - (u.x + u.y);
       u.y = end - u.x;
+      run[nrun - 1] = u;
     } else {
       if (nrun >= IMDG_MAXRUN) {
         ok = false;
         break;
       }
-      T.run[nrun] = make_int4(rb[k], re[k] - rb[k], tot, 0);
+      run[nrun] = make_int4(rb[k], re[k] - rb[k], tot, 0);
       tot += re[k] - rb[k];
       nrun++;
     }
   }
+  if (ok && tot <= IMDG_TILE_CAP)
+    for (int k = 0; k < nrun; k++) T.run[k] = run[k];
   if (!ok || tot > IMDG_TILE_CAP) {
     T.ok = 0;
     T.nrun = 0;
@@ -509,71 +560,64 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
       float hi = cs * (float)(1.0 + IMDG_SCREEN_BAND);
       uint4* lst = reinterpret_cast<uint4*>(a.nbr) +
                    ((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63);
+      // the 27 neighbour cells (code k = (ox+1)*9 + (oy+1)*3 + (oz+1)):
+      // first slot, count, tile position and image shift, all loads in
+      // flight together; a cell beyond an open face has count 0
+      int jb[27], jn[27], tb[27], sxyz[27];
+#pragma unroll
+      for (int k = 0; k < 27; k++) {
+        const int ox = k / 9 - 1, oy = (k / 3) % 3 - 1, oz = k % 3 - 1;
+        int n0 = cx + ox, n1 = cy + oy, n2 = cz + oz;
+        int sx = n0 < 0 ? -1 : (n0 >= g0 ? 1 : 0);
+        int sy = n1 < 0 ? -1 : (n1 >= g1 ? 1 : 0);
+        int sz = n2 < 0 ? -1 : (n2 >= g2 ? 1 : 0);
+        bool valid = (sx == 0 || R.pbc[0] == 1) && (sy == 0 || R.pbc[1] == 1) &&
+                     (sz == 0 || R.pbc[2] == 1);
+        n0 -= sx * g0;
+        n1 -= sy * g1;
+        n2 -= sz * g2;
+        int nc = valid ? R.cell_off + (n0 * g1 + n1) * g2 + n2 : R.cell_off;
+        int cb = a.cell_beg[nc], cn = a.cell_num[nc];
+        jb[k] = cb;
+        jn[k] = valid ? cn : 0;
+        sxyz[k] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+      }
+#pragma unroll
+      for (int k = 0; k < 27; k++)
+        tb[k] = (tok && jn[k] > 0) ? tile_index(truns, nrun, jb[k]) : jb[k];
       uint4 q = make_uint4(0u, 0u, 0u, 0u);
       int cnt = 0;
-      for (int ox = -1; ox <= 1; ox++) {
-        int n0 = cx + ox, sx = 0;
-        if (n0 < 0) {
-          if (R.pbc[0] != 1) continue;
-          n0 += g0;
-          sx = -1;
-        } else if (n0 >= g0) {
-          if (R.pbc[0] != 1) continue;
-          n0 -= g0;
-          sx = 1;
-        }
-        for (int oy = -1; oy <= 1; oy++) {
-          int n1 = cy + oy, sy = 0;
-          if (n1 < 0) {
-            if (R.pbc[1] != 1) continue;
-            n1 += g1;
-            sy = -1;
-          } else if (n1 >= g1) {
-            if (R.pbc[1] != 1) continue;
-            n1 -= g1;
-            sy = 1;
+#pragma unroll
+      for (int k = 0; k < 27; k++) {
+        // owner: the partner's cell lies in this atom's half stencil
+        // (offset lexicographically >= 0, i.e. k >= 13)
+        const bool self = (k == 13), owner = (k >= 13);
+        const uint32_t code = (uint32_t)k;
+        float o0 = off[3 * k], o1 = off[3 * k + 1], o2 = off[3 * k + 2];
+        for (int t = 0; t < jn[k]; t++) {
+          int j = jb[k] + t;
+          if (self && j == i) continue;
+          float4 uj = tok ? trec[tb[k] + t] : rec32(a, j);
+          float d0 = (uj.x - ui.x) + o0;
+          float d1 = (uj.y - ui.y) + o1;
+          float d2 = (uj.z - ui.z) + o2;
+          float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+          bool in = r2 < lo;
+          if (!in && r2 < hi) {
+            int sh = sxyz[k];
+            in = exact_pair(R, pos, i, j, sh % 3 - 1, (sh / 3) % 3 - 1,
+                            sh / 9 - 1, owner);
           }
-          for (int oz = -1; oz <= 1; oz++) {
-            int n2 = cz + oz, sz = 0;
-            if (n2 < 0) {
-              if (R.pbc[2] != 1) continue;
-              n2 += g2;
-              sz = -1;
-            } else if (n2 >= g2) {
-              if (R.pbc[2] != 1) continue;
-              n2 -= g2;
-              sz = 1;
-            }
-            bool self = (ox == 0 && oy == 0 && oz == 0);
-            bool owner =
-                ox > 0 || (ox == 0 && (oy > 0 || (oy == 0 && oz >= 0)));
-            uint32_t code = (uint32_t)((ox + 1) * 9 + (oy + 1) * 3 + (oz + 1));
-            float o0 = off[3 * code], o1 = off[3 * code + 1],
-                  o2 = off[3 * code + 2];
-            int nc = R.cell_off + (n0 * g1 + n1) * g2 + n2;
-            int jb = a.cell_beg[nc], je = jb + a.cell_num[nc];
-            int tb = (tok && je > jb) ? tile_index(truns, nrun, jb) : jb;
-            for (int j = jb; j < je; j++) {
-              if (self && j == i) continue;
-              float4 uj = tok ? trec[tb + (j - jb)] : rec32(a, j);
-              float d0 = (uj.x - ui.x) + o0;
-              float d1 = (uj.y - ui.y) + o1;
-              float d2 = (uj.z - ui.z) + o2;
-              float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-              bool in = r2 < lo;
-              if (!in && r2 < hi) in = exact_pair(R, pos, i, j, sx, sy, sz, owner);
-              if (in) {
-                uint32_t e = (uint32_t)j | (code << IMDG_IDX_BITS);
-                int s = cnt & 3;
-                q.x = s == 0 ? e : q.x;
-                q.y = s == 1 ? e : q.y;
-                q.z = s == 2 ? e : q.z;
-                q.w = s == 3 ? e : q.w;
-                cnt++;
-                if (s == 3 && cnt <= kmax)
-                  lst[(size_t)((cnt - 1) >> 2) * IMDG_WAVE] = q;
-              }
-            }
+          if (in) {
+            uint32_t e = (uint32_t)j | (code << IMDG_IDX_BITS);
+            int s4 = cnt & 3;
+            q.x = s4 == 0 ? e : q.x;
+            q.y = s4 == 1 ? e : q.y;
+            q.z = s4 == 2 ? e : q.z;
+            q.w = s4 == 3 ? e : q.w;
+            cnt++;
+            if (s4 == 3 && cnt <= kmax)
+              lst[(size_t)((cnt - 1) >> 2) * IMDG_WAVE] = q;
           }
         }
       }
@@ -582,10 +626,10 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
       // list fits whenever the list does)
       if (cnt <= kmax) {
         uint32_t pad = (uint32_t)i | ((uint32_t)IMDG_CODE_PAD << IMDG_IDX_BITS);
-        int s = cnt & 3;
-        if (s != 0) {
-          if (s <= 1) q.y = pad;
-          if (s <= 2) q.z = pad;
+        int s4 = cnt & 3;
+        if (s4 != 0) {
+          if (s4 <= 1) q.y = pad;
+          if (s4 <= 2) q.z = pad;
           q.w = pad;
           lst[(size_t)(cnt >> 2) * IMDG_WAVE] = q;
         }

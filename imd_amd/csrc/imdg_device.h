@@ -43,20 +43,28 @@ __device__ __forceinline__ void kahan_add(double& s, double& c, double x) {
   s = t;
 }
 
-template <typename R>
-__device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
+__device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float rmax0(float a) { return fmaxf(a, 0.0f); }
+__device__ __forceinline__ double rmax0(double a) { return fmax(a, 0.0); }
+__device__ __forceinline__ float rfract(float t) {
+  return __builtin_amdgcn_fractf(t);
+}
+__device__ __forceinline__ double rfract(double t) {
+  return __builtin_amdgcn_fract(t);
+}
 
 // Segment index and fraction of IMD's 3-point table access
 // (src/potaccess.h:329-340): argument clamped to [begin, end], segment
-// k = floor((x - begin) / step) capped at len - 1.
+// k = floor((x - begin) / step) capped at len - 1, fraction
+// chi = t - floor(t) before the cap (t >= 0, so floor(t) = (int)t; the
+// subtraction is exact).  A NaN argument clamps to segment 0.
 template <typename R>
 __device__ __forceinline__ int seg_locate(R x, R begin, R end, R invstep,
                                           int len, R& chi) {
-  R a = rmin(x, end) - begin;
-  a = a < (R)0 ? (R)0 : a;
-  R t = a * invstep;
+  R t = rmax0(rmin(x, end) - begin) * invstep;
   int k = (int)t;
-  chi = t - (R)k;
+  chi = rfract(t);
   return k < len - 1 ? k : len - 1;
 }
 

@@ -27,9 +27,11 @@ namespace imdg {
 // Threads per block of the force loop: its 4-neighbour load phases need
 // more than the 128 VGPRs a 1024-thread block leaves a wave, and with the
 // tables in LDS one block per CU is resident anyway.
-template <typename R>
+template <typename R, int MODE>
 struct ForceThreads {
-  static constexpr int value = sizeof(R) == 4 ? 768 : 512;
+  // single species in FP32 fits 128 VGPRs: 16 waves per CU
+  static constexpr int value =
+      sizeof(R) == 4 ? (MODE == 2 ? 768 : 1024) : 512;
 };
 
 // Neighbour-loop variants of the force kernel.
@@ -410,13 +412,26 @@ template <typename R, bool NT1, typename R4, typename R2>
 __device__ __forceinline__ void density_group(
     uint4 e4, const uint4* pf, uint4& slot, const R4* __restrict__ rec,
     const R* off, const R4& ui, int it, int nt, const R2* __restrict__ K,
-    const ColPar<R>* cpr, const DensityCol<R>& c0, R& rho, int& is_short) {
+    const ColPar<R>* cpr, const DensityCol<R>& c0, R& rho, int& is_short,
+    R& r2min) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
   R4 uj[4];
   R o[4][3];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    uj[u] = rec[en[u] & IMDG_IDX_MASK];
+    if (NT1) {
+      // one species: the type is not needed, gather 12 B instead of 16
+      const typename V2<R>::T* p2 =
+          reinterpret_cast<const typename V2<R>::T*>(rec +
+                                                     (en[u] & IMDG_IDX_MASK));
+      typename V2<R>::T xy = p2[0];
+      uj[u].x = xy.x;
+      uj[u].y = xy.y;
+      uj[u].z = reinterpret_cast<const R*>(p2)[2];
+      uj[u].w = (R)0;
+    } else {
+      uj[u] = rec[en[u] & IMDG_IDX_MASK];
+    }
     const R* op = off + 3 * (en[u] >> IMDG_IDX_BITS);
     o[u][0] = op[0];
     o[u][1] = op[1];
@@ -457,7 +472,10 @@ __device__ __forceinline__ void density_group(
   for (int u = 0; u < 4; u++) {
     bool in = r2[u] < ce[u];
     rho = fma(in ? (R)1 : (R)0, knot_val<R>(p[u], q[u], chi[u]), rho);
-    is_short |= (in && r2[u] < cb[u]);
+    if (NT1)
+      r2min = r2[u] < r2min ? r2[u] : r2min;  // r2 < begin implies in range
+    else
+      is_short |= (in && r2[u] < cb[u]);
   }
 }
 
@@ -691,10 +709,12 @@ __global__ void __launch_bounds__(IMDG_DENSITY_THREADS)
       const uint4* nb = list_of(a.nbr, i, a.kmax);
       R rho = (R)0;
       DensityCol<R> col0 = {kb0, len0, b0, e0, is0};
+      R r2min = (R)3.0e38;
       list_walk(nb, a.nnb[i], [&](uint4 e4, const uint4* pf, uint4& slot) {
         density_group<R, NT1>(e4, pf, slot, rec, off, ui, it, nt, K, cpr,
-                              col0, rho, is_short);
+                              col0, rho, is_short, r2min);
       });
+      if (NT1) is_short |= (r2min < b0);
       // embedding: PAIR_INT2 on the embedding column of the type, argument
       // rho (knots in HBM; one lookup per atom)
       const R2* __restrict__ ke = a.knots + te.off[it];
@@ -738,7 +758,7 @@ __global__ void __launch_bounds__(IMDG_DENSITY_THREADS)
 // (calc_forces, move_atoms and check_nblist of one main_loop iteration in
 // one launch).
 template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT>
-__global__ void __launch_bounds__(ForceThreads<R>::value)
+__global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
     k_force(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
   constexpr bool NT1 = MODE != FM_NTN;
   typedef typename V4<R>::T R4;
@@ -956,10 +976,17 @@ int step_grid_density(bool nt1, bool lt, size_t lds, int ncu) {
 }
 
 template <typename R>
+int force_threads(int mode) {
+  return mode == FM_SG    ? ForceThreads<R, FM_SG>::value
+         : mode == FM_NT1 ? ForceThreads<R, FM_NT1>::value
+                          : ForceThreads<R, FM_NTN>::value;
+}
+
+template <typename R>
 int step_grid_force(int mode, bool integrate, bool lt, bool hp, bool he,
                     size_t lds, int ncu) {
   return persistent_grid(force_fn<R>(mode, integrate, lt, hp, he),
-                         ForceThreads<R>::value, lds, ncu);
+                         force_threads<R>(mode), lds, ncu);
 }
 
 template <typename R>
@@ -992,7 +1019,7 @@ hipError_t launch_force(const StepArgs<R>& a, const TabDev<R>* tabs,
   void* kargs[] = {&aa, &tt};
   return hipLaunchKernel(
       force_fn<R>(mode, integrate, lt, a.has_pair != 0, a.has_eam != 0),
-      dim3(grid), dim3(ForceThreads<R>::value), kargs, lds, st);
+      dim3(grid), dim3(force_threads<R>(mode)), kargs, lds, st);
 }
 
 template <typename R>

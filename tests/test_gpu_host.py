@@ -1,0 +1,84 @@
+"""The IMD-style C host on the GPU: `imd_gpu -p file.param` reads IMD's
+parameter keys, format-2 tables and an ASCII configuration, runs the fused
+main loop through the C ABI and writes the `.eng` file; its rows must match
+the oracle's thermo at the same steps (FP64 path).  The executable runs as
+a child process."""
+import dataclasses
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from imd_amd import build as B
+from imd_amd import workloads as W
+from helpers import eam_tables, oracle_eam
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(B.HOST, "imd_gpu")
+
+
+def _read_eng(path):
+    rows = [l.split() for l in open(path) if l.strip() and not l.startswith("#")]
+    return np.array([[float(v) for v in r] for r in rows])
+
+
+@pytest.mark.parametrize("ensemble", ["nve", "nvt"])
+def test_imd_gpu_eng_matches_oracle(tmp_path, ensemble):
+    if not os.path.exists(EXE):
+        pytest.fail("imd_gpu is not built (build() first)")
+    eam = eam_tables()
+    s = W.eam_crystal(n_cells=5, temp_k=700.0, seed=21, displace=0.03)
+    # the host reads v = p/m from the file and forms p = v m: give the
+    # oracle the same momenta
+    s = dataclasses.replace(s, p=(s.p / s.mass[:, None]) * s.mass[:, None])
+    files = eam.write(str(tmp_path / "pot"))
+    cfg = s.write_config(str(tmp_path / "start.chkpt"))
+    nsteps, eng_int = 40, 10
+    kT = W.kT(700.0)
+    tau = 50 * W.FS
+    lines = [
+        "ensemble %s" % ensemble,
+        "ntypes 1",
+        "masses %.17g" % s.mass[0],
+        "timestep %.17g" % W.FS,
+        "startstep 0",
+        "maxsteps %d" % nsteps,
+        "eng_int %d" % eng_int,
+        "coordname %s" % cfg,
+        "outfiles %s" % str(tmp_path / "run"),
+        "core_potential_file %s" % files["core_potential_file"],
+        "embedding_energy_file %s" % files["embedding_energy_file"],
+        "atomic_e-density_file %s" % files["atomic_e-density_file"],
+        "nbl_margin 0.4",
+        "gpu_precision fp64",
+        "box_from_header 1",
+    ]
+    if ensemble == "nvt":
+        lines += ["starttemp %.17g" % kT, "tau_eta %.17g" % tau]
+    param = tmp_path / "run.param"
+    param.write_text("\n".join(lines) + "\n")
+    r = subprocess.run([EXE, "-p", str(param)], capture_output=True,
+                       text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr[-2000:]
+    eng = _read_eng(str(tmp_path / "run.eng"))
+    assert eng.shape[0] == nsteps // eng_int + 1
+
+    import oracle as O
+    kw = {}
+    if ensemble == "nvt":
+        kw = dict(ensemble=O.ORC_NVT, temperature=kT, eta=0.0,
+                  isq_tau_eta=1.0 / tau ** 2)
+    orc = oracle_eam(s, eam, nbl_margin=0.4, timestep=W.FS, **kw)
+    th = orc.run(nsteps + 1)  # steps 0..nsteps
+    n = s.n
+    for row in eng:
+        step = int(round(row[0] / W.FS))
+        epot = th[step, 0] / n
+        temp = 2.0 * th[step, 1] / (3 * n)
+        assert abs(row[1] - epot) <= 1e-9 * abs(epot), (step, row[1], epot)
+        # the temperature column is printed with 6 significant digits
+        assert abs(row[2] - temp) <= 1e-5 * abs(temp), (step, row[2], temp)
+    # the final configuration is written in IMD's format
+    assert os.path.exists(str(tmp_path / "run.end.chkpt"))
