@@ -464,11 +464,12 @@ __device__ __forceinline__ int tile_index(const int4* runs, int nrun, int j) {
 
 // The reference's decision for one candidate: r^2 of (image of the partner)
 // - (owner) in FP64 with separate roundings, owner being the atom whose
-// half stencil holds the other's cell.
-__device__ __forceinline__ bool exact_pair(const RepDev& R,
-                                           const double4* __restrict__ pos,
-                                           int i, int j, int sx, int sy,
-                                           int sz, bool owner) {
+// half stencil holds the other's cell.  Rare (candidates near the radius),
+// kept out of line.
+__device__ __noinline__ bool exact_pair(const RepDev& R,
+                                        const double4* __restrict__ pos,
+                                        int i, int j, int sx, int sy, int sz,
+                                        bool owner) {
 #pragma clang fp contract(off)
   double4 xi = pos[i], xj = pos[j];
   double d0, d1, d2;
@@ -489,10 +490,67 @@ __device__ __forceinline__ bool exact_pair(const RepDev& R,
   return r2 < R.cellsz;
 }
 
-__device__ __forceinline__ float4 rec32(const BuildArgs& a, int j) {
-  if (a.mixed) return reinterpret_cast<const float4*>(a.rec_a)[j];
-  double4 d = reinterpret_cast<const double4*>(a.rec_a)[j];
-  return make_float4((float)d.x, (float)d.y, (float)d.z, (float)d.w);
+// Cell-relative position of slot j in FP32.
+template <bool MIXED>
+__device__ __forceinline__ float3 rec_xyz(const BuildArgs& a, int j) {
+  if (MIXED) {
+    const float* p = reinterpret_cast<const float*>(a.rec_a) + 4 * (size_t)j;
+    return make_float3(p[0], p[1], p[2]);
+  }
+  const double* p = reinterpret_cast<const double*>(a.rec_a) + 4 * (size_t)j;
+  return make_float3((float)p[0], (float)p[1], (float)p[2]);
+}
+
+// What one lane carries through the candidate loops of its atom.
+struct ListLane {
+  int i;
+  float3 ui;
+  float lo, hi;
+  uint4* lst;
+  uint4 q;
+  int cnt, kmax;
+};
+
+__device__ __forceinline__ void list_append(ListLane& L, uint32_t e) {
+  int s4 = L.cnt & 3;
+  L.q.x = s4 == 0 ? e : L.q.x;
+  L.q.y = s4 == 1 ? e : L.q.y;
+  L.q.z = s4 == 2 ? e : L.q.z;
+  L.q.w = s4 == 3 ? e : L.q.w;
+  L.cnt++;
+  if (s4 == 3 && L.cnt <= L.kmax)
+    L.lst[(size_t)((L.cnt - 1) >> 2) * IMDG_WAVE] = L.q;
+}
+
+// Candidates of one neighbour cell, four at a time (their records in
+// flight together): FP32 screen, FP64 decision inside the band, append.
+template <bool MIXED, bool TOK>
+__device__ __forceinline__ void list_cell(
+    const BuildArgs& a, const RepDev& R, const double4* __restrict__ pos,
+    const float3* trec, ListLane& L, int jb, int jn, int tb, float o0,
+    float o1, float o2, bool self, bool owner, uint32_t code, int sh) {
+  for (int t0 = 0; t0 < jn; t0 += 4) {
+    float3 uj[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int t = min(t0 + u, jn - 1);
+      uj[u] = TOK ? trec[tb + t] : rec_xyz<MIXED>(a, jb + t);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int j = jb + t0 + u;
+      bool valid = (t0 + u < jn) && !(self && j == L.i);
+      float d0 = (uj[u].x - L.ui.x) + o0;
+      float d1 = (uj[u].y - L.ui.y) + o1;
+      float d2 = (uj[u].z - L.ui.z) + o2;
+      float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      bool in = valid && r2 < L.lo;
+      if (valid && !in && r2 < L.hi)
+        in = exact_pair(R, pos, L.i, j, sh % 3 - 1, (sh / 3) % 3 - 1,
+                        sh / 9 - 1, owner);
+      if (in) list_append(L, (uint32_t)j | (code << IMDG_IDX_BITS));
+    }
+  }
 }
 
 // Full neighbour list.  For each cell offset o of the 27, the reference
@@ -502,13 +560,15 @@ __device__ __forceinline__ float4 rec32(const BuildArgs& a, int j) {
 // expression decides membership here (directly, or after the FP32 screen
 // for candidates far from the list radius), so each unordered pair is in
 // the list of both atoms exactly when the reference has it in its half
-// list.  One block per 128-slot block; the FP32 records of its tile are
-// staged in LDS; entries are written four at a time.
+// list.  One block per 128-slot block; the FP32 positions of its tile are
+// staged in LDS (12 B each); entries are written four at a time.
+template <bool MIXED>
 __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
-  __shared__ float4 trec[IMDG_TILE_CAP];
+  __shared__ float3 trec[IMDG_TILE_CAP];
   __shared__ int4 truns[IMDG_MAXRUN];
   __shared__ float off[3 * IMDG_NCODES];
   __shared__ int tinfo[2];
+  __shared__ int wmax[IMDG_NWAVE];
   if (!a.ctl->any_rebuild || a.ctl->frozen) return;
   int b = blockIdx.x;
   int r = a.blk_rep[b];
@@ -533,14 +593,14 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
       // all of a thread's loads in flight together: 8 records per pass
       int k = 0;
       for (int t0 = threadIdx.x; t0 < tot; t0 += 8 * IMDG_BLOCK) {
-        float4 v[8];
+        float3 v[8];
         int t[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
           t[u] = t0 + u * IMDG_BLOCK;
           if (t[u] < tot) {
             while (t[u] >= truns[k].z + truns[k].y) k++;
-            v[u] = rec32(a, truns[k].x + (t[u] - truns[k].z));
+            v[u] = rec_xyz<MIXED>(a, truns[k].x + (t[u] - truns[k].z));
           }
         }
 #pragma unroll
@@ -554,12 +614,17 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
       int cx, cy, cz;
       cell_coords(R, a.acell[i], cx, cy, cz);
       int g0 = R.gdim[0], g1 = R.gdim[1], g2 = R.gdim[2];
-      float4 ui = rec32(a, i);
+      ListLane L;
+      L.i = i;
+      L.ui = rec_xyz<MIXED>(a, i);
       float cs = (float)R.cellsz;
-      float lo = cs * (float)(1.0 - IMDG_SCREEN_BAND);
-      float hi = cs * (float)(1.0 + IMDG_SCREEN_BAND);
-      uint4* lst = reinterpret_cast<uint4*>(a.nbr) +
-                   ((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63);
+      L.lo = cs * (float)(1.0 - IMDG_SCREEN_BAND);
+      L.hi = cs * (float)(1.0 + IMDG_SCREEN_BAND);
+      L.lst = reinterpret_cast<uint4*>(a.nbr) +
+              ((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63);
+      L.q = make_uint4(0u, 0u, 0u, 0u);
+      L.cnt = 0;
+      L.kmax = kmax;
       // the 27 neighbour cells (code k = (ox+1)*9 + (oy+1)*3 + (oz+1)):
       // first slot, count, tile position and image shift, all loads in
       // flight together; a cell beyond an open face has count 0
@@ -582,68 +647,46 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
         jn[k] = valid ? cn : 0;
         sxyz[k] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
       }
+      if (tok) {
 #pragma unroll
-      for (int k = 0; k < 27; k++)
-        tb[k] = (tok && jn[k] > 0) ? tile_index(truns, nrun, jb[k]) : jb[k];
-      uint4 q = make_uint4(0u, 0u, 0u, 0u);
-      int cnt = 0;
+        for (int k = 0; k < 27; k++)
+          tb[k] = jn[k] > 0 ? tile_index(truns, nrun, jb[k]) : 0;
 #pragma unroll
-      for (int k = 0; k < 27; k++) {
-        // owner: the partner's cell lies in this atom's half stencil
-        // (offset lexicographically >= 0, i.e. k >= 13)
-        const bool self = (k == 13), owner = (k >= 13);
-        const uint32_t code = (uint32_t)k;
-        float o0 = off[3 * k], o1 = off[3 * k + 1], o2 = off[3 * k + 2];
-        for (int t = 0; t < jn[k]; t++) {
-          int j = jb[k] + t;
-          if (self && j == i) continue;
-          float4 uj = tok ? trec[tb[k] + t] : rec32(a, j);
-          float d0 = (uj.x - ui.x) + o0;
-          float d1 = (uj.y - ui.y) + o1;
-          float d2 = (uj.z - ui.z) + o2;
-          float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-          bool in = r2 < lo;
-          if (!in && r2 < hi) {
-            int sh = sxyz[k];
-            in = exact_pair(R, pos, i, j, sh % 3 - 1, (sh / 3) % 3 - 1,
-                            sh / 9 - 1, owner);
-          }
-          if (in) {
-            uint32_t e = (uint32_t)j | (code << IMDG_IDX_BITS);
-            int s4 = cnt & 3;
-            q.x = s4 == 0 ? e : q.x;
-            q.y = s4 == 1 ? e : q.y;
-            q.z = s4 == 2 ? e : q.z;
-            q.w = s4 == 3 ? e : q.w;
-            cnt++;
-            if (s4 == 3 && cnt <= kmax)
-              lst[(size_t)((cnt - 1) >> 2) * IMDG_WAVE] = q;
-          }
-        }
+        for (int k = 0; k < 27; k++)
+          list_cell<MIXED, true>(a, R, pos, trec, L, jb[k], jn[k], tb[k],
+                                 off[3 * k], off[3 * k + 1], off[3 * k + 2],
+                                 k == 13, k >= 13, (uint32_t)k, sxyz[k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 27; k++)
+          list_cell<MIXED, false>(a, R, pos, trec, L, jb[k], jn[k], 0,
+                                  off[3 * k], off[3 * k + 1], off[3 * k + 2],
+                                  k == 13, k >= 13, (uint32_t)k, sxyz[k]);
       }
       // pad to a multiple of IMDG_LIST_PAD entries with far-away copies of
       // the atom itself (kmax is a multiple of IMDG_LIST_PAD, so the padded
       // list fits whenever the list does)
+      int cnt = L.cnt;
       if (cnt <= kmax) {
         uint32_t pad = (uint32_t)i | ((uint32_t)IMDG_CODE_PAD << IMDG_IDX_BITS);
+        uint4 q = L.q;
         int s4 = cnt & 3;
         if (s4 != 0) {
           if (s4 <= 1) q.y = pad;
           if (s4 <= 2) q.z = pad;
           q.w = pad;
-          lst[(size_t)(cnt >> 2) * IMDG_WAVE] = q;
+          L.lst[(size_t)(cnt >> 2) * IMDG_WAVE] = q;
         }
         int g = (cnt + 3) >> 2, gend = ((cnt + IMDG_LIST_PAD - 1) /
                                         IMDG_LIST_PAD) * (IMDG_LIST_PAD / 4);
         for (; g < gend; g++)
-          lst[(size_t)g * IMDG_WAVE] = make_uint4(pad, pad, pad, pad);
+          L.lst[(size_t)g * IMDG_WAVE] = make_uint4(pad, pad, pad, pad);
       }
       a.nnb[i] = cnt < kmax ? cnt : kmax;
       seen = cnt;
     }
   }
   // block max of the list length (k_list_done reduces the blocks)
-  __shared__ int wmax[IMDG_NWAVE];
   for (int o = 32; o > 0; o >>= 1)
     seen = max(seen, __shfl_xor(seen, o, IMDG_WAVE));
   if ((threadIdx.x % IMDG_WAVE) == 0) wmax[threadIdx.x / IMDG_WAVE] = seen;
@@ -726,7 +769,12 @@ void launch_build(const BuildArgs& b, int grid, hipStream_t st) {
   int gt = (b.nblk + TILE_WAVES - 1) / TILE_WAVES;
   hipLaunchKernelGGL(k_tiles, dim3(gt < 1 ? 1 : gt), dim3(TILE_WAVES * IMDG_WAVE),
                      0, st, b);
-  hipLaunchKernelGGL(k_list, dim3(b.nblk), dim3(IMDG_BLOCK), 0, st, b);
+  if (b.mixed)
+    hipLaunchKernelGGL((k_list<true>), dim3(b.nblk), dim3(IMDG_BLOCK), 0, st,
+                       b);
+  else
+    hipLaunchKernelGGL((k_list<false>), dim3(b.nblk), dim3(IMDG_BLOCK), 0, st,
+                       b);
   hipLaunchKernelGGL(k_list_done, dim3(1), dim3(1024), 0, st, b);
 }
 

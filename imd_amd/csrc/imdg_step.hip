@@ -29,9 +29,8 @@ namespace imdg {
 // tables in LDS one block per CU is resident anyway.
 template <typename R, int MODE>
 struct ForceThreads {
-  // single species in FP32 fits 128 VGPRs: 16 waves per CU
   static constexpr int value =
-      sizeof(R) == 4 ? (MODE == 2 ? 768 : 1024) : 512;
+      sizeof(R) == 4 ? 768 : 512;
 };
 
 // Neighbour-loop variants of the force kernel.
@@ -154,31 +153,87 @@ __device__ __forceinline__ void stage_colpar(const TabDev<R>& t, int klo,
 // finalisation (last block of a step kernel)
 // ---------------------------------------------------------------------------
 
+// Per-replica end of a step from the replica's summed partials (tot[PB_N]
+// is the embedding-energy sum): energies, thermostat, skin decision, thermo
+// record.  Returns whether the replica needs a list rebuild.
+template <typename R>
+__device__ bool replica_step_end(StepArgs<R>& a, int r, const double* tot,
+                                 bool forces, bool moved) {
+  const RepDev& Rp = a.rep[r];
+  RepState& st = a.rst[r];
+  if (forces) {
+    st.epot = tot[PB_EPAIR] + tot[PB_N];
+    st.virial = tot[PB_VIRIAL];
+  }
+  if (!moved) return false;
+  double ekin;
+  if (a.ensemble == 0) {
+    ekin = tot[PB_KE];
+  } else {
+    ekin = (tot[PB_E1] + tot[PB_E2]) / 4.0;
+    double ttt = Rp.nactive * Rp.temperature;
+    st.eta += a.dt * (tot[PB_E2] / ttt - 1.0) * Rp.isq_tau_eta;
+  }
+  int need = tot[PB_MAXD] > Rp.skin2;
+  st.need_rebuild = need;
+  long step = a.ctl->steps_done;
+  ThermoRec& tr = a.ring[(size_t)(step % IMDG_RING) * a.nrep + r];
+  tr.epot = st.epot;
+  tr.ekin = ekin;
+  tr.virial = st.virial;
+  tr.eta = st.eta;
+  tr.max_disp2 = tot[PB_MAXD];
+  tr.step = step;
+  tr.rebuilt = st.rebuilt;
+  tr.natoms = Rp.natoms;
+  st.rebuilt = 0;
+  return need != 0;
+}
+
+// Wavefront-wide compensated combine of per-lane (sum, compensation)
+// pairs in a fixed shuffle order (TwoSum keeps the error); max for
+// PB_MAXD.  Result valid in every lane.
+__device__ __forceinline__ void wave_combine(double (&s)[PB_N + 1],
+                                             double (&c)[PB_N + 1]) {
+#pragma unroll
+  for (int q = 0; q <= PB_N; q++) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      double so = __shfl_xor(s[q], o, IMDG_WAVE);
+      double co = __shfl_xor(c[q], o, IMDG_WAVE);
+      if (q == PB_MAXD) {
+        s[q] = fmax(s[q], so);
+      } else {
+        double t, e;
+        two_sum(s[q], so, t, e);
+        s[q] = t;
+        c[q] = c[q] + co + e;
+      }
+    }
+  }
+}
+
 // Compensated sums of the partials of every replica (fixed order, so the
 // result does not depend on which block finishes last), thermostat, skin
-// decision, thermo record.
+// decision, thermo record.  One replica: the whole block sums its
+// wavefront partials.  Several replicas: one wavefront per replica sums
+// that replica's chunk partials, with no block barrier per replica.
 template <typename R>
 __device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved) {
   __shared__ double lred[IMDG_STEP_WAVES][PB_N + 1][2];
+  __shared__ int any_s;
   int nw = blockDim.x / IMDG_WAVE;
   int w = threadIdx.x / IMDG_WAVE, l = threadIdx.x % IMDG_WAVE;
-  int unitsB = gridDim.x * nw;
-  bool any = false;
-  for (int r = 0; r < a.nrep; r++) {
-    const RepDev& Rp = a.rep[r];
-    int b0, b1, a0, a1;
-    if (a.part_chunk) {
-      b0 = a0 = Rp.blk_off * (IMDG_BLOCK / IMDG_CHUNK);
-      b1 = a1 = (Rp.blk_off + Rp.nblk) * (IMDG_BLOCK / IMDG_CHUNK);
-    } else {
-      b0 = a0 = 0;
-      b1 = unitsB;
-      a1 = a.unitsA;
-    }
+  if (threadIdx.x == 0) any_s = 0;
+  __syncthreads();
+  if (!a.part_chunk) {
+    // one replica: partials per wavefront of the finalising kernel (B) and
+    // of the density kernel (A)
+    int unitsB = gridDim.x * nw;
     double s[PB_N + 1], c[PB_N + 1];
 #pragma unroll
     for (int q = 0; q <= PB_N; q++) s[q] = c[q] = 0.0;
-    for (int u = b0 + threadIdx.x; u < b1; u += blockDim.x) {
+    for (int u = threadIdx.x; u < unitsB; u += blockDim.x) {
       const double* pb = a.partB + (size_t)u * PB_N;
 #pragma unroll
       for (int q = 0; q < PB_N; q++) {
@@ -189,25 +244,12 @@ __device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved) {
       }
     }
     if (forces && a.has_eam)
-      for (int u = a0 + threadIdx.x; u < a1; u += blockDim.x)
+      for (int u = threadIdx.x; u < a.unitsA; u += blockDim.x)
         kahan_add(s[PB_N], c[PB_N], a.partA[u]);
-    // combine across the block in a fixed order (TwoSum keeps the error)
+    wave_combine(s, c);
+    if (l == 0) {
 #pragma unroll
-    for (int q = 0; q <= PB_N; q++) {
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        double so = __shfl_xor(s[q], o, IMDG_WAVE);
-        double co = __shfl_xor(c[q], o, IMDG_WAVE);
-        if (q == PB_MAXD) {
-          s[q] = fmax(s[q], so);
-        } else {
-          double t, e;
-          two_sum(s[q], so, t, e);
-          s[q] = t;
-          c[q] = c[q] + co + e;
-        }
-      }
-      if (l == 0) {
+      for (int q = 0; q <= PB_N; q++) {
         lred[w][q][0] = s[q];
         lred[w][q][1] = c[q];
       }
@@ -230,41 +272,42 @@ __device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved) {
         }
         tot[q] = (q == PB_MAXD) ? ss : ss + cc;
       }
-      RepState& st = a.rst[r];
-      if (forces) {
-        st.epot = tot[PB_EPAIR] + tot[PB_N];
-        st.virial = tot[PB_VIRIAL];
-      }
-      if (moved) {
-        double ekin;
-        if (a.ensemble == 0) {
-          ekin = tot[PB_KE];
-        } else {
-          ekin = (tot[PB_E1] + tot[PB_E2]) / 4.0;
-          double ttt = Rp.nactive * Rp.temperature;
-          st.eta += a.dt * (tot[PB_E2] / ttt - 1.0) * Rp.isq_tau_eta;
+      if (replica_step_end(a, 0, tot, forces, moved)) any_s = 1;
+    }
+  } else {
+    // several replicas: one wavefront per replica, chunk partials
+    for (int r = w; r < a.nrep; r += nw) {
+      const RepDev& Rp = a.rep[r];
+      int c0 = Rp.blk_off * (IMDG_BLOCK / IMDG_CHUNK);
+      int c1 = (Rp.blk_off + Rp.nblk) * (IMDG_BLOCK / IMDG_CHUNK);
+      double s[PB_N + 1], c[PB_N + 1];
+#pragma unroll
+      for (int q = 0; q <= PB_N; q++) s[q] = c[q] = 0.0;
+      for (int u = c0 + l; u < c1; u += IMDG_WAVE) {
+        const double* pb = a.partB + (size_t)u * PB_N;
+#pragma unroll
+        for (int q = 0; q < PB_N; q++) {
+          if (q == PB_MAXD)
+            s[q] = fmax(s[q], pb[q]);
+          else
+            kahan_add(s[q], c[q], pb[q]);
         }
-        int need = tot[PB_MAXD] > Rp.skin2;
-        st.need_rebuild = need;
-        any |= need;
-        long step = a.ctl->steps_done;
-        ThermoRec& tr = a.ring[(size_t)(step % IMDG_RING) * a.nrep + r];
-        tr.epot = st.epot;
-        tr.ekin = ekin;
-        tr.virial = st.virial;
-        tr.eta = st.eta;
-        tr.max_disp2 = tot[PB_MAXD];
-        tr.step = step;
-        tr.rebuilt = st.rebuilt;
-        tr.natoms = Rp.natoms;
-        st.rebuilt = 0;
+        if (forces && a.has_eam) kahan_add(s[PB_N], c[PB_N], a.partA[u]);
+      }
+      wave_combine(s, c);
+      if (l == 0) {
+        double tot[PB_N + 1];
+#pragma unroll
+        for (int q = 0; q <= PB_N; q++)
+          tot[q] = (q == PB_MAXD) ? s[q] : s[q] + c[q];
+        if (replica_step_end(a, r, tot, forces, moved)) atomicOr(&any_s, 1);
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
     if (moved) {
-      a.ctl->any_rebuild = any ? 1 : 0;
+      a.ctl->any_rebuild = any_s ? 1 : 0;
       a.ctl->steps_done += 1;
     }
     __threadfence();
