@@ -506,20 +506,23 @@ struct ListLane {
   int i;
   float3 ui;
   float lo, hi;
-  uint4* lst;
-  uint4 q;
+  uint32_t* lst;  // first entry of the atom's grouped list
   int cnt, kmax;
+  int cx, cy, cz;  // the atom's cell (image shifts of the FP64 decision)
 };
 
+// Entry m of the lane's list: group m/4 of the lane, slot m%4.
 __device__ __forceinline__ void list_append(ListLane& L, uint32_t e) {
-  int s4 = L.cnt & 3;
-  L.q.x = s4 == 0 ? e : L.q.x;
-  L.q.y = s4 == 1 ? e : L.q.y;
-  L.q.z = s4 == 2 ? e : L.q.z;
-  L.q.w = s4 == 3 ? e : L.q.w;
+  if (L.cnt < L.kmax)
+    L.lst[(size_t)(L.cnt >> 2) * (IMDG_WAVE * IMDG_GROUP) + (L.cnt & 3)] = e;
   L.cnt++;
-  if (s4 == 3 && L.cnt <= L.kmax)
-    L.lst[(size_t)((L.cnt - 1) >> 2) * IMDG_WAVE] = L.q;
+}
+
+// Image shift of neighbour cell offset o along an axis: -1 / +1 when the
+// cell index wraps below 0 / beyond the grid.
+__device__ __forceinline__ int wrap_shift(int c, int o, int g) {
+  int n = c + o;
+  return n < 0 ? -1 : (n >= g ? 1 : 0);
 }
 
 // Candidates of one neighbour cell, four at a time (their records in
@@ -528,7 +531,7 @@ template <bool MIXED, bool TOK>
 __device__ __forceinline__ void list_cell(
     const BuildArgs& a, const RepDev& R, const double4* __restrict__ pos,
     const float3* trec, ListLane& L, int jb, int jn, int tb, float o0,
-    float o1, float o2, bool self, bool owner, uint32_t code, int sh) {
+    float o1, float o2, bool self, bool owner, uint32_t code) {
   for (int t0 = 0; t0 < jn; t0 += 4) {
     float3 uj[4];
 #pragma unroll
@@ -546,8 +549,10 @@ __device__ __forceinline__ void list_cell(
       float r2 = d0 * d0 + d1 * d1 + d2 * d2;
       bool in = valid && r2 < L.lo;
       if (valid && !in && r2 < L.hi)
-        in = exact_pair(R, pos, L.i, j, sh % 3 - 1, (sh / 3) % 3 - 1,
-                        sh / 9 - 1, owner);
+        in = exact_pair(R, pos, L.i, j,
+                        wrap_shift(L.cx, (int)code / 9 - 1, R.gdim[0]),
+                        wrap_shift(L.cy, ((int)code / 3) % 3 - 1, R.gdim[1]),
+                        wrap_shift(L.cz, (int)code % 3 - 1, R.gdim[2]), owner);
       if (in) list_append(L, (uint32_t)j | (code << IMDG_IDX_BITS));
     }
   }
@@ -620,15 +625,17 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
       float cs = (float)R.cellsz;
       L.lo = cs * (float)(1.0 - IMDG_SCREEN_BAND);
       L.hi = cs * (float)(1.0 + IMDG_SCREEN_BAND);
-      L.lst = reinterpret_cast<uint4*>(a.nbr) +
-              ((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63);
-      L.q = make_uint4(0u, 0u, 0u, 0u);
+      L.lst = a.nbr + (((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63)) *
+                          IMDG_GROUP;
       L.cnt = 0;
       L.kmax = kmax;
+      L.cx = cx;
+      L.cy = cy;
+      L.cz = cz;
       // the 27 neighbour cells (code k = (ox+1)*9 + (oy+1)*3 + (oz+1)):
       // first slot, count, tile position and image shift, all loads in
       // flight together; a cell beyond an open face has count 0
-      int jb[27], jn[27], tb[27], sxyz[27];
+      int jb[27], jn[27], tb[27];
 #pragma unroll
       for (int k = 0; k < 27; k++) {
         const int ox = k / 9 - 1, oy = (k / 3) % 3 - 1, oz = k % 3 - 1;
@@ -645,7 +652,6 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
         int cb = a.cell_beg[nc], cn = a.cell_num[nc];
         jb[k] = cb;
         jn[k] = valid ? cn : 0;
-        sxyz[k] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
       }
       if (tok) {
 #pragma unroll
@@ -655,13 +661,13 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
         for (int k = 0; k < 27; k++)
           list_cell<MIXED, true>(a, R, pos, trec, L, jb[k], jn[k], tb[k],
                                  off[3 * k], off[3 * k + 1], off[3 * k + 2],
-                                 k == 13, k >= 13, (uint32_t)k, sxyz[k]);
+                                 k == 13, k >= 13, (uint32_t)k);
       } else {
 #pragma unroll
         for (int k = 0; k < 27; k++)
           list_cell<MIXED, false>(a, R, pos, trec, L, jb[k], jn[k], 0,
                                   off[3 * k], off[3 * k + 1], off[3 * k + 2],
-                                  k == 13, k >= 13, (uint32_t)k, sxyz[k]);
+                                  k == 13, k >= 13, (uint32_t)k);
       }
       // pad to a multiple of IMDG_LIST_PAD entries with far-away copies of
       // the atom itself (kmax is a multiple of IMDG_LIST_PAD, so the padded
@@ -669,18 +675,11 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
       int cnt = L.cnt;
       if (cnt <= kmax) {
         uint32_t pad = (uint32_t)i | ((uint32_t)IMDG_CODE_PAD << IMDG_IDX_BITS);
-        uint4 q = L.q;
-        int s4 = cnt & 3;
-        if (s4 != 0) {
-          if (s4 <= 1) q.y = pad;
-          if (s4 <= 2) q.z = pad;
-          q.w = pad;
-          L.lst[(size_t)(cnt >> 2) * IMDG_WAVE] = q;
+        int mend = ((cnt + IMDG_LIST_PAD - 1) / IMDG_LIST_PAD) * IMDG_LIST_PAD;
+        for (int m = cnt; m < mend; m++) {
+          L.cnt = m;
+          list_append(L, pad);
         }
-        int g = (cnt + 3) >> 2, gend = ((cnt + IMDG_LIST_PAD - 1) /
-                                        IMDG_LIST_PAD) * (IMDG_LIST_PAD / 4);
-        for (; g < gend; g++)
-          L.lst[(size_t)g * IMDG_WAVE] = make_uint4(pad, pad, pad, pad);
       }
       a.nnb[i] = cnt < kmax ? cnt : kmax;
       seen = cnt;
