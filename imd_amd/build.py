@@ -24,7 +24,15 @@ LIB = os.path.join(PKG, "libimdg.so")
 HOSTLIB = os.path.join(HOST, "libimdhost.so")
 HOSTEXE = os.path.join(HOST, "imd_gpu")
 
-HIP_SOURCES = ["imdg_build.hip", "imdg_step.hip", "imdg_api.hip"]
+# (source, extra flags, object): the step kernels are compiled once per
+# precision so that the two halves build in parallel
+HIP_UNITS = [
+    ("imdg_build.hip", [], "imdg_build.o"),
+    ("imdg_step.hip", ["-DIMDG_STEP_FP32"], "imdg_step_f32.o"),
+    ("imdg_step.hip", ["-DIMDG_STEP_FP64"], "imdg_step_f64.o"),
+    ("imdg_api.hip", [], "imdg_api.o"),
+]
+HIP_SOURCES = sorted({u[0] for u in HIP_UNITS})
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -53,10 +61,10 @@ def _run(cmd, cwd=None):
 
 def build_lib(verbose: bool = False) -> str:
     from concurrent.futures import ThreadPoolExecutor
-    objs = [os.path.join(CSRC, src.replace(".hip", ".o")) for src in HIP_SOURCES]
-    cmds = [[hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-c",
+    objs = [os.path.join(CSRC, o) for _, _, o in HIP_UNITS]
+    cmds = [[hipcc(), *HIPCC_FLAGS, *flags, "-I", INCLUDE, "-c",
              os.path.join(CSRC, src), "-o", obj]
-            for src, obj in zip(HIP_SOURCES, objs)]
+            for (src, flags, _), obj in zip(HIP_UNITS, objs)]
     with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
         for out in ex.map(_run, cmds):
             if verbose and out.strip():

@@ -17,10 +17,13 @@
 
 namespace imdg {
 template <typename R>
-int step_grid_density(bool nt1, bool lt, size_t lds, int ncu);
+int step_grid_density(bool nt1, bool lt, size_t lds, int ncu, int lpa,
+                      int threads);
 template <typename R>
 int step_grid_force(int mode, bool integrate, bool lt, bool hp, bool he,
-                    size_t lds, int ncu);
+                    size_t lds, int ncu, int lpa, int threads);
+template <typename R>
+int force_threads(int mode);
 template <typename R>
 int step_grid_move(int ncu);
 template <typename R>
@@ -113,6 +116,9 @@ struct imdg_ctx {
   int knots_embed = 0;
   int knots_joint = -1;                // interleaved (pair, rho), one species
   int force_mode = 0;                  // 0 one species, 1 shared grid, 2 several
+  int lpa_den = 1, lpa_frc = 1;        // lanes per slot of the two loops
+  int thr_den = 0, thr_frc = 0;        // threads per block (0: default)
+  int units_cap = 0;                   // wavefront partial slots allocated
   int lds_den_lo = 0, lds_den_n = 0;   // knots staged by the density loop
   int lds_frc_lo = 0, lds_frc_n = 0;   // knots staged by the force loop
   int grid_den = 0, grid_frc[2] = {0, 0}, grid_move = 0;
@@ -150,6 +156,9 @@ int fail(imdg_ctx* c, int code, const char* fmt, ...) {
       return fail(ctx, IMDG_ERR_HIP, "%s: %s (%s:%d)", #call,                \
                   hipGetErrorString(e_), __FILE__, __LINE__);                \
   } while (0)
+
+// Lanes per slot of the neighbour loops: 1, 2 or 4.
+int lpa_value(int v) { return v >= 4 ? 4 : (v >= 2 ? 2 : 1); }
 
 // Drop the captured step (its kernel arguments hold buffer pointers).
 void drop_graph(imdg_ctx* c) {
@@ -300,6 +309,9 @@ int plan_steps(imdg_ctx* c) {
   bool hp = c->tab[0].present, he = c->tab[1].present;
   c->force_mode = !nt1 ? 2 : (c->knots_joint >= 0 ? 1 : 0);
   int mode = c->force_mode;
+  if (c->thr_den <= 0) c->thr_den = IMDG_DENSITY_THREADS;
+  if (c->thr_frc <= 0) c->thr_frc = force_threads<R>(mode);
+  int td = c->thr_den, tf = c->thr_frc;
   int no_lds = 0;
   if (const char* e = getenv("IMDG_TABLES_IN_LDS")) no_lds = (atoi(e) == 0);
   // density: the rho knots
@@ -308,10 +320,11 @@ int plan_steps(imdg_ctx* c) {
   c->grid_den = 0;
   if (he && !no_lds)
     c->grid_den = step_grid_density<R>(nt1, true, c->lds_den_n * sizeof(R2),
-                                       c->ncu);
+                                       c->ncu, c->lpa_den, td);
   if (he && c->grid_den == 0) {
     c->lds_den_n = 0;
-    c->grid_den = step_grid_density<R>(nt1, false, 0, c->ncu);
+    c->grid_den =
+        step_grid_density<R>(nt1, false, 0, c->ncu, c->lpa_den, td);
   }
   // force: pair and rho knots (or the interleaved table)
   if (mode == 1) {
@@ -325,13 +338,15 @@ int plan_steps(imdg_ctx* c) {
     int g = 0;
     if (!no_lds)
       g = step_grid_force<R>(mode, integ, true, hp, he,
-                             c->lds_frc_n * sizeof(R2), c->ncu);
+                             c->lds_frc_n * sizeof(R2), c->ncu, c->lpa_frc,
+                             tf);
     if (g == 0) {
       c->lds_frc_n = 0;
-      g = step_grid_force<R>(mode, integ, false, hp, he, 0, c->ncu);
+      g = step_grid_force<R>(mode, integ, false, hp, he, 0, c->ncu,
+                             c->lpa_frc, tf);
       if (integ == 1)  // keep both variants on the same table placement
-        c->grid_frc[0] =
-            step_grid_force<R>(mode, 0, false, hp, he, 0, c->ncu);
+        c->grid_frc[0] = step_grid_force<R>(mode, 0, false, hp, he, 0,
+                                            c->ncu, c->lpa_frc, tf);
     }
     c->grid_frc[integ] = g;
   }
@@ -339,9 +354,14 @@ int plan_steps(imdg_ctx* c) {
   if (c->grid_frc[0] == 0 || c->grid_frc[1] == 0 || c->grid_move == 0 ||
       (he && c->grid_den == 0))
     return fail(c, IMDG_ERR_HIP, "cannot size the persistent step kernels");
-  int gmax = std::max(std::max(c->grid_den, c->grid_move),
-                      std::max(c->grid_frc[0], c->grid_frc[1]));
-  c->units_max = gmax * IMDG_STEP_WAVES;
+  int units = std::max(std::max(c->grid_den * (td / IMDG_WAVE),
+                                c->grid_move * IMDG_STEP_WAVES),
+                       std::max(c->grid_frc[0], c->grid_frc[1]) *
+                           (tf / IMDG_WAVE));
+  c->units_max = units;
+  if (c->units_cap > 0 && units > c->units_cap)
+    return fail(c, IMDG_ERR_HIP, "step kernels need %d partial slots, %d "
+                "allocated", units, c->units_cap);
   return IMDG_OK;
 }
 
@@ -367,6 +387,8 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.nnb = c->d_nnb;
   a.knots = (const typename V2<R>::T*)c->d_knots;
   a.joint = c->knots_joint;
+  a.lpa_den = c->lpa_den;
+  a.lpa_frc = c->lpa_frc;
   a.rho = (R*)c->d_rho;
   a.epot = (R*)c->d_epot;
   a.frc = (R4*)c->d_frc;
@@ -376,7 +398,9 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.nrep = c->nrep;
   a.nchunk = c->nchunk;
   a.part_chunk = c->part_chunk;
-  a.unitsA = c->grid_den * (IMDG_DENSITY_THREADS / IMDG_WAVE);
+  a.unitsA = c->grid_den * (c->thr_den / IMDG_WAVE);
+  a.thr_den = c->thr_den;
+  a.thr_frc = c->thr_frc;
   a.kmax = c->kmax;
   a.ntypes = c->cfg.ntypes;
   a.ensemble = c->cfg.ensemble;
@@ -537,8 +561,8 @@ int fix_overflow(imdg_ctx* c, int* fixed) {
   int rc = read_ctl(c, &h);
   if (rc) return rc;
   if (!h.frozen) return IMDG_OK;
-  int newk = std::max(c->kmax + 8, (int)ceil(h.kmax_seen * 1.2));
-  newk = (newk + 7) & ~7;
+  int newk = std::max(c->kmax + IMDG_LIST_PAD, (int)ceil(h.kmax_seen * 1.2));
+  newk = (newk + IMDG_LIST_PAD - 1) / IMDG_LIST_PAD * IMDG_LIST_PAD;
   drop_graph(c);
   dfree(c, c->d_nbr);
   c->d_nbr = nullptr;
@@ -602,6 +626,111 @@ int read_slots(imdg_ctx* c, SlotView& v, bool state) {
   return IMDG_OK;
 }
 
+// Mean time of `reps` launches of one neighbour loop (the density loop, or
+// the force loop without integration) on the current state, after one
+// untimed launch.  Both only recompute per-step outputs from the positions
+// and the list; the force loop's per-atom energies build on the density
+// loop's, so the caller finishes with one regular force evaluation.
+template <typename R>
+int time_loop(imdg_ctx* c, bool density, int reps, double* ms) {
+  StepArgs<R> a = step_args<R>(c);
+  const TabDev<R>* t = (const TabDev<R>*)c->d_tabs;
+  bool nt1 = c->cfg.ntypes == 1;
+  auto launch = [&]() -> hipError_t {
+    if (density) {
+      a.lds_lo = c->lds_den_lo;
+      a.lds_n = c->lds_den_n;
+      return launch_density<R>(a, t, nt1, c->grid_den, c->stream);
+    }
+    a.lds_lo = c->lds_frc_lo;
+    a.lds_n = c->lds_frc_n;
+    return launch_force<R>(a, t, c->force_mode, false, c->grid_frc[0],
+                           c->stream);
+  };
+  hipEvent_t e0 = get_event(c), e1 = get_event(c);
+  hipError_t e = launch();
+  if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+  for (int k = 0; k < reps && e == hipSuccess; k++) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t_ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t_ms, e0, e1);
+  c->event_pool.push_back(e0);
+  c->event_pool.push_back(e1);
+  HIPCHK(c, e);
+  *ms = t_ms / std::max(1, reps);
+  return IMDG_OK;
+}
+
+int plan(imdg_ctx* c) {
+  return c->mixed ? plan_steps<float>(c) : plan_steps<double>(c);
+}
+
+// Try every lanes-per-slot x block-size variant of the density loop, then
+// of the force loop, on the current state; keep the fastest of each.
+template <typename R>
+int autotune_t(imdg_ctx* c, int reps, imdg_tuning* out) {
+  static const int lanes[3] = {1, 2, 4};
+  const int fmax = force_threads<R>(c->force_mode);
+  const int thr_d[3] = {256, 512, 1024};
+  const int thr_f[3] = {256, 512, fmax};
+  int rc;
+  double best_d = 0.0, best_f = 0.0;
+  if (c->tab[1].present) {
+    int bl = c->lpa_den, bt = c->thr_den;
+    best_d = 1e300;
+    for (int li = 0; li < 3; li++)
+      for (int ti = 0; ti < 3; ti++) {
+        c->lpa_den = lanes[li];
+        c->thr_den = thr_d[ti];
+        if (plan(c) != IMDG_OK) continue;  // not launchable this way
+        double ms;
+        if ((rc = time_loop<R>(c, true, reps, &ms))) return rc;
+        if (ms < best_d) {
+          best_d = ms;
+          bl = lanes[li];
+          bt = thr_d[ti];
+        }
+      }
+    c->lpa_den = bl;
+    c->thr_den = bt;
+    if ((rc = plan(c))) return rc;
+    double ms;
+    if ((rc = time_loop<R>(c, true, 0, &ms))) return rc;  // records for B
+  }
+  {
+    int bl = c->lpa_frc, bt = c->thr_frc;
+    best_f = 1e300;
+    for (int li = 0; li < 3; li++)
+      for (int ti = 0; ti < 3; ti++) {
+        if (ti == 2 && thr_f[2] <= thr_f[1]) continue;
+        c->lpa_frc = lanes[li];
+        c->thr_frc = thr_f[ti];
+        if (plan(c) != IMDG_OK) continue;
+        double ms;
+        if ((rc = time_loop<R>(c, false, reps, &ms))) return rc;
+        if (ms < best_f) {
+          best_f = ms;
+          bl = lanes[li];
+          bt = thr_f[ti];
+        }
+      }
+    c->lpa_frc = bl;
+    c->thr_frc = bt;
+  }
+  if ((rc = plan(c))) return rc;
+  c->err.clear();
+  if (out) {
+    out->density_lanes = c->lpa_den;
+    out->density_threads = c->thr_den;
+    out->force_lanes = c->lpa_frc;
+    out->force_threads = c->thr_frc;
+    out->density_ms = best_d;
+    out->force_ms = best_f;
+  }
+  return IMDG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -615,7 +744,8 @@ long imdg_struct_size(const char* name) {
              {"imdg_table", (long)sizeof(imdg_table)},
              {"imdg_replica", (long)sizeof(imdg_replica)},
              {"imdg_thermo", (long)sizeof(imdg_thermo)},
-             {"imdg_info", (long)sizeof(imdg_info)}};
+             {"imdg_info", (long)sizeof(imdg_info)},
+             {"imdg_tuning", (long)sizeof(imdg_tuning)}};
   for (const auto& e : tab)
     if (!strcmp(e.n, name)) return e.size;
   return -1;
@@ -642,6 +772,10 @@ int imdg_ctx_create(const imdg_config* cfg, imdg_ctx** out) {
   c->xcd_remap = remap ? atoi(remap) != 0 : 1;
   const char* graph = getenv("IMDG_GRAPH");
   c->use_graph = graph ? atoi(graph) != 0 : false;
+  if (cfg->lanes_per_atom > 0)
+    c->lpa_den = c->lpa_frc = lpa_value(cfg->lanes_per_atom);
+  if (const char* e = getenv("IMDG_LPA_DENSITY")) c->lpa_den = lpa_value(atoi(e));
+  if (const char* e = getenv("IMDG_LPA_FORCE")) c->lpa_frc = lpa_value(atoi(e));
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 ||
       cfg->device < 0 || cfg->device >= ndev) {
@@ -787,7 +921,7 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
   c->ncells = cell;
   c->nstile = stile;
   c->kmax = std::max(16, (int)ceil(kest * c->cfg.nbl_size * 1.25));
-  c->kmax = (c->kmax + 7) & ~7;
+  c->kmax = (c->kmax + IMDG_LIST_PAD - 1) / IMDG_LIST_PAD * IMDG_LIST_PAD;
 
   int rc;
 #define ALLOC(ptr, bytes)                                        \
@@ -830,10 +964,13 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
     return rc;
   c->nchunk = c->nslots / IMDG_CHUNK;
   c->part_chunk = nrep > 1 ? 1 : 0;
+  c->units_cap = 0;  // partials are allocated below, for any plan
   if ((rc = c->mixed ? plan_steps<float>(c) : plan_steps<double>(c)))
     return rc;
   {
-    size_t units = std::max(c->nchunk, c->units_max);
+    // room for any launch plan: at most 32 resident wavefronts per CU
+    c->units_cap = std::max(c->units_max, 32 * c->ncu);
+    size_t units = std::max(c->nchunk, c->units_cap);
     ALLOC(c->d_partA, sizeof(double) * units);
     ALLOC(c->d_partB, sizeof(double) * PB_N * units);
   }
@@ -1311,7 +1448,7 @@ int imdg_get_info(imdg_ctx* c, imdg_info* info) {
   info->natoms = c->natoms_tot;
   info->nslots = c->nslots;
   info->kmax = c->kmax;
-  info->lanes_per_atom = 1;
+  info->lanes_per_atom = c->lpa_frc;
   info->list_bytes = (double)c->nbr_bytes;
   if (!c->atoms_set) return IMDG_OK;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1344,6 +1481,66 @@ int imdg_profile(imdg_ctx* c, int enable) {
     c->kc_ms[k] = 0;
     c->kc_n[k] = 0;
   }
+  return IMDG_OK;
+}
+
+int imdg_get_tuning(imdg_ctx* c, imdg_tuning* out) {
+  if (!c || !out) return IMDG_ERR_ARG;
+  memset(out, 0, sizeof *out);
+  out->density_lanes = c->lpa_den;
+  out->density_threads = c->thr_den > 0 ? c->thr_den : IMDG_DENSITY_THREADS;
+  out->force_lanes = c->lpa_frc;
+  out->force_threads = c->thr_frc;
+  return IMDG_OK;
+}
+
+int imdg_set_tuning(imdg_ctx* c, const imdg_tuning* t) {
+  if (!c || !t) return IMDG_ERR_ARG;
+  auto lanes_ok = [](int v) { return v == 1 || v == 2 || v == 4; };
+  auto thr_ok = [](int v) {
+    return v >= IMDG_WAVE && v <= IMDG_STEP_THREADS && v % IMDG_WAVE == 0;
+  };
+  if (!lanes_ok(t->density_lanes) || !lanes_ok(t->force_lanes) ||
+      !thr_ok(t->density_threads) || !thr_ok(t->force_threads))
+    return fail(c, IMDG_ERR_ARG, "invalid tuning (lanes 1/2/4, threads a "
+                "multiple of 64 up to %d)", IMDG_STEP_THREADS);
+  int old[4] = {c->lpa_den, c->thr_den, c->lpa_frc, c->thr_frc};
+  c->lpa_den = t->density_lanes;
+  c->thr_den = t->density_threads;
+  c->lpa_frc = t->force_lanes;
+  c->thr_frc = t->force_threads;
+  if (!c->atoms_set) return IMDG_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (plan(c) != IMDG_OK) {
+    std::string why = c->err;
+    c->lpa_den = old[0];
+    c->thr_den = old[1];
+    c->lpa_frc = old[2];
+    c->thr_frc = old[3];
+    int rc = plan(c);
+    if (rc) return rc;
+    return fail(c, IMDG_ERR_ARG, "tuning not launchable: %s", why.c_str());
+  }
+  drop_graph(c);
+  return IMDG_OK;
+}
+
+int imdg_autotune(imdg_ctx* c, int reps, imdg_tuning* chosen) {
+  if (!c || reps < 1) return IMDG_ERR_ARG;
+  int rc = ensure_ready(c);
+  if (rc) return rc;
+  if ((rc = build_until_fits(c))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  bool prof = c->profiling;
+  c->profiling = false;
+  rc = c->mixed ? autotune_t<float>(c, reps, chosen)
+                : autotune_t<double>(c, reps, chosen);
+  c->profiling = prof;
+  if (rc) return rc;
+  drop_graph(c);
+  // leave forces and energies as calc_forces would
+  if ((rc = enqueue_forces(c, false))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return IMDG_OK;
 }
 

@@ -40,7 +40,7 @@
 #define IMDG_CHUNK 64        // slots per wavefront step of the step kernels
 #define IMDG_STEP_THREADS 1024   // largest block of the step kernels
 #define IMDG_STEP_WAVES (IMDG_STEP_THREADS / IMDG_WAVE)
-#define IMDG_DENSITY_THREADS 512 // density loop: several blocks per CU
+#define IMDG_DENSITY_THREADS 512 // density loop default: several blocks per CU
 #define IMDG_MAXTYPES 8
 #define IMDG_MAXCOLS (IMDG_MAXTYPES * IMDG_MAXTYPES)
 #define IMDG_RING 4096
@@ -51,7 +51,7 @@
 #define IMDG_NCODES 28
 #define IMDG_PAD_SHIFT 1.0e6
 #define IMDG_GROUP 4          // list entries per lane and load
-#define IMDG_LIST_PAD 8       // lists padded to whole pairs of groups
+#define IMDG_LIST_PAD 16      // lists padded to whole bodies of 4 lanes
 #define IMDG_TILE_CAP 2048    // records of one block's LDS tile
 #define IMDG_MAXRUN 40        // slot runs of one tile
 
@@ -203,6 +203,8 @@ struct StepArgs {
   int lds_lo, lds_n;      // knots staged in LDS by this launch (n = 0: none)
   int joint;              // first knot of the interleaved single-species
                           // (pair, rho) table, -1 if there is none
+  int lpa_den, lpa_frc;   // lanes per slot of the density / force loops
+  int thr_den, thr_frc;   // threads per block of the density / force loops
 };
 
 struct BuildArgs {

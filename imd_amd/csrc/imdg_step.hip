@@ -73,12 +73,18 @@ __device__ __forceinline__ Sched make_sched(int nchunk, int xcd_remap) {
 }
 
 // Cell-offset vectors of a replica, in the force precision, into the
-// wavefront's own LDS row.
-template <typename R>
-__device__ __forceinline__ void wave_offsets(const RepDev& Rp, R* off) {
+// wavefront's own LDS row (one 4-vector per code: one LDS load per pair).
+template <typename R4>
+__device__ __forceinline__ void wave_offsets(const RepDev& Rp, R4* off) {
   int l = threadIdx.x & (IMDG_WAVE - 1);
-  for (int k = l; k < 3 * IMDG_NCODES; k += IMDG_WAVE)
-    off[k] = (R)((&Rp.offs[0][0])[k]);
+  if (l < IMDG_NCODES) {
+    R4 o;
+    o.x = Rp.offs[l][0];
+    o.y = Rp.offs[l][1];
+    o.z = Rp.offs[l][2];
+    o.w = 0;
+    off[l] = o;
+  }
   wave_lds_sync();
 }
 
@@ -407,28 +413,87 @@ __device__ __forceinline__ void integrate_slot(StepArgs<R>& a,
 // neighbour-list walk
 // ---------------------------------------------------------------------------
 
-// Walk a slot's list in pairs of groups (lists are padded to a multiple of
-// 8 entries, so every list has an even number of groups).  Groups A = 2h
-// and B = 2h+1 sit in two registers of their own; each group's body issues
-// its record gathers, then (through pf/slot) the load of the group two
+// Lanes per atom of the neighbour loops.  With L > 1 a wavefront walks its
+// 64-slot chunk in L passes of 64/L slots; the L lanes of a slot take
+// interleaved entries of its list, so the records one gather instruction
+// reads for a slot are L consecutive list entries -- mostly neighbouring
+// slots of one cell, one or two cache lines instead of L.  Pass p ends with
+// a sum over the slot's L lanes, and lane p of the slot keeps it: after the
+// L passes lane l holds the sums of chunk slot epi_slot(l), the slot whose
+// per-atom work (embedding, integration, writes) it then does.
+__device__ __forceinline__ int epi_slot(int lane, int L) {
+  return (IMDG_WAVE / L) * (lane % L) + lane / L;
+}
+
+// Sum over the L lanes of a slot (aligned lane groups); every lane of the
+// group gets the same bits (each butterfly step adds the same two values).
+template <int L, typename R>
+__device__ __forceinline__ R lane_group_sum(R v) {
+  if (L >= 2) v += __shfl_xor(v, 1, IMDG_WAVE);
+  if (L >= 4) v += __shfl_xor(v, 2, IMDG_WAVE);
+  return v;
+}
+template <int L, typename R>
+__device__ __forceinline__ R lane_group_min(R v) {
+  if (L >= 2) v = rmin(v, __shfl_xor(v, 1, IMDG_WAVE));
+  if (L >= 4) v = rmin(v, __shfl_xor(v, 2, IMDG_WAVE));
+  return v;
+}
+
+// Four list entries of one lane: with L lanes per slot, one body covers L
+// groups of the grouped layout (4L entries of the slot); lane s of the slot
+// reads words s*(4/L) .. s*(4/L) + 4/L - 1 of each of them.
+template <int L>
+struct ListRef {
+  const uint32_t* p;  // the lane's first word of the body
+  __device__ __forceinline__ uint4 load() const {
+    constexpr int G = IMDG_WAVE * IMDG_GROUP;  // words per group
+    if (L == 1) return *reinterpret_cast<const uint4*>(p);
+    if (L == 2) {
+      uint2 x = *reinterpret_cast<const uint2*>(p);
+      uint2 y = *reinterpret_cast<const uint2*>(p + G);
+      return make_uint4(x.x, x.y, y.x, y.y);
+    }
+    return make_uint4(p[0], p[G], p[2 * G], p[3 * G]);
+  }
+};
+
+// First word of lane s's share of slot i's list.
+template <int L>
+__device__ __forceinline__ const uint32_t* list_words(const uint32_t* nbr,
+                                                      int i, int kmax, int s) {
+  return reinterpret_cast<const uint32_t*>(list_of(nbr, i, kmax)) +
+         s * (IMDG_GROUP / L);
+}
+
+// Walk a slot's list, one body (four entries per lane) at a time, in pairs
+// A = 2h, B = 2h+1.  A and B sit in two registers of their own; each body
+// issues its record gathers, then (through pf/slot) the load of the body two
 // ahead into the register it has just consumed, then computes.  No register
 // changes role between iterations, so nothing waits for the prefetch
-// before the next group's gathers need it.  The initial values are
-// selects rather than plain loads: a phi of loads would be folded into one
-// load next to its use, which undoes the prefetch.
-template <typename F>
-__device__ __forceinline__ void list_walk(const uint4* nb, int nnb, F body) {
-  const int npair = (nnb + 7) >> 3;
-  const int glast = npair > 0 ? 2 * npair - 1 : 0;
-  uint4 A = nb[0];
-  uint4 B = nb[(size_t)min(1, glast) * IMDG_WAVE];
-  A = make_uint4(npair > 0 ? A.x : 0u, npair > 0 ? A.y : 0u,
-                 npair > 0 ? A.z : 0u, npair > 0 ? A.w : 0u);
-  B = make_uint4(npair > 0 ? B.x : 0u, npair > 0 ? B.y : 0u,
-                 npair > 0 ? B.z : 0u, npair > 0 ? B.w : 0u);
-  for (int h = 0; h < npair; h++) {
-    body(A, nb + (size_t)min(2 * h + 2, glast) * IMDG_WAVE, A);
-    body(B, nb + (size_t)min(2 * h + 3, glast) * IMDG_WAVE, B);
+// before the next body's gathers need it.  The initial values are selects
+// rather than plain loads: a phi of loads would be folded into one load
+// next to its use, which undoes the prefetch.  Lists are padded with
+// far-away entries to a multiple of 16, so with 1 or 2 lanes per slot the
+// body count is rounded up to even inside the padding and B always runs;
+// with 4 lanes a body is 16 entries and B past the last body is skipped.
+template <int L, typename F>
+__device__ __forceinline__ void list_walk(const uint32_t* w, int nnb, F body) {
+  constexpr int E = IMDG_GROUP * L;                    // entries per body
+  constexpr size_t BW = (size_t)L * IMDG_WAVE * IMDG_GROUP;  // words per body
+  int nbody = (nnb + E - 1) / E;
+  if (L <= 2) nbody = (nbody + 1) & ~1;
+  const int blast = nbody > 0 ? nbody - 1 : 0;
+  uint4 A = ListRef<L>{w}.load();
+  uint4 B = ListRef<L>{w + (size_t)min(1, blast) * BW}.load();
+  A = make_uint4(nbody > 0 ? A.x : 0u, nbody > 0 ? A.y : 0u,
+                 nbody > 0 ? A.z : 0u, nbody > 0 ? A.w : 0u);
+  B = make_uint4(nbody > 0 ? B.x : 0u, nbody > 0 ? B.y : 0u,
+                 nbody > 0 ? B.z : 0u, nbody > 0 ? B.w : 0u);
+  for (int h = 0; 2 * h < nbody; h++) {
+    body(A, ListRef<L>{w + (size_t)min(2 * h + 2, blast) * BW}, A);
+    if (L <= 2 || 2 * h + 1 < nbody)
+      body(B, ListRef<L>{w + (size_t)min(2 * h + 3, blast) * BW}, B);
   }
 }
 
@@ -451,15 +516,14 @@ struct ForceCols {
 // Four neighbours of the density loop: records and offsets, the prefetch,
 // r^2 and segments, all knot loads together, accumulation with mask
 // multiplies (no branches around the loads).
-template <typename R, bool NT1, typename R4, typename R2>
+template <typename R, bool NT1, typename R4, typename R2, typename PF>
 __device__ __forceinline__ void density_group(
-    uint4 e4, const uint4* pf, uint4& slot, const R4* __restrict__ rec,
-    const R* off, const R4& ui, int it, int nt, const R2* __restrict__ K,
+    uint4 e4, PF pf, uint4& slot, const R4* __restrict__ rec,
+    const R4* off, const R4& ui, int it, int nt, const R2* __restrict__ K,
     const ColPar<R>* cpr, const DensityCol<R>& c0, R& rho, int& is_short,
     R& r2min) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
-  R4 uj[4];
-  R o[4][3];
+  R4 uj[4], o[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     if (NT1) {
@@ -475,22 +539,19 @@ __device__ __forceinline__ void density_group(
     } else {
       uj[u] = rec[en[u] & IMDG_IDX_MASK];
     }
-    const R* op = off + 3 * (en[u] >> IMDG_IDX_BITS);
-    o[u][0] = op[0];
-    o[u][1] = op[1];
-    o[u][2] = op[2];
+    o[u] = off[en[u] >> IMDG_IDX_BITS];
   }
   // issued after the gathers: loads complete in order, and the gathers
   // must not wait for this (HBM) one
   __builtin_amdgcn_sched_barrier(0);
-  slot = *pf;
+  slot = pf.load();
   R r2[4], chi[4], cb[4], ce[4];
   int kk[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    R dx = (uj[u].x - ui.x) + o[u][0];
-    R dy = (uj[u].y - ui.y) + o[u][1];
-    R dz = (uj[u].z - ui.z) + o[u][2];
+    R dx = (uj[u].x - ui.x) + o[u].x;
+    R dy = (uj[u].y - ui.y) + o[u].y;
+    R dz = (uj[u].z - ui.z) + o[u].z;
     r2[u] = dx * dx + dy * dy + dz * dz;
     if (NT1) {
       kk[u] = c0.kb + seg_locate(r2[u], c0.begin, c0.end, c0.invstep, c0.len,
@@ -523,34 +584,31 @@ __device__ __forceinline__ void density_group(
 }
 
 // Four neighbours of the force loop (same phases as density_group).
-template <typename R, bool NT1, bool HP, bool HE, typename R4, typename R2>
+template <typename R, bool NT1, bool HP, bool HE, typename R4, typename R2,
+          typename PF>
 __device__ __forceinline__ void force_group(
-    uint4 e4, const uint4* pf, uint4& slot, const R4* __restrict__ rec,
-    const signed char* __restrict__ atype, const R* off, const R4& ui, R dFi,
+    uint4 e4, PF pf, uint4& slot, const R4* __restrict__ rec,
+    const signed char* __restrict__ atype, const R4* off, const R4& ui, R dFi,
     int it, int nt, const R2* __restrict__ K, const ColPar<R>* cpp,
     const ColPar<R>* cpr, const ForceCols<R>& c, R& fx, R& fy, R& fz, R& ep,
     R& vir, int& is_short) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
-  R4 uj[4];
+  R4 uj[4], o[4];
   int jt[4];
-  R o[4][3];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     uj[u] = rec[en[u] & IMDG_IDX_MASK];
     jt[u] = NT1 ? 0 : (int)atype[en[u] & IMDG_IDX_MASK];
-    const R* op = off + 3 * (en[u] >> IMDG_IDX_BITS);
-    o[u][0] = op[0];
-    o[u][1] = op[1];
-    o[u][2] = op[2];
+    o[u] = off[en[u] >> IMDG_IDX_BITS];
   }
   __builtin_amdgcn_sched_barrier(0);
-  slot = *pf;
+  slot = pf.load();
   R dx[4], dy[4], dz[4], r2[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    dx[u] = (uj[u].x - ui.x) + o[u][0];
-    dy[u] = (uj[u].y - ui.y) + o[u][1];
-    dz[u] = (uj[u].z - ui.z) + o[u][2];
+    dx[u] = (uj[u].x - ui.x) + o[u].x;
+    dy[u] = (uj[u].y - ui.y) + o[u].y;
+    dz[u] = (uj[u].z - ui.z) + o[u].z;
     r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
   }
   R gg[4];
@@ -646,32 +704,28 @@ __device__ __forceinline__ void force_group(
 // (phi p0, phi dv, rho p0, rho dv) read as two 16 B LDS loads.  The common
 // factor 2 istep of both gradients is applied once per atom by the caller
 // (sums s* are in units of 1 / (2 istep)); hFi = F'(rho_i) = 2F'/2.
-template <typename R, typename R4>
+template <typename R, typename R4, typename PF>
 __device__ __forceinline__ void force_group_sg(
-    uint4 e4, const uint4* pf, uint4& slot, const R4* __restrict__ rec,
-    const R* off, const R4& ui, R hFi, const R4* __restrict__ J, int jb,
+    uint4 e4, PF pf, uint4& slot, const R4* __restrict__ rec,
+    const R4* off, const R4& ui, R hFi, const R4* __restrict__ J, int jb,
     int len, R begin, R end, R invstep, R& sx, R& sy, R& sz, R& ep, R& svir,
     R& r2min) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
-  R4 uj[4];
-  R o[4][3];
+  R4 uj[4], o[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     uj[u] = rec[en[u] & IMDG_IDX_MASK];
-    const R* op = off + 3 * (en[u] >> IMDG_IDX_BITS);
-    o[u][0] = op[0];
-    o[u][1] = op[1];
-    o[u][2] = op[2];
+    o[u] = off[en[u] >> IMDG_IDX_BITS];
   }
   __builtin_amdgcn_sched_barrier(0);
-  slot = *pf;
+  slot = pf.load();
   R dx[4], dy[4], dz[4], r2[4], chi[4];
   int kk[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    dx[u] = (uj[u].x - ui.x) + o[u][0];
-    dy[u] = (uj[u].y - ui.y) + o[u][1];
-    dz[u] = (uj[u].z - ui.z) + o[u][2];
+    dx[u] = (uj[u].x - ui.x) + o[u].x;
+    dy[u] = (uj[u].y - ui.y) + o[u].y;
+    dz[u] = (uj[u].z - ui.z) + o[u].z;
     r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
     kk[u] = jb + seg_locate(r2[u], begin, end, invstep, len, chi[u]);
   }
@@ -709,14 +763,14 @@ __device__ __forceinline__ void force_group_sg(
 // Host density, embedding energy and 2F'(rho) (src/imd_forces_nbl.c:586-611,
 // 1079-1112): rho_i = sum_j rho_{type(i) type(j)}(r_ij^2) over the full
 // list, then F and 2F' from the embedding column of type(i).  Writes the
-// (u, 2F') records the force loop gathers.
-template <typename R, bool NT1, bool LT>
-__global__ void __launch_bounds__(IMDG_DENSITY_THREADS)
+// (u, 2F') records the force loop gathers.  L lanes per slot.
+template <typename R, bool NT1, bool LT, int L>
+__global__ void __launch_bounds__(IMDG_STEP_THREADS)
     k_density(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
   typedef typename V4<R>::T R4;
   typedef typename V2<R>::T R2;
   extern __shared__ __align__(16) unsigned char dyn_lds[];
-  __shared__ R woff[IMDG_STEP_WAVES][3 * IMDG_NCODES];
+  __shared__ R4 woff[IMDG_STEP_WAVES][IMDG_NCODES];
   __shared__ ColPar<R> cpr[IMDG_MAXCOLS];
   if (a.ctl->frozen) return;
   const TabDev<R>& tr = tabs[1];
@@ -734,7 +788,8 @@ __global__ void __launch_bounds__(IMDG_DENSITY_THREADS)
 
   Sched S = make_sched(a.nchunk, a.xcd_remap);
   const int lane = threadIdx.x & (IMDG_WAVE - 1);
-  R* off = woff[threadIdx.x / IMDG_WAVE];
+  const int ga = lane / L, gs = lane % L;
+  R4* off = woff[threadIdx.x / IMDG_WAVE];
   int cur = -1, is_short = 0;
   double acc = 0.0;
   for (int c = S.c; c < S.c1; c += S.stride) {
@@ -744,20 +799,33 @@ __global__ void __launch_bounds__(IMDG_DENSITY_THREADS)
       wave_offsets(Rp, off);
       cur = r;
     }
-    int i = c * IMDG_CHUNK + lane;
+    R rho_keep = (R)0;
+#pragma unroll 1
+    for (int p = 0; p < L; p++) {
+      int i = c * IMDG_CHUNK + p * (IMDG_WAVE / L) + ga;
+      R rho = (R)0;
+      if (i - Rp.atom_off < Rp.natoms) {
+        R4 ui = rec[i];
+        int it = NT1 ? 0 : (int)ui.w;
+        DensityCol<R> col0 = {kb0, len0, b0, e0, is0};
+        R r2min = (R)3.0e38;
+        list_walk<L>(list_words<L>(a.nbr, i, a.kmax, gs), a.nnb[i],
+                     [&](uint4 e4, ListRef<L> pf, uint4& slot) {
+                       density_group<R, NT1>(e4, pf, slot, rec, off, ui, it,
+                                             nt, K, cpr, col0, rho, is_short,
+                                             r2min);
+                     });
+        if (NT1) is_short |= (r2min < b0);
+      }
+      rho = lane_group_sum<L>(rho);
+      if (gs == p) rho_keep = rho;
+    }
+    int i = c * IMDG_CHUNK + epi_slot(lane, L);
     double emb = 0.0;
     if (i - Rp.atom_off < Rp.natoms) {
       R4 ui = rec[i];
       int it = NT1 ? 0 : (int)ui.w;
-      const uint4* nb = list_of(a.nbr, i, a.kmax);
-      R rho = (R)0;
-      DensityCol<R> col0 = {kb0, len0, b0, e0, is0};
-      R r2min = (R)3.0e38;
-      list_walk(nb, a.nnb[i], [&](uint4 e4, const uint4* pf, uint4& slot) {
-        density_group<R, NT1>(e4, pf, slot, rec, off, ui, it, nt, K, cpr,
-                              col0, rho, is_short, r2min);
-      });
-      if (NT1) is_short |= (r2min < b0);
+      R rho = rho_keep;
       // embedding: PAIR_INT2 on the embedding column of the type, argument
       // rho (knots in HBM; one lookup per atom)
       const R2* __restrict__ ke = a.knots + te.off[it];
@@ -799,15 +867,16 @@ __global__ void __launch_bounds__(IMDG_DENSITY_THREADS)
 // over the full list of (u, 2F') records; with INTEGRATE the leapfrog step,
 // the skin test and the step's reductions follow in the same thread
 // (calc_forces, move_atoms and check_nblist of one main_loop iteration in
-// one launch).
-template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT>
+// one launch).  L lanes per slot.
+template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT,
+          int L>
 __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
     k_force(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
   constexpr bool NT1 = MODE != FM_NTN;
   typedef typename V4<R>::T R4;
   typedef typename V2<R>::T R2;
   extern __shared__ __align__(16) unsigned char dyn_lds[];
-  __shared__ R woff[IMDG_STEP_WAVES][3 * IMDG_NCODES];
+  __shared__ R4 woff[IMDG_STEP_WAVES][IMDG_NCODES];
   __shared__ ColPar<R> cpp[IMDG_MAXCOLS], cpr[IMDG_MAXCOLS];
   if (a.ctl->frozen) return;
   const TabDev<R>& tp = tabs[0];
@@ -835,7 +904,8 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
 
   Sched S = make_sched(a.nchunk, a.xcd_remap);
   const int lane = threadIdx.x & (IMDG_WAVE - 1);
-  R* off = woff[threadIdx.x / IMDG_WAVE];
+  const int ga = lane / L, gs = lane % L;
+  R4* off = woff[threadIdx.x / IMDG_WAVE];
   int cur = -1, is_short = 0;
   double acc[PB_N] = {0, 0, 0, 0, 0, 0};
   for (int c = S.c; c < S.c1; c += S.stride) {
@@ -845,50 +915,75 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
       wave_offsets(Rp, off);
       cur = r;
     }
-    int i = c * IMDG_CHUNK + lane;
+    // the slot's sums kept by this lane (pass gs)
+    R kx = 0, ky = 0, kz = 0, kep = 0, kvir = 0;
+#pragma unroll 1
+    for (int p = 0; p < L; p++) {
+      int i = c * IMDG_CHUNK + p * (IMDG_WAVE / L) + ga;
+      R fx = 0, fy = 0, fz = 0, ep = 0, vir = 0;
+      if (i - Rp.atom_off < Rp.natoms) {
+        R4 ui = rec[i];
+        int it = NT1 ? 0 : (int)atype[i];
+        R dFi = HE ? ui.w : (R)0;
+        const uint32_t* nw = list_words<L>(a.nbr, i, a.kmax, gs);
+        if (MODE == FM_SG) {
+          // one grid: sums in units of 1/(2 istep), scaled once per slot
+          R r2min = (R)3.0e38;
+          const R4* __restrict__ J =
+              reinterpret_cast<const R4*>(K + (a.joint - klo));
+          list_walk<L>(nw, a.nnb[i],
+                       [&](uint4 e4, ListRef<L> pf, uint4& slot) {
+                         force_group_sg<R>(e4, pf, slot, rec, off, ui,
+                                           (R)0.5 * dFi, J, 0, plen, pb, pe,
+                                           pis, fx, fy, fz, ep, vir, r2min);
+                       });
+          is_short |= (r2min < pb);
+        } else {
+          ForceCols<R> fc = {pk0, plen, pb, pe, pis, p2is,
+                             rk0, rlen, rb, re, ris, r2is};
+          list_walk<L>(nw, a.nnb[i],
+                       [&](uint4 e4, ListRef<L> pf, uint4& slot) {
+                         force_group<R, NT1, HP, HE>(
+                             e4, pf, slot, rec, atype, off, ui, dFi, it, nt,
+                             K, cpp, cpr, fc, fx, fy, fz, ep, vir, is_short);
+                       });
+        }
+      }
+      fx = lane_group_sum<L>(fx);
+      fy = lane_group_sum<L>(fy);
+      fz = lane_group_sum<L>(fz);
+      ep = lane_group_sum<L>(ep);
+      vir = lane_group_sum<L>(vir);
+      if (gs == p) {
+        kx = fx;
+        ky = fy;
+        kz = fz;
+        kep = ep;
+        kvir = vir;
+      }
+    }
+    if (MODE == FM_SG) {
+      kx *= p2is;
+      ky *= p2is;
+      kz *= p2is;
+      kvir *= p2is;
+    }
+    int i = c * IMDG_CHUNK + epi_slot(lane, L);
     double v[PB_N] = {0, 0, 0, 0, 0, 0};
     if (i - Rp.atom_off < Rp.natoms) {
-      R4 ui = rec[i];
-      int it = NT1 ? 0 : (int)atype[i];
-      R dFi = HE ? ui.w : (R)0;
-      const uint4* nb = list_of(a.nbr, i, a.kmax);
-      R fx = 0, fy = 0, fz = 0, ep = 0, vir = 0;
-      if (MODE == FM_SG) {
-        // one grid: sums in units of 1/(2 istep), scaled once per atom
-        R r2min = (R)3.0e38;
-        const R4* __restrict__ J =
-            reinterpret_cast<const R4*>(K + (a.joint - klo));
-        list_walk(nb, a.nnb[i], [&](uint4 e4, const uint4* pf, uint4& slot) {
-          force_group_sg<R>(e4, pf, slot, rec, off, ui, (R)0.5 * dFi, J, 0,
-                            plen, pb, pe, pis, fx, fy, fz, ep, vir, r2min);
-        });
-        fx *= p2is;
-        fy *= p2is;
-        fz *= p2is;
-        vir *= p2is;
-        is_short |= (r2min < pb);
-      } else {
-        ForceCols<R> fc = {pk0, plen, pb, pe, pis, p2is,
-                           rk0, rlen, rb, re, ris, r2is};
-        list_walk(nb, a.nnb[i], [&](uint4 e4, const uint4* pf, uint4& slot) {
-          force_group<R, NT1, HP, HE>(e4, pf, slot, rec, atype, off, ui, dFi,
-                                      it, nt, K, cpp, cpr, fc, fx, fy, fz, ep,
-                                      vir, is_short);
-        });
-      }
       // per-atom energy: half of every pair term plus F(rho_i)
-      R ei = (R)0.5 * ep + (HE ? a.epot[i] : (R)0);
+      R ei = (R)0.5 * kep + (HE ? a.epot[i] : (R)0);
       a.epot[i] = ei;
       R4 fv;
-      fv.x = fx;
-      fv.y = fy;
-      fv.z = fz;
+      fv.x = kx;
+      fv.y = ky;
+      fv.z = kz;
       fv.w = (R)0;
       a.frc[i] = fv;
-      v[PB_EPAIR] = 0.5 * (double)ep;
-      v[PB_VIRIAL] = -0.5 * (double)vir;
+      v[PB_EPAIR] = 0.5 * (double)kep;
+      v[PB_VIRIAL] = -0.5 * (double)kvir;
       if (INTEGRATE)
-        integrate_slot(a, Rp, r, i, (double)fx, (double)fy, (double)fz, v);
+        integrate_slot(a, Rp, r, i, (double)kx, (double)ky, (double)kz, v);
     }
     put_part(a.partB, a.part_chunk, c, v, acc);
   }
@@ -934,46 +1029,60 @@ __global__ void __launch_bounds__(256) k_copyrec(StepArgs<R> a, int nslots) {
 // ---------------------------------------------------------------------------
 
 template <typename R, bool NT1, bool LT>
-const void* density_fn() {
-  return reinterpret_cast<const void*>(&k_density<R, NT1, LT>);
+const void* density_fn(int lpa) {
+  if (lpa == 4) return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 4>);
+  if (lpa == 2) return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 2>);
+  return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 1>);
 }
 
 template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT>
-const void* force_fn3() {
+const void* force_fn3(int lpa) {
+  if (lpa == 4)
+    return reinterpret_cast<const void*>(
+        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 4>);
+  if (lpa == 2)
+    return reinterpret_cast<const void*>(
+        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 2>);
   return reinterpret_cast<const void*>(
-      &k_force<R, MODE, INTEGRATE, HP, HE, LT>);
+      &k_force<R, MODE, INTEGRATE, HP, HE, LT, 1>);
 }
 
 template <typename R, int MODE, bool INTEGRATE, bool LT>
-const void* force_fn2(bool hp, bool he) {
-  if (MODE == FM_SG) return force_fn3<R, MODE, INTEGRATE, true, true, LT>();
-  if (hp && he) return force_fn3<R, MODE, INTEGRATE, true, true, LT>();
-  if (hp) return force_fn3<R, MODE, INTEGRATE, true, false, LT>();
-  return force_fn3<R, MODE, INTEGRATE, false, true, LT>();
+const void* force_fn2(bool hp, bool he, int lpa) {
+  if (MODE == FM_SG)
+    return force_fn3<R, MODE, INTEGRATE, true, true, LT>(lpa);
+  if (hp && he) return force_fn3<R, MODE, INTEGRATE, true, true, LT>(lpa);
+  if (hp) return force_fn3<R, MODE, INTEGRATE, true, false, LT>(lpa);
+  return force_fn3<R, MODE, INTEGRATE, false, true, LT>(lpa);
 }
 
 template <typename R, int MODE>
-const void* force_fn1(bool integrate, bool lt, bool hp, bool he) {
+const void* force_fn1(bool integrate, bool lt, bool hp, bool he, int lpa) {
   if (integrate)
-    return lt ? force_fn2<R, MODE, true, true>(hp, he)
-              : force_fn2<R, MODE, true, false>(hp, he);
-  return lt ? force_fn2<R, MODE, false, true>(hp, he)
-            : force_fn2<R, MODE, false, false>(hp, he);
+    return lt ? force_fn2<R, MODE, true, true>(hp, he, lpa)
+              : force_fn2<R, MODE, true, false>(hp, he, lpa);
+  return lt ? force_fn2<R, MODE, false, true>(hp, he, lpa)
+            : force_fn2<R, MODE, false, false>(hp, he, lpa);
 }
 
 // mode: 0 one species, 1 one species on a shared grid (needs hp and he),
-// 2 several species
+// 2 several species; lpa: lanes per slot (1, 2 or 4)
 template <typename R>
-const void* force_fn(int mode, bool integrate, bool lt, bool hp, bool he) {
-  if (mode == FM_SG) return force_fn1<R, FM_SG>(integrate, lt, true, true);
-  if (mode == FM_NT1) return force_fn1<R, FM_NT1>(integrate, lt, hp, he);
-  return force_fn1<R, FM_NTN>(integrate, lt, hp, he);
+const void* force_fn(int mode, bool integrate, bool lt, bool hp, bool he,
+                     int lpa) {
+  if (mode == FM_SG)
+    return force_fn1<R, FM_SG>(integrate, lt, true, true, lpa);
+  if (mode == FM_NT1) return force_fn1<R, FM_NT1>(integrate, lt, hp, he, lpa);
+  return force_fn1<R, FM_NTN>(integrate, lt, hp, he, lpa);
 }
 
 template <typename R>
-const void* density_fn(bool nt1, bool lt) {
-  if (nt1) return lt ? density_fn<R, true, true>() : density_fn<R, true, false>();
-  return lt ? density_fn<R, false, true>() : density_fn<R, false, false>();
+const void* density_fn(bool nt1, bool lt, int lpa) {
+  if (nt1)
+    return lt ? density_fn<R, true, true>(lpa)
+              : density_fn<R, true, false>(lpa);
+  return lt ? density_fn<R, false, true>(lpa)
+            : density_fn<R, false, false>(lpa);
 }
 
 // Persistent grid of a kernel: resident blocks per CU (from the occupancy
@@ -1013,9 +1122,9 @@ static int persistent_grid(const void* fn, int threads, size_t lds, int ncu) {
 }
 
 template <typename R>
-int step_grid_density(bool nt1, bool lt, size_t lds, int ncu) {
-  return persistent_grid(density_fn<R>(nt1, lt), IMDG_DENSITY_THREADS, lds,
-                         ncu);
+int step_grid_density(bool nt1, bool lt, size_t lds, int ncu, int lpa,
+                      int threads) {
+  return persistent_grid(density_fn<R>(nt1, lt, lpa), threads, lds, ncu);
 }
 
 template <typename R>
@@ -1027,9 +1136,10 @@ int force_threads(int mode) {
 
 template <typename R>
 int step_grid_force(int mode, bool integrate, bool lt, bool hp, bool he,
-                    size_t lds, int ncu) {
-  return persistent_grid(force_fn<R>(mode, integrate, lt, hp, he),
-                         force_threads<R>(mode), lds, ncu);
+                    size_t lds, int ncu, int lpa, int threads) {
+  if (threads > force_threads<R>(mode)) return 0;
+  return persistent_grid(force_fn<R>(mode, integrate, lt, hp, he, lpa),
+                         threads, lds, ncu);
 }
 
 template <typename R>
@@ -1048,8 +1158,8 @@ hipError_t launch_density(const StepArgs<R>& a, const TabDev<R>* tabs,
     const TabDev<R>* t;
   } pk = {a, tabs};
   void* kargs[] = {&pk.a, &pk.t};
-  return hipLaunchKernel(density_fn<R>(nt1, lt), dim3(grid),
-                         dim3(IMDG_DENSITY_THREADS), kargs, lds, st);
+  return hipLaunchKernel(density_fn<R>(nt1, lt, a.lpa_den), dim3(grid),
+                         dim3(a.thr_den), kargs, lds, st);
 }
 
 template <typename R>
@@ -1061,8 +1171,9 @@ hipError_t launch_force(const StepArgs<R>& a, const TabDev<R>* tabs,
   const TabDev<R>* tt = tabs;
   void* kargs[] = {&aa, &tt};
   return hipLaunchKernel(
-      force_fn<R>(mode, integrate, lt, a.has_pair != 0, a.has_eam != 0),
-      dim3(grid), dim3(force_threads<R>(mode)), kargs, lds, st);
+      force_fn<R>(mode, integrate, lt, a.has_pair != 0, a.has_eam != 0,
+                  a.lpa_frc),
+      dim3(grid), dim3(a.thr_frc), kargs, lds, st);
 }
 
 template <typename R>
@@ -1081,8 +1192,10 @@ hipError_t launch_copyrec(const StepArgs<R>& a, int nslots, hipStream_t st) {
 }
 
 #define IMDG_INSTANTIATE(R)                                                   \
-  template int step_grid_density<R>(bool, bool, size_t, int);                \
-  template int step_grid_force<R>(int, bool, bool, bool, bool, size_t, int);  \
+  template int step_grid_density<R>(bool, bool, size_t, int, int, int);      \
+  template int step_grid_force<R>(int, bool, bool, bool, bool, size_t, int,  \
+                                  int, int);                                  \
+  template int force_threads<R>(int);                                         \
   template int step_grid_move<R>(int);                                        \
   template hipError_t launch_density<R>(const StepArgs<R>&,                   \
                                         const TabDev<R>*, bool, int,          \
@@ -1092,7 +1205,13 @@ hipError_t launch_copyrec(const StepArgs<R>& a, int nslots, hipStream_t st) {
   template hipError_t launch_move<R>(const StepArgs<R>&, int, hipStream_t);   \
   template hipError_t launch_copyrec<R>(const StepArgs<R>&, int, hipStream_t);
 
+// One precision per object when IMDG_STEP_FP32 or IMDG_STEP_FP64 is
+// defined (the build compiles this file once per type, in parallel).
+#if defined(IMDG_STEP_FP32) || !defined(IMDG_STEP_FP64)
 IMDG_INSTANTIATE(float)
+#endif
+#if defined(IMDG_STEP_FP64) || !defined(IMDG_STEP_FP32)
 IMDG_INSTANTIATE(double)
+#endif
 
 }  // namespace imdg

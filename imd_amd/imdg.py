@@ -28,7 +28,8 @@ SYMBOLS = [
     "imdg_deallocate_nblist", "imdg_run", "imdg_sync", "imdg_get_atoms",
     "imdg_get_thermo", "imdg_get_thermo_history", "imdg_get_cells",
     "imdg_get_pairs", "imdg_get_grid", "imdg_get_info", "imdg_profile",
-    "imdg_kernel_times", "imdg_struct_size",
+    "imdg_kernel_times", "imdg_struct_size", "imdg_get_tuning",
+    "imdg_set_tuning", "imdg_autotune",
 ]
 
 
@@ -68,6 +69,12 @@ class Info(C.Structure):
                 ("is_short", C.c_int), ("lanes_per_atom", C.c_int),
                 ("list_bytes", C.c_double), ("mean_k", C.c_double),
                 ("tiles_fallback", C.c_int)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("density_lanes", C.c_int), ("density_threads", C.c_int),
+                ("force_lanes", C.c_int), ("force_threads", C.c_int),
+                ("density_ms", C.c_double), ("force_ms", C.c_double)]
 
 
 _lib = None
@@ -335,6 +342,29 @@ class Context:
         i = Info()
         self._chk(lib().imdg_get_info(self._h, C.byref(i)), "imdg_get_info")
         return {k: getattr(i, k) for k, _ in Info._fields_}
+
+    def tuning(self):
+        """Launch shape of the density and force loops (lanes per atom,
+        threads per block)."""
+        t = Tuning()
+        self._chk(lib().imdg_get_tuning(self._h, C.byref(t)),
+                  "imdg_get_tuning")
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def set_tuning(self, density_lanes, density_threads, force_lanes,
+                   force_threads):
+        t = Tuning(density_lanes, density_threads, force_lanes,
+                   force_threads, 0.0, 0.0)
+        self._chk(lib().imdg_set_tuning(self._h, C.byref(t)),
+                  "imdg_set_tuning")
+
+    def autotune(self, reps=5):
+        """Time every launch shape on the current state, keep the fastest;
+        returns the chosen shape with its mean launch times (ms)."""
+        t = Tuning()
+        self._chk(lib().imdg_autotune(self._h, int(reps), C.byref(t)),
+                  "imdg_autotune")
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     def profile(self, enable=True):
         self._chk(lib().imdg_profile(self._h, int(enable)), "imdg_profile")

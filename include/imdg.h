@@ -65,7 +65,8 @@ typedef struct {
   double nbl_margin;     /* IMD key "nbl_margin" (list skin) */
   double nbl_size;       /* IMD key "nbl_size": list capacity headroom */
   int    device;         /* HIP device ordinal */
-  int    lanes_per_atom; /* 1, 2, 4, 8 or 0 = choose automatically */
+  int    lanes_per_atom; /* lanes per atom of both neighbour loops: 1, 2
+                            or 4; 0 = the default (see imdg_tuning) */
 } imdg_config;
 
 /* A read IMD function table (pot_table_t, src/types.h:416-428), after
@@ -112,7 +113,7 @@ typedef struct {
   long   nbl_count;     /* list builds of replica 0 */
   long   steps_done;    /* device step counter */
   int    is_short;      /* some r^2 fell below a table's begin */
-  int    lanes_per_atom;
+  int    lanes_per_atom;/* lanes per atom of the force loop */
   double list_bytes;    /* bytes of list storage */
   double mean_k;        /* mean full-list length per real atom */
   int    tiles_fallback;/* blocks whose neighbour tile did not fit in LDS
@@ -195,8 +196,38 @@ int  imdg_profile(imdg_ctx *ctx, int enable);
 int  imdg_kernel_times(imdg_ctx *ctx, const char **names, double *ms,
                        long *launches, int n);
 
+/* Launch shape of the two neighbour loops (no IMD counterpart: IMD runs
+   them as plain loops over atoms).  Each atom's list is walked by `lanes`
+   lanes of a wavefront, lane s taking entries s, s+lanes, ... (so every
+   lane handles about K/lanes neighbours, and the records one gather reads
+   for an atom are consecutive list entries); a block of `threads` threads
+   holds threads/lanes atoms at a time.  Results differ between shapes only
+   by the order of floating-point sums. */
+typedef struct {
+  int    density_lanes;   /* 1, 2 or 4 */
+  int    density_threads; /* 256, 512 or 1024 */
+  int    force_lanes;     /* 1, 2 or 4 */
+  int    force_threads;   /* 256, 512, or up to the force loop's maximum
+                             (768 mixed, 512 FP64) */
+  double density_ms;      /* imdg_autotune: mean time of the chosen density
+                             launch (0 otherwise) */
+  double force_ms;        /* imdg_autotune: same for the force launch */
+} imdg_tuning;
+
+int  imdg_get_tuning(imdg_ctx *ctx, imdg_tuning *out);
+/* Takes effect for the next launch; IMDG_ERR_ARG (and no change) for a
+   shape the loops cannot be launched with. */
+int  imdg_set_tuning(imdg_ctx *ctx, const imdg_tuning *t);
+/* Times every lanes x threads shape of the density loop and then of the
+   force loop on the current state (reps launches each, the list is built
+   first if needed) and keeps the fastest of each.  The state does not
+   advance; forces and energies are left as imdg_calc_forces leaves them.
+   *chosen (may be NULL) receives the result. */
+int  imdg_autotune(imdg_ctx *ctx, int reps, imdg_tuning *chosen);
+
 /* sizeof() of a public struct by name ("imdg_config", "imdg_table",
-   "imdg_replica", "imdg_thermo", "imdg_info"), -1 for an unknown name: lets
+   "imdg_replica", "imdg_thermo", "imdg_info", "imdg_tuning"), -1 for an
+   unknown name: lets
    a binding in another language check its layouts against the library's.
    No IMD counterpart (IMD links its force module statically). */
 long imdg_struct_size(const char *name);

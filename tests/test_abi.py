@@ -47,7 +47,7 @@ def test_struct_sizes_match_library(lib):
     lib.imdg_struct_size.restype = C.c_long
     for name, cls in (("imdg_config", G.Config), ("imdg_table", G.Table),
                       ("imdg_replica", G.Replica), ("imdg_thermo", G.Thermo),
-                      ("imdg_info", G.Info)):
+                      ("imdg_info", G.Info), ("imdg_tuning", G.Tuning)):
         assert lib.imdg_struct_size(name.encode()) == C.sizeof(cls), name
     assert lib.imdg_struct_size(b"nonsense") == -1
 
