@@ -273,10 +273,22 @@ def config_run(args):
     else:
         raise SystemExit("unknown config %s" % name)
     natoms = sum(x.n for x in systems)
+    # the batched replicas replay one captured step per step (hipGraph)
+    graph = name == "c3"
     res = {}
     for label, prec in runs:
-        ctx = make(prec)
+        old = os.environ.get("IMDG_GRAPH")
+        os.environ["IMDG_GRAPH"] = "1" if graph else "0"
+        try:
+            ctx = make(prec)
+        finally:
+            if old is None:
+                del os.environ["IMDG_GRAPH"]
+            else:
+                os.environ["IMDG_GRAPH"] = old
         ctx.run(args.warmup)
+        tuning = ctx.tuning() if args.no_autotune else ctx.autotune(reps=3)
+        ctx.run(2)  # capture the step with the chosen shapes
         nb0 = ctx.info()["nbl_count"]
         ctx.sync()
         t0 = time.perf_counter()
@@ -288,7 +300,9 @@ def config_run(args):
         res[label] = {"atom_steps_per_s": natoms * args.steps / el,
                       "ms_per_step": 1e3 * el / args.steps,
                       "mean_neighbours": info["mean_k"],
-                      "list_rebuilds_timed": int(info["nbl_count"] - nb0)}
+                      "list_rebuilds_timed": int(info["nbl_count"] - nb0),
+                      "launch_shape": _shape(tuning),
+                      "step": "hipGraph replay" if graph else "stream launches"}
     out = {"metric": "atom-steps/s", "config": {"workload": CONFIGS[name],
                                                  "name": name,
                                                  "natoms": natoms,
@@ -299,6 +313,16 @@ def config_run(args):
                                   / res["fp64"]["atom_steps_per_s"])
     print(json.dumps(out))
     return 0
+
+
+def _shape(t):
+    """Launch shape of the two neighbour loops as reported by the library
+    (atoms per block = threads / lanes)."""
+    return {"density_lanes_per_atom": t["density_lanes"],
+            "density_threads": t["density_threads"],
+            "force_lanes_per_atom": t["force_lanes"],
+            "force_threads": t["force_threads"],
+            "autotuned": t["force_ms"] > 0}
 
 
 def main(argv=None):
@@ -325,6 +349,9 @@ def main(argv=None):
     ap.add_argument("--config", choices=sorted(CONFIGS), default=None,
                     help="measure one of the other configurations instead "
                          "of the 2M-atom metric workload")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of timing "
+                         "every lanes-per-atom x block-size shape first")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the timed run of the other precision and the "
                          "per-quantity errors of mixed vs FP64")
@@ -359,21 +386,34 @@ def main(argv=None):
                             ensemble=G.NVT, timestep=W.FS, nbl_margin=0.4)
         ctx.set_nvt(0, W.kT(args.temp), 0.0, isq)
         ctx.run(args.warmup)
+        # launch shapes: timed on the warmed-up state (which it leaves as it
+        # is), before the timed region
+        tuning = ctx.tuning() if args.no_autotune else ctx.autotune(reps=3)
         nb0 = ctx.info()["nbl_count"]
+        # the timed region records HIP events around the force launches only
+        # (the roofline kernel); the other classes are timed afterwards
         if profile:
-            ctx.profile(True)
+            ctx.profile(True, classes=("force",))
         ctx.sync()
         t0 = time.perf_counter()
         ctx.run(args.steps, sync=False)
         ctx.sync()
         el = time.perf_counter() - t0
         info = ctx.info()
-        kt = ctx.kernel_times() if profile else {}
         th = ctx.thermo()
+        kt = {}
+        if profile:
+            kt = {"force": ctx.kernel_times()["force"]}
+            ctx.profile(True)
+            ctx.run(min(args.steps, 200))
+            other = ctx.kernel_times()
+            for k in ("density", "rebuild", "finalize"):
+                kt[k] = other[k]
+            ctx.profile(False)
         ctx.close()
-        return el, info, kt, th, info["nbl_count"] - nb0
+        return el, info, kt, th, info["nbl_count"] - nb0, tuning
 
-    el, info, kt, th, rebuilds = run_one(mixed, not args.no_profile)
+    el, info, kt, th, rebuilds, tuning = run_one(mixed, not args.no_profile)
     natoms = s.n
     value = natoms * args.steps / el
     ms_per_step = 1e3 * el / args.steps
@@ -399,6 +439,9 @@ def main(argv=None):
                 / (dms / dn / 1e3) / 1e9, 1)
         if "rebuild" in kt and kt["rebuild"][1] > 0:
             roofline["rebuild_ms_per_step"] = round(kt["rebuild"][0] / kt["rebuild"][1], 4)
+        if "finalize" in kt and kt["finalize"][1] > 0:
+            roofline["finalize_avg_launch_us"] = round(
+                kt["finalize"][0] / kt["finalize"][1] * 1e3, 2)
 
     if roofline is not None and traffic is not None:
         # HBM bytes per launch: calibrated FETCH_SIZE + WRITE_SIZE
@@ -412,10 +455,11 @@ def main(argv=None):
 
     compare = None
     if not args.no_compare:
-        el2, _, _, _, _ = run_one(not mixed, False)
+        el2, _, _, _, _, tun2 = run_one(not mixed, False)
         other = natoms * args.steps / el2
         compare = {"other_precision": "fp64" if mixed else "mixed",
                    "other_value": round(other, 1),
+                   "other_tuning": _shape(tun2),
                    "mixed_over_fp64": round(value / other if mixed
                                             else other / value, 3),
                    "errors_mixed_vs_fp64": precision_errors(G, eam, s)}
@@ -441,6 +485,7 @@ def main(argv=None):
                    "mean_neighbours": round(k_mean, 2),
                    "list_rebuilds_timed": int(rebuilds),
                    "epot_per_atom": th["epot"] / natoms,
+                   "launch_shape": _shape(tuning),
                    "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

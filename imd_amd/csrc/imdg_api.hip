@@ -35,6 +35,8 @@ hipError_t launch_force(const StepArgs<R>&, const TabDev<R>*, int, bool,
 template <typename R>
 hipError_t launch_move(const StepArgs<R>&, int, hipStream_t);
 template <typename R>
+hipError_t launch_finalize(const StepArgs<R>&, bool, bool, hipStream_t);
+template <typename R>
 hipError_t launch_copyrec(const StepArgs<R>&, int, hipStream_t);
 void launch_build(const BuildArgs&, int, hipStream_t);
 void launch_check(const RepDev*, Ctl*, const int*, const double4*,
@@ -45,8 +47,9 @@ using namespace imdg;
 
 namespace {
 
-enum { KC_BUILD = 0, KC_DENSITY, KC_FORCE, KC_MOVE, KC_N };
-const char* kKernelNames[KC_N] = {"rebuild", "density", "force", "move"};
+enum { KC_BUILD = 0, KC_DENSITY, KC_FORCE, KC_MOVE, KC_FINAL, KC_N };
+const char* kKernelNames[KC_N] = {"rebuild", "density", "force", "move",
+                                  "finalize"};
 
 struct HostTable {
   int ncols = 0;
@@ -115,8 +118,9 @@ struct imdg_ctx {
   int knots_pair = 0, knots_rho = 0;   // first knot of the rho / embed tables
   int knots_embed = 0;
   int knots_joint = -1;                // interleaved (pair, rho), one species
+  int knots_dpoly = -1;                // FP32 rho segment polynomials
   int force_mode = 0;                  // 0 one species, 1 shared grid, 2 several
-  int lpa_den = 1, lpa_frc = 1;        // lanes per slot of the two loops
+  int lpa_den = 0, lpa_frc = 0;        // lanes per slot of the two loops
   int thr_den = 0, thr_frc = 0;        // threads per block (0: default)
   int units_cap = 0;                   // wavefront partial slots allocated
   int lds_den_lo = 0, lds_den_n = 0;   // knots staged by the density loop
@@ -126,15 +130,23 @@ struct imdg_ctx {
   // one captured step (rebuild chain + density + force), replayed by
   // imdg_run when graphs are on (IMDG_GRAPH=1) and profiling is off
   bool use_graph = false;
+  // fused step: forces + integration in one launch (false: force launch,
+  // then the move launch; IMDG_SPLIT_MOVE=1)
+  bool split_move = false;
+  // the last block of the force / move launch ends the step (true:
+  // IMDG_FINAL_KERNEL=0) or a one-block launch after it does (default: a
+  // release fence in every block of the launch costs more than the launch)
+  bool final_inline = false;
   hipGraphExec_t step_graph = nullptr;
   bool atoms_set = false;
   long target_steps = 0;
   // profiling
   bool profiling = false;
+  unsigned prof_mask = 0;  // kernel classes timed (bit k: class k)
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> event_pool;
-  double kc_ms[KC_N] = {0, 0, 0, 0};
-  long kc_n[KC_N] = {0, 0, 0, 0};
+  double kc_ms[KC_N] = {};
+  long kc_n[KC_N] = {};
 };
 
 namespace {
@@ -266,9 +278,17 @@ int upload_tables(imdg_ctx* c) {
   c->knots_pair = first[0];
   c->knots_rho = first[1];
   c->knots_embed = first[2];
-  // one species with pair and rho on the same grid: also an interleaved
-  // table J[k] = (phi p0, phi dv, rho p0, rho dv), at an even knot index so
-  // that it can be read as 16 B (FP32) / 32 B (FP64) records
+  // one species with pair and rho on the same grid: also a segment table
+  // for the force loop at an even knot index, for the len segments k, with
+  // dv = p1 - p0 and d2v = p2 - 2 p1 + p0 as PAIR_INT2 forms them
+  // (src/potaccess.h:347-349), in FP64 and then rounded once.
+  //   FP64: S[k] = (phi p0, phi dv, phi d2v, rho dv), then rho d2v[k].
+  //   FP32: Q[k] = (phi p0, phi A, phi B, phi D), then (rho A, rho D)[k],
+  //         with A = dv - d2v/2, B = d2v/2, D = d2v: value p0 + chi (A +
+  //         B chi) and gradient / (2 istep) A + D chi.  Segment len-1 is all
+  //         zero: the reference evaluates it only at r^2 >= end, where the
+  //         shifted radial tables vanish, so a clamp of the table argument
+  //         replaces the cutoff tests.
   c->knots_joint = -1;
   const HostTable &hp = c->tab[0], &hr = c->tab[1];
   bool same = c->cfg.ntypes == 1 && hp.present && hr.present &&
@@ -279,9 +299,74 @@ int upload_tables(imdg_ctx* c) {
   if (same) {
     if (kn.size() & 1) kn.push_back(kn.back());
     c->knots_joint = (int)kn.size();
-    for (int k = 0; k <= hp.len[0]; k++) {
-      kn.push_back(kn[td[0].off[0] + k]);
-      kn.push_back(kn[td[1].off[0] + k]);
+    const int len = hp.len[0];
+    auto at = [](const HostTable& h, int k) {
+      return h.table[(size_t)k * h.ncols];
+    };
+    auto d2v = [&](const HostTable& h, int k) {
+      double p0 = at(h, k), p1 = at(h, k + 1), p2 = at(h, k + 2);
+      return p2 - 2 * p1 + p0;
+    };
+    if (sizeof(R) == 8) {
+      for (int k = 0; k < len; k++) {
+        R2 a, b;
+        a.x = (R)at(hp, k);
+        a.y = (R)(at(hp, k + 1) - at(hp, k));
+        b.x = (R)d2v(hp, k);
+        b.y = (R)(at(hr, k + 1) - at(hr, k));
+        kn.push_back(a);
+        kn.push_back(b);
+      }
+      for (int k = 0; k < len; k += 2) {
+        R2 q;
+        q.x = (R)d2v(hr, k);
+        q.y = k + 1 < len ? (R)d2v(hr, k + 1) : (R)0;
+        kn.push_back(q);
+      }
+    } else {
+      std::vector<R2> rr;
+      for (int k = 0; k < len; k++) {
+        R2 a, b, r;
+        a.x = a.y = b.x = b.y = r.x = r.y = (R)0;
+        if (k < len - 1) {
+          double dp = at(hp, k + 1) - at(hp, k), ddp = d2v(hp, k);
+          double dr = at(hr, k + 1) - at(hr, k), ddr = d2v(hr, k);
+          a.x = (R)at(hp, k);
+          a.y = (R)(dp - 0.5 * ddp);
+          b.x = (R)(0.5 * ddp);
+          b.y = (R)ddp;
+          r.x = (R)(dr - 0.5 * ddr);
+          r.y = (R)ddr;
+        }
+        kn.push_back(a);
+        kn.push_back(b);
+        rr.push_back(r);
+      }
+      kn.insert(kn.end(), rr.begin(), rr.end());
+    }
+  }
+  // FP32, one species: the density loop's rho table as segment polynomials
+  // (p0, A, B, 0) (value p0 + chi (A + B chi), A = dv - d2v/2, B = d2v/2),
+  // segment len-1 zero, like the force loop's segment table
+  c->knots_dpoly = -1;
+  if (sizeof(R) == 4 && c->cfg.ntypes == 1 && hr.present) {
+    if (kn.size() & 1) kn.push_back(kn.back());
+    c->knots_dpoly = (int)kn.size();
+    const int len = hr.len[0];
+    for (int k = 0; k < len; k++) {
+      R2 a, b;
+      a.x = a.y = b.x = b.y = (R)0;
+      if (k < len - 1) {
+        const double* t = hr.table.data();
+        const size_t nc = hr.ncols;
+        double p0 = t[k * nc], p1 = t[(k + 1) * nc], p2 = t[(k + 2) * nc];
+        double dv = p1 - p0, d2v = p2 - 2 * p1 + p0;
+        a.x = (R)p0;
+        a.y = (R)(dv - 0.5 * d2v);
+        b.x = (R)(0.5 * d2v);
+      }
+      kn.push_back(a);
+      kn.push_back(b);
     }
   }
   int rc = dalloc(c, &c->d_knots, kn.size() * sizeof(R2));
@@ -309,7 +394,13 @@ int plan_steps(imdg_ctx* c) {
   bool hp = c->tab[0].present, he = c->tab[1].present;
   c->force_mode = !nt1 ? 2 : (c->knots_joint >= 0 ? 1 : 0);
   int mode = c->force_mode;
-  if (c->thr_den <= 0) c->thr_den = IMDG_DENSITY_THREADS;
+  // default shapes (what imdg_autotune picks on the 2M-atom EAM crystal):
+  // four lanes per atom and full blocks, except the FP64 force loop, whose
+  // registers leave one lane per atom and 512 threads fastest
+  const bool fp32 = sizeof(R) == 4;
+  if (c->lpa_den <= 0) c->lpa_den = 4;
+  if (c->lpa_frc <= 0) c->lpa_frc = fp32 ? 4 : 1;
+  if (c->thr_den <= 0) c->thr_den = IMDG_STEP_THREADS;
   if (c->thr_frc <= 0) c->thr_frc = force_threads<R>(mode);
   int td = c->thr_den, tf = c->thr_frc;
   int no_lds = 0;
@@ -317,6 +408,10 @@ int plan_steps(imdg_ctx* c) {
   // density: the rho knots
   c->lds_den_lo = c->knots_rho;
   c->lds_den_n = he ? c->knots_embed - c->knots_rho : 0;
+  if (c->knots_dpoly >= 0) {
+    c->lds_den_lo = c->knots_dpoly;
+    c->lds_den_n = 2 * c->tab[1].len[0];
+  }
   c->grid_den = 0;
   if (he && !no_lds)
     c->grid_den = step_grid_density<R>(nt1, true, c->lds_den_n * sizeof(R2),
@@ -328,8 +423,9 @@ int plan_steps(imdg_ctx* c) {
   }
   // force: pair and rho knots (or the interleaved table)
   if (mode == 1) {
+    int len = c->tab[0].len[0];
     c->lds_frc_lo = c->knots_joint;
-    c->lds_frc_n = 2 * (c->tab[0].len[0] + 1);
+    c->lds_frc_n = sizeof(R) == 8 ? 2 * len + (len + 1) / 2 : 3 * len;
   } else {
     c->lds_frc_lo = hp ? c->knots_pair : c->knots_rho;
     c->lds_frc_n = c->knots_embed - c->lds_frc_lo;
@@ -387,6 +483,8 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.nnb = c->d_nnb;
   a.knots = (const typename V2<R>::T*)c->d_knots;
   a.joint = c->knots_joint;
+  a.dpoly = c->knots_dpoly;
+  a.final_inline = c->final_inline ? 1 : 0;
   a.lpa_den = c->lpa_den;
   a.lpa_frc = c->lpa_frc;
   a.rho = (R*)c->d_rho;
@@ -398,7 +496,7 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.nrep = c->nrep;
   a.nchunk = c->nchunk;
   a.part_chunk = c->part_chunk;
-  a.unitsA = c->grid_den * (c->thr_den / IMDG_WAVE);
+  a.unitsA = c->grid_den;  // one partial per block
   a.thr_den = c->thr_den;
   a.thr_frc = c->thr_frc;
   a.kmax = c->kmax;
@@ -467,15 +565,17 @@ struct ProfScope {
   imdg_ctx* c;
   int kc;
   hipEvent_t a = nullptr, b = nullptr;
-  ProfScope(imdg_ctx* c_, int kc_) : c(c_), kc(kc_) {
-    if (c->profiling) {
+  bool on;
+  ProfScope(imdg_ctx* c_, int kc_)
+      : c(c_), kc(kc_), on(c_->profiling && ((c_->prof_mask >> kc_) & 1)) {
+    if (on) {
       a = get_event(c);
       b = get_event(c);
       hipEventRecord(a, c->stream);
     }
   }
   ~ProfScope() {
-    if (c->profiling) {
+    if (on) {
       hipEventRecord(b, c->stream);
       c->pending.push_back({kc, {a, b}});
     }
@@ -527,11 +627,35 @@ hipError_t enqueue_forces_t(imdg_ctx* c, bool integrate) {
     }
     if (e != hipSuccess) return e;
   }
-  ProfScope ps(c, KC_FORCE);
   a.lds_lo = c->lds_frc_lo;
   a.lds_n = c->lds_frc_n;
-  return launch_force<R>(a, t, c->force_mode, integrate,
-                         c->grid_frc[integrate ? 1 : 0], c->stream);
+  int grid = c->grid_frc[integrate ? 1 : 0];
+  a.unitsB = grid;
+  {
+    ProfScope ps(c, KC_FORCE);
+    e = launch_force<R>(a, t, c->force_mode, integrate, grid, c->stream);
+  }
+  if (e == hipSuccess && !c->final_inline) {
+    ProfScope ps(c, KC_FINAL);
+    e = launch_finalize<R>(a, true, integrate, c->stream);
+  }
+  return e;
+}
+
+template <typename R>
+hipError_t enqueue_move_t(imdg_ctx* c) {
+  StepArgs<R> a = step_args<R>(c);
+  a.unitsB = c->grid_move;
+  hipError_t e;
+  {
+    ProfScope ps(c, KC_MOVE);
+    e = launch_move<R>(a, c->grid_move, c->stream);
+  }
+  if (e == hipSuccess && !c->final_inline) {
+    ProfScope ps(c, KC_FINAL);
+    e = launch_finalize<R>(a, false, true, c->stream);
+  }
+  return e;
 }
 
 int enqueue_forces(imdg_ctx* c, bool integrate) {
@@ -539,6 +663,15 @@ int enqueue_forces(imdg_ctx* c, bool integrate) {
     HIPCHK(c, enqueue_forces_t<float>(c, integrate));
   else
     HIPCHK(c, enqueue_forces_t<double>(c, integrate));
+  HIPCHK(c, hipGetLastError());
+  return IMDG_OK;
+}
+
+int enqueue_move(imdg_ctx* c) {
+  if (c->mixed)
+    HIPCHK(c, enqueue_move_t<float>(c));
+  else
+    HIPCHK(c, enqueue_move_t<double>(c));
   HIPCHK(c, hipGetLastError());
   return IMDG_OK;
 }
@@ -644,6 +777,7 @@ int time_loop(imdg_ctx* c, bool density, int reps, double* ms) {
     }
     a.lds_lo = c->lds_frc_lo;
     a.lds_n = c->lds_frc_n;
+    a.unitsB = c->grid_frc[0];
     return launch_force<R>(a, t, c->force_mode, false, c->grid_frc[0],
                            c->stream);
   };
@@ -673,7 +807,7 @@ int autotune_t(imdg_ctx* c, int reps, imdg_tuning* out) {
   static const int lanes[3] = {1, 2, 4};
   const int fmax = force_threads<R>(c->force_mode);
   const int thr_d[3] = {256, 512, 1024};
-  const int thr_f[3] = {256, 512, fmax};
+  const int thr_f[4] = {256, 512, 768, 1024};
   int rc;
   double best_d = 0.0, best_f = 0.0;
   if (c->tab[1].present) {
@@ -702,8 +836,8 @@ int autotune_t(imdg_ctx* c, int reps, imdg_tuning* out) {
     int bl = c->lpa_frc, bt = c->thr_frc;
     best_f = 1e300;
     for (int li = 0; li < 3; li++)
-      for (int ti = 0; ti < 3; ti++) {
-        if (ti == 2 && thr_f[2] <= thr_f[1]) continue;
+      for (int ti = 0; ti < 4; ti++) {
+        if (thr_f[ti] > fmax) continue;
         c->lpa_frc = lanes[li];
         c->thr_frc = thr_f[ti];
         if (plan(c) != IMDG_OK) continue;
@@ -772,6 +906,9 @@ int imdg_ctx_create(const imdg_config* cfg, imdg_ctx** out) {
   c->xcd_remap = remap ? atoi(remap) != 0 : 1;
   const char* graph = getenv("IMDG_GRAPH");
   c->use_graph = graph ? atoi(graph) != 0 : false;
+  if (const char* e = getenv("IMDG_SPLIT_MOVE")) c->split_move = atoi(e) != 0;
+  if (const char* e = getenv("IMDG_FINAL_KERNEL"))
+    c->final_inline = atoi(e) == 0;
   if (cfg->lanes_per_atom > 0)
     c->lpa_den = c->lpa_frc = lpa_value(cfg->lanes_per_atom);
   if (const char* e = getenv("IMDG_LPA_DENSITY")) c->lpa_den = lpa_value(atoi(e));
@@ -1094,16 +1231,7 @@ int imdg_move_atoms(imdg_ctx* c) {
   if (!c) return IMDG_ERR_ARG;
   int rc = ensure_ready(c);
   if (rc) return rc;
-  {
-    ProfScope ps(c, KC_MOVE);
-    if (c->mixed)
-      HIPCHK(c, launch_move<float>(step_args<float>(c), c->grid_move,
-                                   c->stream));
-    else
-      HIPCHK(c, launch_move<double>(step_args<double>(c), c->grid_move,
-                                    c->stream));
-  }
-  HIPCHK(c, hipGetLastError());
+  if ((rc = enqueue_move(c))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   collect_profile(c);
   c->target_steps += 1;
@@ -1184,7 +1312,8 @@ static int capture_step(imdg_ctx* c) {
   hipGraph_t g = nullptr;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = enqueue_build(c);
-  if (!rc) rc = enqueue_forces(c, true);
+  if (!rc) rc = enqueue_forces(c, !c->split_move);
+  if (!rc && c->split_move) rc = enqueue_move(c);
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
@@ -1208,7 +1337,8 @@ int imdg_run(imdg_ctx* c, long nsteps) {
   } else {
     for (long s = 0; s < nsteps; s++) {
       if ((rc = enqueue_build(c))) return rc;
-      if ((rc = enqueue_forces(c, true))) return rc;
+      if ((rc = enqueue_forces(c, !c->split_move))) return rc;
+      if (c->split_move && (rc = enqueue_move(c))) return rc;
     }
   }
   c->target_steps += nsteps;
@@ -1477,6 +1607,7 @@ int imdg_profile(imdg_ctx* c, int enable) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   collect_profile(c);
   c->profiling = enable != 0;
+  c->prof_mask = enable == 1 ? ~0u : (unsigned)enable;
   for (int k = 0; k < KC_N; k++) {
     c->kc_ms[k] = 0;
     c->kc_n[k] = 0;
@@ -1488,7 +1619,7 @@ int imdg_get_tuning(imdg_ctx* c, imdg_tuning* out) {
   if (!c || !out) return IMDG_ERR_ARG;
   memset(out, 0, sizeof *out);
   out->density_lanes = c->lpa_den;
-  out->density_threads = c->thr_den > 0 ? c->thr_den : IMDG_DENSITY_THREADS;
+  out->density_threads = c->thr_den;
   out->force_lanes = c->lpa_frc;
   out->force_threads = c->thr_frc;
   return IMDG_OK;

@@ -40,7 +40,6 @@
 #define IMDG_CHUNK 64        // slots per wavefront step of the step kernels
 #define IMDG_STEP_THREADS 1024   // largest block of the step kernels
 #define IMDG_STEP_WAVES (IMDG_STEP_THREADS / IMDG_WAVE)
-#define IMDG_DENSITY_THREADS 512 // density loop default: several blocks per CU
 #define IMDG_MAXTYPES 8
 #define IMDG_MAXCOLS (IMDG_MAXTYPES * IMDG_MAXTYPES)
 #define IMDG_RING 4096
@@ -194,6 +193,9 @@ struct StepArgs {
   int nchunk;             // 64-slot chunks of all replicas
   int part_chunk;         // partials per chunk (1) or per wavefront (0)
   int unitsA;             // wavefront partials written by the density loop
+  int unitsB;             // ... by the finalising force / move launch
+  int final_inline;       // the last block of that launch ends the step
+                          // (0: a separate one-block launch does)
   int kmax;
   int ntypes;
   int ensemble;
@@ -203,6 +205,8 @@ struct StepArgs {
   int lds_lo, lds_n;      // knots staged in LDS by this launch (n = 0: none)
   int joint;              // first knot of the interleaved single-species
                           // (pair, rho) table, -1 if there is none
+  int dpoly;              // first knot of the FP32 rho segment polynomials
+                          // (one species), -1 if there are none
   int lpa_den, lpa_frc;   // lanes per slot of the density / force loops
   int thr_den, thr_frc;   // threads per block of the density / force loops
 };

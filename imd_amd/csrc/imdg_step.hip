@@ -24,13 +24,14 @@
 
 namespace imdg {
 
-// Threads per block of the force loop: its 4-neighbour load phases need
-// more than the 128 VGPRs a 1024-thread block leaves a wave, and with the
-// tables in LDS one block per CU is resident anyway.
+// Largest block of the force loop (its launch bound).  FP32: 1024 threads
+// (16 wavefronts; the loop fits the 128 VGPRs that leaves each); FP64: 512
+// (its doubles need more registers).  With the tables in LDS one block per
+// CU is resident, so the block size is the resident wavefront count; the
+// launch may use fewer threads (imdg_set_tuning).
 template <typename R, int MODE>
 struct ForceThreads {
-  static constexpr int value =
-      sizeof(R) == 4 ? 768 : 512;
+  static constexpr int value = sizeof(R) == 4 ? 1024 : 512;
 };
 
 // Neighbour-loop variants of the force kernel.
@@ -118,14 +119,44 @@ __device__ __forceinline__ void put_part(double* partB, int part_chunk,
   }
 }
 
-__device__ __forceinline__ void flush_part(double* partB, int part_chunk,
-                                           int unit, double (&acc)[PB_N]) {
-  if (part_chunk) return;
-  wave_reduce_part(acc);
+// Sums of NQ per-wavefront values over the wavefronts of the block, in
+// wavefront order with TwoSum (the result does not depend on timing),
+// written as the block's partial out[q] by thread q.  Max instead of sum
+// for q == qmax.  Called by every thread of the block, after its loops.
+template <int NQ>
+__device__ __forceinline__ void block_partial(double* out, const double* v,
+                                              int qmax) {
+  __shared__ double lp[IMDG_STEP_WAVES][NQ];
+  int w = threadIdx.x / IMDG_WAVE, nw = blockDim.x / IMDG_WAVE;
   if ((threadIdx.x & (IMDG_WAVE - 1)) == 0) {
 #pragma unroll
-    for (int q = 0; q < PB_N; q++) partB[(size_t)unit * PB_N + q] = acc[q];
+    for (int q = 0; q < NQ; q++) lp[w][q] = v[q];
   }
+  __syncthreads();
+  int q = threadIdx.x;
+  if (q < NQ) {
+    double s = lp[0][q], c = 0.0;
+    for (int k = 1; k < nw; k++) {
+      if (q == qmax) {
+        s = fmax(s, lp[k][q]);
+      } else {
+        double t, e;
+        two_sum(s, lp[k][q], t, e);
+        s = t;
+        c += e;
+      }
+    }
+    out[q] = s + c;
+  }
+}
+
+// One replica: the wavefronts' running sums, reduced over the block into
+// the block's partial (partB[block]).
+__device__ __forceinline__ void flush_part(double* partB, int part_chunk,
+                                           double (&acc)[PB_N]) {
+  if (part_chunk) return;
+  wave_reduce_part(acc);
+  block_partial<PB_N>(partB + (size_t)blockIdx.x * PB_N, acc, PB_MAXD);
 }
 
 __device__ __forceinline__ void flag_short(Ctl* ctl, int is_short) {
@@ -235,7 +266,7 @@ __device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved) {
   if (!a.part_chunk) {
     // one replica: partials per wavefront of the finalising kernel (B) and
     // of the density kernel (A)
-    int unitsB = gridDim.x * nw;
+    int unitsB = a.unitsB;
     double s[PB_N + 1], c[PB_N + 1];
 #pragma unroll
     for (int q = 0; q <= PB_N; q++) s[q] = c[q] = 0.0;
@@ -342,14 +373,22 @@ __device__ __forceinline__ void last_block_finalize(StepArgs<R>& a,
 // integration
 // ---------------------------------------------------------------------------
 
+// Kinetic terms of one atom's step: NVE ke = (|p|^2 before + after) / 4m;
+// NVT e1, e2 = |p|^2 / m before and after.
+struct KinTerms {
+  double ke, e1, e2;
+};
+
 // Leapfrog on momenta (src/imd_integrate.c:213-217, 329-358 and
-// 1020-1056), FP64, separately rounded like the reference.
-__device__ __forceinline__ void leapfrog(int ensemble, double dt, double eta,
-                                         double fx, double fy, double fz,
-                                         double4& p, double& x0, double& x1,
-                                         double& x2, double& ke, double& e1,
-                                         double& e2) {
+// 1020-1056), FP64, separately rounded like the reference.  The NVT
+// factors reib and einsd are the same for every atom of the replica.
+__device__ __forceinline__ KinTerms leapfrog(int ensemble, double dt,
+                                             double reib, double einsd,
+                                             double fx, double fy, double fz,
+                                             double4& p, double& x0,
+                                             double& x1, double& x2) {
 #pragma clang fp contract(off)
+  KinTerms kt;
   double m = p.w;
   if (ensemble == 0) {
     double k1 = p.x * p.x + p.y * p.y + p.z * p.z;
@@ -357,22 +396,30 @@ __device__ __forceinline__ void leapfrog(int ensemble, double dt, double eta,
     p.y += dt * fy;
     p.z += dt * fz;
     double k2 = p.x * p.x + p.y * p.y + p.z * p.z;
-    ke = (k1 + k2) / (4 * m);
-    e1 = e2 = 0.0;
+    kt.ke = (k1 + k2) / (4 * m);
+    kt.e1 = kt.e2 = 0.0;
   } else {
-    double reib = 1.0 - eta * dt / 2.0;
-    double einsd = 1.0 / (1.0 + eta * dt / 2.0);
-    e1 = (p.x * p.x + p.y * p.y + p.z * p.z) / m;
+    kt.e1 = (p.x * p.x + p.y * p.y + p.z * p.z) / m;
     p.x = (p.x * reib + dt * fx) * einsd;
     p.y = (p.y * reib + dt * fy) * einsd;
     p.z = (p.z * reib + dt * fz) * einsd;
-    e2 = (p.x * p.x + p.y * p.y + p.z * p.z) / m;
-    ke = 0.0;
+    kt.e2 = (p.x * p.x + p.y * p.y + p.z * p.z) / m;
+    kt.ke = 0.0;
   }
   double tmp = dt / m;
   x0 += tmp * p.x;
   x1 += tmp * p.y;
   x2 += tmp * p.z;
+  return kt;
+}
+
+// reib = 1 - eta dt/2, einsd = 1 / (1 + eta dt/2) of a replica
+// (src/imd_integrate.c:907-908).
+__device__ __forceinline__ void nvt_factors(double eta, double dt,
+                                            double& reib, double& einsd) {
+#pragma clang fp contract(off)
+  reib = 1.0 - eta * dt / 2.0;
+  einsd = 1.0 / (1.0 + eta * dt / 2.0);
 }
 
 // Integrate slot i, refresh its cell-relative record and return the
@@ -381,16 +428,18 @@ template <typename R>
 __device__ __forceinline__ void integrate_slot(StepArgs<R>& a,
                                                const RepDev& Rp, int r, int i,
                                                double fx, double fy, double fz,
-                                               double* v) {
+                                               double (&v)[PB_N]) {
   typedef typename V4<R>::T R4;
   int pc = a.ctl->pos_cur;
   double4* pos = a.pos[pc];
   double4* mom = a.mom[pc];
   double4 xi = pos[i];
   double4 p = mom[i];
-  double x0 = xi.x, x1 = xi.y, x2 = xi.z, ke, e1, e2;
-  leapfrog(a.ensemble, a.dt, a.rst[r].eta, fx, fy, fz, p, x0, x1, x2, ke, e1,
-           e2);
+  double x0 = xi.x, x1 = xi.y, x2 = xi.z;
+  double reib = 1.0, einsd = 1.0;
+  if (a.ensemble != 0) nvt_factors(a.rst[r].eta, a.dt, reib, einsd);
+  KinTerms kt =
+      leapfrog(a.ensemble, a.dt, reib, einsd, fx, fy, fz, p, x0, x1, x2);
   mom[i] = p;
   double4 xn = make_double4(x0, x1, x2, xi.w);
   pos[i] = xn;
@@ -402,9 +451,9 @@ __device__ __forceinline__ void integrate_slot(StepArgs<R>& a,
   ra.z = (R)(x2 - c[2]);
   ra.w = (R)xi.w;
   a.rec_a[i] = ra;
-  v[PB_KE] = ke;
-  v[PB_E1] = e1;
-  v[PB_E2] = e2;
+  v[PB_KE] = kt.ke;
+  v[PB_E1] = kt.e1;
+  v[PB_E2] = kt.e2;
   v[PB_MAXD] = disp2(xn, a.xnbl[i]);
 }
 
@@ -583,6 +632,52 @@ __device__ __forceinline__ void density_group(
   }
 }
 
+// FP32 single-species variant of density_group on the rho segment
+// polynomials DQ[k] = (p0, A, B, 0): value p0 + chi (A + B chi); the table
+// argument is clamped to [0, len - 1/2] (med3), so r^2 at or beyond end
+// reads the zero segment len - 1 and no cutoff test is needed.
+template <typename PF>
+__device__ __forceinline__ void density_group_poly(
+    uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
+    const float4* off, const float4& ui, const float4* __restrict__ DQ,
+    float tc0, float istep, float tz, float& rho, float& r2min) {
+  uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
+  float ux[4], uy[4], uz[4];
+  float4 o[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    // the type is not needed: gather 12 B
+    const float* p = reinterpret_cast<const float*>(rec + (en[u] & IMDG_IDX_MASK));
+    float2 xy = *reinterpret_cast<const float2*>(p);
+    ux[u] = xy.x;
+    uy[u] = xy.y;
+    uz[u] = p[2];
+    o[u] = off[en[u] >> IMDG_IDX_BITS];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  slot = pf.load();
+  float r2[4], chi[4];
+  int kk[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    float dx = (ux[u] - ui.x) + o[u].x;
+    float dy = (uy[u] - ui.y) + o[u].y;
+    float dz = (uz[u] - ui.z) + o[u].z;
+    r2[u] = dx * dx + dy * dy + dz * dz;
+    float t = __builtin_amdgcn_fmed3f(fmaf(r2[u], istep, tc0), 0.0f, tz);
+    kk[u] = (int)t;
+    chi[u] = __builtin_amdgcn_fractf(t);
+  }
+  float4 q[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) q[u] = DQ[kk[u]];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    rho += fmaf(chi[u], fmaf(chi[u], q[u].z, q[u].y), q[u].x);
+    r2min = fminf(r2min, r2[u]);
+  }
+}
+
 // Four neighbours of the force loop (same phases as density_group).
 template <typename R, bool NT1, bool HP, bool HE, typename R4, typename R2,
           typename PF>
@@ -700,16 +795,16 @@ __device__ __forceinline__ void force_group(
 }
 
 // Four neighbours of the single-species force loop when the pair and rho
-// tables share their grid: one segment per neighbour, knots J[k] =
-// (phi p0, phi dv, rho p0, rho dv) read as two 16 B LDS loads.  The common
-// factor 2 istep of both gradients is applied once per atom by the caller
-// (sums s* are in units of 1 / (2 istep)); hFi = F'(rho_i) = 2F'/2.
+// tables share their grid: one segment per neighbour, its record S[k] =
+// (phi p0, phi dv, phi d2v, rho dv) and Sr[k] = rho d2v.  The common factor
+// 2 istep of both gradients is applied once per atom by the caller (sums
+// s* are in units of 1 / (2 istep)); hFi = F'(rho_i) = 2F'/2.
 template <typename R, typename R4, typename PF>
 __device__ __forceinline__ void force_group_sg(
     uint4 e4, PF pf, uint4& slot, const R4* __restrict__ rec,
-    const R4* off, const R4& ui, R hFi, const R4* __restrict__ J, int jb,
-    int len, R begin, R end, R invstep, R& sx, R& sy, R& sz, R& ep, R& svir,
-    R& r2min) {
+    const R4* off, const R4& ui, R hFi, const R4* __restrict__ S,
+    const R* __restrict__ Sr, int len, R begin, R end, R invstep, R& sx,
+    R& sy, R& sz, R& ep, R& svir, R& r2min) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
   R4 uj[4], o[4];
 #pragma unroll
@@ -727,22 +822,22 @@ __device__ __forceinline__ void force_group_sg(
     dy[u] = (uj[u].y - ui.y) + o[u].y;
     dz[u] = (uj[u].z - ui.z) + o[u].z;
     r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-    kk[u] = jb + seg_locate(r2[u], begin, end, invstep, len, chi[u]);
+    kk[u] = seg_locate(r2[u], begin, end, invstep, len, chi[u]);
   }
-  R4 j0[4], j1[4];
+  R4 s4[4];
+  R d2r[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    j0[u] = J[kk[u]];
-    j1[u] = J[kk[u] + 1];
+    s4[u] = S[kk[u]];
+    d2r[u] = Sr[kk[u]];
   }
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     R c = chi[u] - (R)0.5;
-    R d2p = j1[u].y - j0[u].y;
-    R d2r = j1[u].w - j0[u].w;
-    R gp = fma(c, d2p, j0[u].y);
-    R gr = fma(c, d2r, j0[u].w);
-    R vp = fma(chi[u], fma((R)0.5 * (chi[u] - (R)1), d2p, j0[u].y), j0[u].x);
+    R gp = fma(c, s4[u].z, s4[u].y);
+    R gr = fma(c, d2r[u], s4[u].w);
+    R vp = fma(chi[u], fma((R)0.5 * (chi[u] - (R)1), s4[u].z, s4[u].y),
+               s4[u].x);
     // pair: r2 <= end (PAIR_INT2 guard); rho: r2 < end (VAL/DERIV_FUNC2)
     bool inp = r2[u] <= end, inr = r2[u] < end;
     R h = fma((R)0.5, uj[u].w, hFi);
@@ -753,6 +848,63 @@ __device__ __forceinline__ void force_group_sg(
     sz = fma(dz[u], g, sz);
     svir = fma(r2[u], g, svir);
     r2min = r2[u] < r2min ? r2[u] : r2min;
+  }
+}
+
+// FP32 variant of force_group_sg on the polynomial segment table: Q[k] =
+// (phi p0, phi A, phi B, phi D), Rr[k] = (rho A, rho D); value p0 + chi (A +
+// B chi), gradients / (2 istep) A + D chi.  The table argument t = (r^2 -
+// begin) istep is clamped to [0, len - 1/2] (med3): r^2 below begin reads
+// segment 0 at chi = 0 like the reference, r^2 at or beyond end reads the
+// zero segment len - 1, so no cutoff test is needed.
+template <typename PF>
+__device__ __forceinline__ void force_group_sgf(
+    uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
+    const float4* off, const float4& ui, float hFi,
+    const float4* __restrict__ Q, const float2* __restrict__ Rr, float tc0,
+    float istep, float tz, float& sx, float& sy, float& sz, float& ep,
+    float& svir, float& r2min) {
+  uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
+  float4 uj[4], o[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    uj[u] = rec[en[u] & IMDG_IDX_MASK];
+    o[u] = off[en[u] >> IMDG_IDX_BITS];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  slot = pf.load();
+  float dx[4], dy[4], dz[4], r2[4], chi[4];
+  int kk[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    dx[u] = (uj[u].x - ui.x) + o[u].x;
+    dy[u] = (uj[u].y - ui.y) + o[u].y;
+    dz[u] = (uj[u].z - ui.z) + o[u].z;
+    r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+    float t = __builtin_amdgcn_fmed3f(fmaf(r2[u], istep, tc0), 0.0f, tz);
+    kk[u] = (int)t;
+    chi[u] = __builtin_amdgcn_fractf(t);
+  }
+  float4 q[4];
+  float2 rr[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    q[u] = Q[kk[u]];
+    rr[u] = Rr[kk[u]];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    float gp = fmaf(chi[u], q[u].w, q[u].y);
+    float gr = fmaf(chi[u], rr[u].y, rr[u].x);
+    float vp = fmaf(chi[u], fmaf(chi[u], q[u].z, q[u].y), q[u].x);
+    float h = fmaf(0.5f, uj[u].w, hFi);
+    float g = fmaf(h, gr, gp);
+    ep += vp;
+    sx = fmaf(dx[u], g, sx);
+    sy = fmaf(dy[u], g, sy);
+    sz = fmaf(dz[u], g, sz);
+    svir = fmaf(r2[u], g, svir);
+    r2min = fminf(r2min, r2[u]);
   }
 }
 
@@ -809,12 +961,24 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
         int it = NT1 ? 0 : (int)ui.w;
         DensityCol<R> col0 = {kb0, len0, b0, e0, is0};
         R r2min = (R)3.0e38;
-        list_walk<L>(list_words<L>(a.nbr, i, a.kmax, gs), a.nnb[i],
-                     [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                       density_group<R, NT1>(e4, pf, slot, rec, off, ui, it,
-                                             nt, K, cpr, col0, rho, is_short,
-                                             r2min);
-                     });
+        const uint32_t* nw = list_words<L>(a.nbr, i, a.kmax, gs);
+        if constexpr (sizeof(R) == 4 && NT1) {
+          const float4* __restrict__ DQ =
+              reinterpret_cast<const float4*>(K + (a.dpoly - klo));
+          const float tc0 = -b0 * is0, tz = (float)len0 - 0.5f;
+          list_walk<L>(nw, a.nnb[i],
+                       [&](uint4 e4, ListRef<L> pf, uint4& slot) {
+                         density_group_poly(e4, pf, slot, rec, off, ui, DQ,
+                                            tc0, is0, tz, rho, r2min);
+                       });
+        } else {
+          list_walk<L>(nw, a.nnb[i],
+                       [&](uint4 e4, ListRef<L> pf, uint4& slot) {
+                         density_group<R, NT1>(e4, pf, slot, rec, off, ui, it,
+                                               nt, K, cpr, col0, rho,
+                                               is_short, r2min);
+                       });
+        }
         if (NT1) is_short |= (r2min < b0);
       }
       rho = lane_group_sum<L>(rho);
@@ -854,7 +1018,7 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
   }
   if (!a.part_chunk) {
     double s = wave_sum(acc);
-    if (lane == 0) a.partA[S.unit] = s;
+    block_partial<1>(a.partA + blockIdx.x, &s, -1);
   }
   flag_short(a.ctl, is_short);
 }
@@ -929,14 +1093,31 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
         if (MODE == FM_SG) {
           // one grid: sums in units of 1/(2 istep), scaled once per slot
           R r2min = (R)3.0e38;
-          const R4* __restrict__ J =
-              reinterpret_cast<const R4*>(K + (a.joint - klo));
-          list_walk<L>(nw, a.nnb[i],
-                       [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                         force_group_sg<R>(e4, pf, slot, rec, off, ui,
-                                           (R)0.5 * dFi, J, 0, plen, pb, pe,
-                                           pis, fx, fy, fz, ep, vir, r2min);
-                       });
+          if constexpr (sizeof(R) == 4) {
+            const float4* __restrict__ Q =
+                reinterpret_cast<const float4*>(K + (a.joint - klo));
+            const float2* __restrict__ Rr =
+                reinterpret_cast<const float2*>(K + (a.joint - klo) + 2 * plen);
+            const float tc0 = -pb * pis, tz = (float)plen - 0.5f;
+            list_walk<L>(nw, a.nnb[i],
+                         [&](uint4 e4, ListRef<L> pf, uint4& slot) {
+                           force_group_sgf(e4, pf, slot, rec, off, ui,
+                                           0.5f * dFi, Q, Rr, tc0, pis, tz,
+                                           fx, fy, fz, ep, vir, r2min);
+                         });
+          } else {
+            const R4* __restrict__ S =
+                reinterpret_cast<const R4*>(K + (a.joint - klo));
+            const R* __restrict__ Sr =
+                reinterpret_cast<const R*>(K + (a.joint - klo) + 2 * plen);
+            list_walk<L>(nw, a.nnb[i],
+                         [&](uint4 e4, ListRef<L> pf, uint4& slot) {
+                           force_group_sg<R>(e4, pf, slot, rec, off, ui,
+                                             (R)0.5 * dFi, S, Sr, plen, pb,
+                                             pe, pis, fx, fy, fz, ep, vir,
+                                             r2min);
+                         });
+          }
           is_short |= (r2min < pb);
         } else {
           ForceCols<R> fc = {pk0, plen, pb, pe, pis, p2is,
@@ -987,9 +1168,9 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
     }
     put_part(a.partB, a.part_chunk, c, v, acc);
   }
-  flush_part(a.partB, a.part_chunk, S.unit, acc);
+  flush_part(a.partB, a.part_chunk, acc);
   flag_short(a.ctl, is_short);
-  last_block_finalize(a, true, INTEGRATE);
+  if (a.final_inline) last_block_finalize(a, true, INTEGRATE);
 }
 
 // move_atoms + check_nblist on the stored forces (the unfused entry point).
@@ -1011,8 +1192,17 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS) k_move(StepArgs<R> a) {
     }
     put_part(a.partB, a.part_chunk, c, v, acc);
   }
-  flush_part(a.partB, a.part_chunk, S.unit, acc);
-  last_block_finalize(a, false, true);
+  flush_part(a.partB, a.part_chunk, acc);
+  if (a.final_inline) last_block_finalize(a, false, true);
+}
+
+// The end of a step as a launch of its own (one block), when the step
+// kernels leave it out (final_inline = 0).
+template <typename R>
+__global__ void __launch_bounds__(IMDG_STEP_THREADS)
+    k_finalize(StepArgs<R> a, int forces, int moved) {
+  if (a.ctl->frozen) return;
+  finalize_step(a, forces != 0, moved != 0);
 }
 
 // Pair-only systems: the force loop gathers rec_b, which holds a copy of the
@@ -1185,6 +1375,14 @@ hipError_t launch_move(const StepArgs<R>& a, int grid, hipStream_t st) {
 }
 
 template <typename R>
+hipError_t launch_finalize(const StepArgs<R>& a, bool forces, bool moved,
+                           hipStream_t st) {
+  hipLaunchKernelGGL((k_finalize<R>), dim3(1), dim3(IMDG_STEP_THREADS), 0, st,
+                     a, forces ? 1 : 0, moved ? 1 : 0);
+  return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_copyrec(const StepArgs<R>& a, int nslots, hipStream_t st) {
   hipLaunchKernelGGL((k_copyrec<R>), dim3((nslots + 255) / 256), dim3(256), 0,
                      st, a, nslots);
@@ -1203,7 +1401,9 @@ hipError_t launch_copyrec(const StepArgs<R>& a, int nslots, hipStream_t st) {
   template hipError_t launch_force<R>(const StepArgs<R>&, const TabDev<R>*,   \
                                       int, bool, int, hipStream_t);           \
   template hipError_t launch_move<R>(const StepArgs<R>&, int, hipStream_t);   \
-  template hipError_t launch_copyrec<R>(const StepArgs<R>&, int, hipStream_t);
+  template hipError_t launch_copyrec<R>(const StepArgs<R>&, int, hipStream_t); \
+  template hipError_t launch_finalize<R>(const StepArgs<R>&, bool, bool,     \
+                                         hipStream_t);
 
 // One precision per object when IMDG_STEP_FP32 or IMDG_STEP_FP64 is
 // defined (the build compiles this file once per type, in parallel).

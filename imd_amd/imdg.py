@@ -366,8 +366,21 @@ class Context:
                   "imdg_autotune")
         return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
-    def profile(self, enable=True):
-        self._chk(lib().imdg_profile(self._h, int(enable)), "imdg_profile")
+    KERNEL_CLASSES = ("rebuild", "density", "force", "move", "finalize")
+
+    def profile(self, enable=True, classes=None):
+        """Time launches with HIP events: every kernel class, or only the
+        named ones (KERNEL_CLASSES)."""
+        if not enable:
+            mask = 0
+        elif classes is None:
+            mask = 1
+        else:
+            mask = sum(1 << self.KERNEL_CLASSES.index(k) for k in classes)
+            if mask == 1:  # only "rebuild": 1 would mean every class
+                mask = 1 | (1 << 31)
+        self._chk(lib().imdg_profile(self._h, int(np.int32(np.uint32(mask)))),
+                  "imdg_profile")
 
     def kernel_times(self):
         n = 8

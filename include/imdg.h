@@ -189,7 +189,9 @@ int  imdg_get_grid(imdg_ctx *ctx, int rep, int *gdim3);
 int  imdg_get_info(imdg_ctx *ctx, imdg_info *info);
 
 /* Kernel timing with HIP events on the context's stream.  enable=1 starts
-   recording the duration of every launch of each kernel class; names and
+   recording the duration of every launch of each kernel class, any other
+   non-zero value only of the classes whose bit (1 << class index, in the
+   order imdg_kernel_times reports them) is set, 0 stops; names and
    accumulated milliseconds/launch counts come back from imdg_kernel_times
    (returns the number of classes, writes at most n). */
 int  imdg_profile(imdg_ctx *ctx, int enable);
@@ -214,6 +216,8 @@ typedef struct {
   double force_ms;        /* imdg_autotune: same for the force launch */
 } imdg_tuning;
 
+/* Fields are 0 until imdg_set_atoms has planned the launches (0 = the
+   default shape for the context's precision). */
 int  imdg_get_tuning(imdg_ctx *ctx, imdg_tuning *out);
 /* Takes effect for the next launch; IMDG_ERR_ARG (and no change) for a
    shape the loops cannot be launched with. */
