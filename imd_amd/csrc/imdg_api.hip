@@ -609,8 +609,9 @@ int enqueue_build(imdg_ctx* c) {
 }
 
 template <typename R>
-hipError_t enqueue_forces_t(imdg_ctx* c, bool integrate) {
+hipError_t enqueue_forces_t(imdg_ctx* c, bool integrate, bool store) {
   StepArgs<R> a = step_args<R>(c);
+  a.store = store ? 1 : 0;
   const TabDev<R>* t = (const TabDev<R>*)c->d_tabs;
   bool nt1 = c->cfg.ntypes == 1;
   hipError_t e;
@@ -658,11 +659,14 @@ hipError_t enqueue_move_t(imdg_ctx* c) {
   return e;
 }
 
-int enqueue_forces(imdg_ctx* c, bool integrate) {
+// store: write rho, epot and frc (always without integration: the move
+// launch reads frc).
+int enqueue_forces(imdg_ctx* c, bool integrate, bool store = true) {
+  store = store || !integrate;
   if (c->mixed)
-    HIPCHK(c, enqueue_forces_t<float>(c, integrate));
+    HIPCHK(c, enqueue_forces_t<float>(c, integrate, store));
   else
-    HIPCHK(c, enqueue_forces_t<double>(c, integrate));
+    HIPCHK(c, enqueue_forces_t<double>(c, integrate, store));
   HIPCHK(c, hipGetLastError());
   return IMDG_OK;
 }
@@ -1337,7 +1341,8 @@ int imdg_run(imdg_ctx* c, long nsteps) {
   } else {
     for (long s = 0; s < nsteps; s++) {
       if ((rc = enqueue_build(c))) return rc;
-      if ((rc = enqueue_forces(c, !c->split_move))) return rc;
+      // per-atom outputs only on the last step: nothing reads them between
+      if ((rc = enqueue_forces(c, !c->split_move, s == nsteps - 1))) return rc;
       if (c->split_move && (rc = enqueue_move(c))) return rc;
     }
   }

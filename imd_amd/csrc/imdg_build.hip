@@ -525,36 +525,69 @@ __device__ __forceinline__ int wrap_shift(int c, int o, int g) {
   return n < 0 ? -1 : (n >= g ? 1 : 0);
 }
 
+// The block's LDS tile of FP32 positions, addressed as LDS (32-bit
+// addresses; a generic pointer would be indexed in 64 bits).
+typedef __attribute__((address_space(3))) const float3 lds_float3;
+typedef __attribute__((address_space(3))) const char lds_char;
+typedef __attribute__((address_space(3))) const float lds_float;
+
+// Record k of the tile (k < IMDG_TILE_CAP, so the byte offset is a 24-bit
+// product).
+__device__ __forceinline__ float3 tile_rec(lds_float3* t, int k) {
+  unsigned off = __umul24((unsigned)k, (unsigned)sizeof(float3));
+  lds_float* p = (lds_float*)((lds_char*)t + off);
+  return make_float3(p[0], p[1], p[2]);
+}
+
 // Candidates of one neighbour cell, four at a time (their records in
-// flight together): FP32 screen, FP64 decision inside the band, append.
+// flight together): FP32 screen, FP64 decision inside the band (one branch
+// per four candidates; the band is rarely hit), append.  The separation is
+// u_j + (o - u_i), with o - u_i formed once per cell: two roundings of a
+// few Angstrom, far inside the screening band.
 template <bool MIXED, bool TOK>
 __device__ __forceinline__ void list_cell(
     const BuildArgs& a, const RepDev& R, const double4* __restrict__ pos,
-    const float3* trec, ListLane& L, int jb, int jn, int tb, float o0,
+    lds_float3* trec, ListLane& L, int jb, int jn, int tb, float o0,
     float o1, float o2, bool self, bool owner, uint32_t code) {
+  const float w0 = o0 - L.ui.x, w1 = o1 - L.ui.y, w2 = o2 - L.ui.z;
   for (int t0 = 0; t0 < jn; t0 += 4) {
     float3 uj[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       int t = min(t0 + u, jn - 1);
-      uj[u] = TOK ? trec[tb + t] : rec_xyz<MIXED>(a, jb + t);
+      if (TOK) {
+        uj[u] = tile_rec(trec, tb + t);
+      } else {
+        uj[u] = rec_xyz<MIXED>(a, jb + t);
+      }
     }
+    bool in[4], band[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       int j = jb + t0 + u;
       bool valid = (t0 + u < jn) && !(self && j == L.i);
-      float d0 = (uj[u].x - L.ui.x) + o0;
-      float d1 = (uj[u].y - L.ui.y) + o1;
-      float d2 = (uj[u].z - L.ui.z) + o2;
+      float d0 = uj[u].x + w0;
+      float d1 = uj[u].y + w1;
+      float d2 = uj[u].z + w2;
       float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-      bool in = valid && r2 < L.lo;
-      if (valid && !in && r2 < L.hi)
-        in = exact_pair(R, pos, L.i, j,
-                        wrap_shift(L.cx, (int)code / 9 - 1, R.gdim[0]),
-                        wrap_shift(L.cy, ((int)code / 3) % 3 - 1, R.gdim[1]),
-                        wrap_shift(L.cz, (int)code % 3 - 1, R.gdim[2]), owner);
-      if (in) list_append(L, (uint32_t)j | (code << IMDG_IDX_BITS));
+      in[u] = valid && r2 < L.lo;
+      band[u] = valid && !in[u] && r2 < L.hi;
     }
+    if (band[0] || band[1] || band[2] || band[3]) {
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (band[u])
+          in[u] = exact_pair(R, pos, L.i, jb + t0 + u,
+                             wrap_shift(L.cx, (int)code / 9 - 1, R.gdim[0]),
+                             wrap_shift(L.cy, ((int)code / 3) % 3 - 1,
+                                        R.gdim[1]),
+                             wrap_shift(L.cz, (int)code % 3 - 1, R.gdim[2]),
+                             owner);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (in[u])
+        list_append(L, (uint32_t)(jb + t0 + u) | (code << IMDG_IDX_BITS));
   }
 }
 
@@ -659,13 +692,13 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
           tb[k] = jn[k] > 0 ? tile_index(truns, nrun, jb[k]) : 0;
 #pragma unroll
         for (int k = 0; k < 27; k++)
-          list_cell<MIXED, true>(a, R, pos, trec, L, jb[k], jn[k], tb[k],
+          list_cell<MIXED, true>(a, R, pos, (lds_float3*)trec, L, jb[k], jn[k], tb[k],
                                  off[3 * k], off[3 * k + 1], off[3 * k + 2],
                                  k == 13, k >= 13, (uint32_t)k);
       } else {
 #pragma unroll
         for (int k = 0; k < 27; k++)
-          list_cell<MIXED, false>(a, R, pos, trec, L, jb[k], jn[k], 0,
+          list_cell<MIXED, false>(a, R, pos, (lds_float3*)trec, L, jb[k], jn[k], 0,
                                   off[3 * k], off[3 * k + 1], off[3 * k + 2],
                                   k == 13, k >= 13, (uint32_t)k);
       }

@@ -194,6 +194,8 @@ struct StepArgs {
   int part_chunk;         // partials per chunk (1) or per wavefront (0)
   int unitsA;             // wavefront partials written by the density loop
   int unitsB;             // ... by the finalising force / move launch
+  int store;              // write the per-atom outputs rho, epot, frc (the
+                          // fused steps of a run skip them but on the last)
   int final_inline;       // the last block of that launch ends the step
                           // (0: a separate one-block launch does)
   int kmax;

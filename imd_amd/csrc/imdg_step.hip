@@ -1005,8 +1005,10 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
       rb.z = ui.z;
       rb.w = dF;
       a.rec_b[i] = rb;
-      a.rho[i] = rho;
-      a.epot[i] = F;
+      if (a.store) {
+        a.rho[i] = rho;
+        a.epot[i] = F;
+      }
       emb = (double)F;
     }
     if (a.part_chunk) {
@@ -1152,15 +1154,17 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
     int i = c * IMDG_CHUNK + epi_slot(lane, L);
     double v[PB_N] = {0, 0, 0, 0, 0, 0};
     if (i - Rp.atom_off < Rp.natoms) {
-      // per-atom energy: half of every pair term plus F(rho_i)
-      R ei = (R)0.5 * kep + (HE ? a.epot[i] : (R)0);
-      a.epot[i] = ei;
-      R4 fv;
-      fv.x = kx;
-      fv.y = ky;
-      fv.z = kz;
-      fv.w = (R)0;
-      a.frc[i] = fv;
+      if (a.store) {
+        // per-atom energy: half of every pair term plus F(rho_i)
+        R ei = (R)0.5 * kep + (HE ? a.epot[i] : (R)0);
+        a.epot[i] = ei;
+        R4 fv;
+        fv.x = kx;
+        fv.y = ky;
+        fv.z = kz;
+        fv.w = (R)0;
+        a.frc[i] = fv;
+      }
       v[PB_EPAIR] = 0.5 * (double)kep;
       v[PB_VIRIAL] = -0.5 * (double)kvir;
       if (INTEGRATE)
