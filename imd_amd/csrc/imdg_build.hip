@@ -364,50 +364,79 @@ __global__ void __launch_bounds__(TILE_WAVES * IMDG_WAVE) k_tiles(BuildArgs a) {
     sre[w][pre + k] = me[k];
   }
   wave_lds_sync();
-  if (l != 0) return;
+  // sort the ranges by first slot (rank sort, one range per lane; ties by
+  // position), then merge overlapping or touching ranges into runs: a run
+  // starts where a range begins after the furthest end of all ranges before
+  // it (wave prefix max of the ends)
   int* rb = srb[w];
   int* re = sre[w];
-  // sort ranges by first slot, merge overlapping or touching ones
-  for (int k = 1; k < nr; k++) {
-    int vb = rb[k], ve = re[k], m = k - 1;
-    while (m >= 0 && rb[m] > vb) {
-      rb[m + 1] = rb[m];
-      re[m + 1] = re[m];
-      m--;
+  if (nr > IMDG_WAVE) ok = false;  // more ranges than lanes: no tile
+  int vb = 0, ve = 0, rank = 0;
+  bool mine = ok && l < nr;
+  if (mine) {
+    vb = rb[l];
+    ve = re[l];
+    for (int m = 0; m < nr; m++) {
+      int bm = rb[m];
+      rank += (bm < vb || (bm == vb && m < l)) ? 1 : 0;
     }
-    rb[m + 1] = vb;
-    re[m + 1] = ve;
   }
-  int nrun = 0, tot = 0;
+  wave_lds_sync();
+  if (mine) {
+    rb[rank] = vb;
+    re[rank] = ve;
+  }
+  wave_lds_sync();
+  int sb = 0, se = 0;
+  if (mine) {
+    sb = rb[l];
+    se = re[l];
+  }
+  // prefix max of the ends over the lanes before this one
+  int pmax = mine ? se : -1;
+#pragma unroll
+  for (int o = 1; o < IMDG_WAVE; o <<= 1) {
+    int y = __shfl_up(pmax, o, IMDG_WAVE);
+    if (l >= o) pmax = max(pmax, y);
+  }
+  int prev = __shfl_up(pmax, 1, IMDG_WAVE);
+  bool start = mine && (l == 0 || sb > prev);
+  unsigned long long starts = __ballot(start);
+  int nrun = __popcll(starts);
+  // run of this lane: number of starts at or below it, minus one
+  int runid = __popcll(starts & (~0ull >> (63 - l))) - 1;
+  bool last_of_run =
+      mine && (l == nr - 1 || (l < IMDG_WAVE - 1 && ((starts >> (l + 1)) & 1ull)));
+  if (nrun > IMDG_MAXRUN) ok = false;
+  // run begin (at its start lane) and end (the prefix max at its last lane)
   int4* run = srun[w];
-  for (int k = 0; k < nr && ok; k++) {
-    if (nrun > 0 && rb[k] <= run[nrun - 1].x + run[nrun - 1].y) {
-      int4 u = run[nrun - 1];
-      int end = max(u.x + u.y, re[k]);
-      tot += end - (u.x + u.y);
-      u.y = end - u.x;
-      run[nrun - 1] = u;
-    } else {
-      if (nrun >= IMDG_MAXRUN) {
-        ok = false;
-        break;
-      }
-      run[nrun] = make_int4(rb[k], re[k] - rb[k], tot, 0);
-      tot += re[k] - rb[k];
-      nrun++;
-    }
+  if (ok && start) run[runid].x = sb;
+  if (ok && last_of_run) run[runid].z = pmax;
+  wave_lds_sync();
+  // run lengths and their exclusive prefix sums (tile positions)
+  int len = 0;
+  if (ok && l < nrun) len = run[l].z - run[l].x;
+  int pos = len;
+#pragma unroll
+  for (int o = 1; o < IMDG_WAVE; o <<= 1) {
+    int y = __shfl_up(pos, o, IMDG_WAVE);
+    if (l >= o) pos += y;
   }
-  if (ok && tot <= IMDG_TILE_CAP)
-    for (int k = 0; k < nrun; k++) T.run[k] = run[k];
-  if (!ok || tot > IMDG_TILE_CAP) {
-    T.ok = 0;
-    T.nrun = 0;
-    T.natoms = 0;
-    atomicAdd(&a.ctl->tiles_fallback, 1);
-  } else {
-    T.ok = 1;
-    T.nrun = nrun;
-    T.natoms = tot;
+  int tot = __shfl(pos, IMDG_WAVE - 1, IMDG_WAVE);
+  pos -= len;
+  ok = ok && tot <= IMDG_TILE_CAP;
+  if (ok && l < nrun) T.run[l] = make_int4(run[l].x, len, pos, 0);
+  if (l == 0) {
+    if (!ok) {
+      T.ok = 0;
+      T.nrun = 0;
+      T.natoms = 0;
+      atomicAdd(&a.ctl->tiles_fallback, 1);
+    } else {
+      T.ok = 1;
+      T.nrun = nrun;
+      T.natoms = tot;
+    }
   }
 }
 

@@ -68,7 +68,8 @@
 #define DC_LIST 1
 #define DC_FINAL 2
 #define DC_SCAN 3
-#define DC_N 4
+#define DC_FINBLK 4   // finalising blocks of a step done (several replicas)
+#define DC_N 5
 
 #define IMDG_SCAN_TILE 1024  // cells per block of the cell-count scan
 

@@ -252,11 +252,14 @@ __device__ __forceinline__ void wave_combine(double (&s)[PB_N + 1],
 
 // Compensated sums of the partials of every replica (fixed order, so the
 // result does not depend on which block finishes last), thermostat, skin
-// decision, thermo record.  One replica: the whole block sums its
-// wavefront partials.  Several replicas: one wavefront per replica sums
-// that replica's chunk partials, with no block barrier per replica.
+// decision, thermo record.  One replica: the whole block sums the block
+// partials (fblk = 0, nfblk = 1).  Several replicas: one wavefront per
+// replica sums that replica's chunk partials, with no block barrier per
+// replica; the replicas are spread over the nfblk finalising blocks (this
+// is block fblk), which OR their rebuild decisions into the control block.
 template <typename R>
-__device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved) {
+__device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved,
+                              int fblk, int nfblk) {
   __shared__ double lred[IMDG_STEP_WAVES][PB_N + 1][2];
   __shared__ int any_s;
   int nw = blockDim.x / IMDG_WAVE;
@@ -313,7 +316,7 @@ __device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved) {
     }
   } else {
     // several replicas: one wavefront per replica, chunk partials
-    for (int r = w; r < a.nrep; r += nw) {
+    for (int r = fblk * nw + w; r < a.nrep; r += nfblk * nw) {
       const RepDev& Rp = a.rep[r];
       int c0 = Rp.blk_off * (IMDG_BLOCK / IMDG_CHUNK);
       int c1 = (Rp.blk_off + Rp.nblk) * (IMDG_BLOCK / IMDG_CHUNK);
@@ -343,9 +346,19 @@ __device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (moved) {
+    if (moved && nfblk == 1) {
       a.ctl->any_rebuild = any_s ? 1 : 0;
       a.ctl->steps_done += 1;
+    } else if (moved) {
+      // every block has read steps_done (its thermo records) before it
+      // counts itself done; the last one advances the step.  The flag is
+      // clear at this point (the last rebuild cleared it).
+      if (any_s) atomicOr(&a.ctl->any_rebuild, 1);
+      unsigned t = atomicAdd(&a.ctl->done[DC_FINBLK], 1u);
+      if (t == (unsigned)nfblk - 1) {
+        a.ctl->done[DC_FINBLK] = 0;
+        a.ctl->steps_done += 1;
+      }
     }
     __threadfence();
   }
@@ -365,7 +378,7 @@ __device__ __forceinline__ void last_block_finalize(StepArgs<R>& a,
   __syncthreads();
   if (is_last) {
     __threadfence();
-    finalize_step(a, forces, moved);
+    finalize_step(a, forces, moved, 0, 1);
   }
 }
 
@@ -1206,7 +1219,7 @@ template <typename R>
 __global__ void __launch_bounds__(IMDG_STEP_THREADS)
     k_finalize(StepArgs<R> a, int forces, int moved) {
   if (a.ctl->frozen) return;
-  finalize_step(a, forces != 0, moved != 0);
+  finalize_step(a, forces != 0, moved != 0, blockIdx.x, gridDim.x);
 }
 
 // Pair-only systems: the force loop gathers rec_b, which holds a copy of the
@@ -1381,8 +1394,10 @@ hipError_t launch_move(const StepArgs<R>& a, int grid, hipStream_t st) {
 template <typename R>
 hipError_t launch_finalize(const StepArgs<R>& a, bool forces, bool moved,
                            hipStream_t st) {
-  hipLaunchKernelGGL((k_finalize<R>), dim3(1), dim3(IMDG_STEP_THREADS), 0, st,
-                     a, forces ? 1 : 0, moved ? 1 : 0);
+  // one block per IMDG_STEP_WAVES replicas (one wavefront each)
+  int nblk = a.part_chunk ? (a.nrep + IMDG_STEP_WAVES - 1) / IMDG_STEP_WAVES : 1;
+  hipLaunchKernelGGL((k_finalize<R>), dim3(nblk), dim3(IMDG_STEP_THREADS), 0,
+                     st, a, forces ? 1 : 0, moved ? 1 : 0);
   return hipGetLastError();
 }
 
