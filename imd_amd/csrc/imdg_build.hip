@@ -224,12 +224,31 @@ __global__ void __launch_bounds__(256) k_cellsort(BuildArgs a) {
   }
 }
 
+// Tiles: for every block of a rebuilt replica, the slot runs of all cells
+// adjacent (27-stencil, periodic wrap, nothing beyond an open face) to the
+// cells of the block's atoms, sorted and merged.  One wavefront per block
+// (the first of the block's permutation launch): lane l handles neighbour
+// column l % 9 of the block's column l / 9 (at most 7 columns; more falls
+// back to a tile-less list), producing at most two slot ranges (a z range
+// split by the periodic wrap); the wavefront sorts and merges them.
+#define TILE_MAXRANGE 126
+#define TILE_MAXCOL 7
+
+// LDS rows of one wavefront's tile construction.
+struct TileLds {
+  int rb[TILE_MAXRANGE], re[TILE_MAXRANGE];
+  int4 run[IMDG_MAXRUN];
+};
+
+__device__ void build_tile(const BuildArgs& a, int b, TileLds& S);
+
 // Gather every slot into the other buffers (new cell-sorted order for the
 // rebuilt replicas, identity for the others) and write the cell-relative
 // records of the moved atoms.  The buffers are flipped by k_list_done once
 // the list fits (on an overflow the whole chain runs again from the old
 // buffers).
 __global__ void __launch_bounds__(IMDG_BLOCK) k_permute(BuildArgs a) {
+  __shared__ TileLds tl;
   if (!a.ctl->any_rebuild || a.ctl->frozen) return;
   int pc = a.ctl->pos_cur, rc = a.ctl->rec_cur;
   const double4* pin = a.pos[pc];
@@ -262,28 +281,17 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_permute(BuildArgs a) {
       reinterpret_cast<double4*>(a.rec_a)[i] =
           make_double4(p.x - c[0], p.y - c[1], p.z - c[2], p.w);
   }
+  // the block's tile for the list build (needs the cell ranges only)
+  if (a.rst[r].need_rebuild && threadIdx.x < IMDG_WAVE)
+    build_tile(a, blockIdx.x, tl);
 }
 
-// Tiles: for every block of a rebuilt replica, the slot runs of all cells
-// adjacent (27-stencil, periodic wrap, nothing beyond an open face) to the
-// cells of the block's atoms, sorted and merged.  One wavefront per block:
-// lane l handles neighbour column l % 9 of the block's column l / 9 (at
-// most 7 columns; more falls back to a tile-less list), producing at most
-// two slot ranges (a z range split by the periodic wrap); lane 0 sorts and
-// merges them.
-#define TILE_MAXRANGE 126
-#define TILE_MAXCOL 7
-#define TILE_WAVES 4
-
-__global__ void __launch_bounds__(TILE_WAVES * IMDG_WAVE) k_tiles(BuildArgs a) {
-  __shared__ int srb[TILE_WAVES][TILE_MAXRANGE], sre[TILE_WAVES][TILE_MAXRANGE];
-  __shared__ int4 srun[TILE_WAVES][IMDG_MAXRUN];
-  if (!a.ctl->any_rebuild || a.ctl->frozen) return;
-  int w = threadIdx.x / IMDG_WAVE, l = threadIdx.x % IMDG_WAVE;
-  int b = blockIdx.x * TILE_WAVES + w;
-  if (b >= a.nblk) return;  // whole wavefront; no block barrier below
+// The tile of 128-slot block b of a rebuilt replica, built by one whole
+// wavefront (lane l); the cells of the block's first and last slots are
+// those the permutation gives them.
+__device__ void build_tile(const BuildArgs& a, int b, TileLds& S) {
+  int l = threadIdx.x % IMDG_WAVE;
   int r = __builtin_amdgcn_readfirstlane(a.blk_rep[b]);
-  if (!a.rst[r].need_rebuild) return;
   const RepDev& R = a.rep[r];
   Tile& T = a.tiles[b];
   int s0 = b * IMDG_BLOCK;
@@ -297,7 +305,8 @@ __global__ void __launch_bounds__(TILE_WAVES * IMDG_WAVE) k_tiles(BuildArgs a) {
     return;
   }
   int g0 = R.gdim[0], g1 = R.gdim[1], g2 = R.gdim[2];
-  int lc0 = a.acell[s0] - R.cell_off, lc1 = a.acell[s1] - R.cell_off;
+  int lc0 = a.cellid[a.perm[s0]] - R.cell_off;
+  int lc1 = a.cellid[a.perm[s1]] - R.cell_off;
   int col0 = lc0 / g2, col1 = lc1 / g2, ncol = col1 - col0 + 1;
   bool ok = ncol <= TILE_MAXCOL;
   int nmine = 0, mb[2] = {0, 0}, me[2] = {0, 0};
@@ -360,16 +369,16 @@ __global__ void __launch_bounds__(TILE_WAVES * IMDG_WAVE) k_tiles(BuildArgs a) {
   int nr = __shfl(pre, IMDG_WAVE - 1, IMDG_WAVE);
   pre -= nmine;
   for (int k = 0; k < nmine; k++) {
-    srb[w][pre + k] = mb[k];
-    sre[w][pre + k] = me[k];
+    S.rb[pre + k] = mb[k];
+    S.re[pre + k] = me[k];
   }
   wave_lds_sync();
   // sort the ranges by first slot (rank sort, one range per lane; ties by
   // position), then merge overlapping or touching ranges into runs: a run
   // starts where a range begins after the furthest end of all ranges before
   // it (wave prefix max of the ends)
-  int* rb = srb[w];
-  int* re = sre[w];
+  int* rb = S.rb;
+  int* re = S.re;
   if (nr > IMDG_WAVE) ok = false;  // more ranges than lanes: no tile
   int vb = 0, ve = 0, rank = 0;
   bool mine = ok && l < nr;
@@ -409,7 +418,7 @@ __global__ void __launch_bounds__(TILE_WAVES * IMDG_WAVE) k_tiles(BuildArgs a) {
       mine && (l == nr - 1 || (l < IMDG_WAVE - 1 && ((starts >> (l + 1)) & 1ull)));
   if (nrun > IMDG_MAXRUN) ok = false;
   // run begin (at its start lane) and end (the prefix max at its last lane)
-  int4* run = srun[w];
+  int4* run = S.run;
   if (ok && start) run[runid].x = sb;
   if (ok && last_of_run) run[runid].z = pmax;
   wave_lds_sync();
@@ -747,7 +756,8 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
       seen = cnt;
     }
   }
-  // block max of the list length (k_list_done reduces the blocks)
+  // block max of the list length (k_list_done reduces the blocks: one
+  // atomic word updated by every block would serialise them)
   for (int o = 32; o > 0; o >>= 1)
     seen = max(seen, __shfl_xor(seen, o, IMDG_WAVE));
   if ((threadIdx.x % IMDG_WAVE) == 0) wmax[threadIdx.x / IMDG_WAVE] = seen;
@@ -827,9 +837,6 @@ void launch_build(const BuildArgs& b, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_place, dim3(b.nblk), dim3(IMDG_BLOCK), 0, st, b);
   hipLaunchKernelGGL(k_cellsort, dim3(grid), dim3(256), 0, st, b);
   hipLaunchKernelGGL(k_permute, dim3(b.nblk), dim3(IMDG_BLOCK), 0, st, b);
-  int gt = (b.nblk + TILE_WAVES - 1) / TILE_WAVES;
-  hipLaunchKernelGGL(k_tiles, dim3(gt < 1 ? 1 : gt), dim3(TILE_WAVES * IMDG_WAVE),
-                     0, st, b);
   if (b.mixed)
     hipLaunchKernelGGL((k_list<true>), dim3(b.nblk), dim3(IMDG_BLOCK), 0, st,
                        b);
