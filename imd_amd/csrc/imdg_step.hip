@@ -295,23 +295,28 @@ __device__ void finalize_step(StepArgs<R>& a, bool forces, bool moved,
       }
     }
     __syncthreads();
+    // the wavefront results in wavefront order, one quantity per thread
+    __shared__ double ltot[PB_N + 1];
+    int q = threadIdx.x;
+    if (q <= PB_N) {
+      double ss = lred[0][q][0], cc = lred[0][q][1];
+      for (int k = 1; k < nw; k++) {
+        if (q == PB_MAXD) {
+          ss = fmax(ss, lred[k][q][0]);
+        } else {
+          double t, e;
+          two_sum(ss, lred[k][q][0], t, e);
+          ss = t;
+          cc = cc + lred[k][q][1] + e;
+        }
+      }
+      ltot[q] = (q == PB_MAXD) ? ss : ss + cc;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
       double tot[PB_N + 1];
 #pragma unroll
-      for (int q = 0; q <= PB_N; q++) {
-        double ss = lred[0][q][0], cc = lred[0][q][1];
-        for (int k = 1; k < nw; k++) {
-          if (q == PB_MAXD) {
-            ss = fmax(ss, lred[k][q][0]);
-          } else {
-            double t, e;
-            two_sum(ss, lred[k][q][0], t, e);
-            ss = t;
-            cc = cc + lred[k][q][1] + e;
-          }
-        }
-        tot[q] = (q == PB_MAXD) ? ss : ss + cc;
-      }
+      for (int k = 0; k <= PB_N; k++) tot[k] = ltot[k];
       if (replica_step_end(a, 0, tot, forces, moved)) any_s = 1;
     }
   } else {

@@ -54,6 +54,7 @@ int imd_gpu_init(void) {
   cfg.nbl_margin = nbl_margin;
   cfg.nbl_size = nbl_size;
   cfg.device = gpu_device;
+  cfg.lanes_per_atom = gpu_lanes;
   check(imdg_ctx_create(&cfg, &ctx), "imdg_ctx_create");
   imdg_table tp, tr, te;
   if (have_potfile) tp = as_imdg(&pair_pot);
@@ -76,6 +77,14 @@ int imd_gpu_init(void) {
   int n = (int)natoms;
   check(imdg_set_atoms(ctx, 1, &rep, &n, atom_x, atom_p, atom_type, atom_mass),
         "imdg_set_atoms");
+  if (gpu_autotune) {
+    /* builds the list, times the launch shapes, leaves the state as is */
+    imdg_tuning t;
+    check(imdg_autotune(ctx, 3, &t), "imdg_autotune");
+    printf("gpu_autotune: density %d lanes x %d threads, force %d lanes x %d "
+           "threads\n", t.density_lanes, t.density_threads, t.force_lanes,
+           t.force_threads);
+  }
   move_atoms = (ensemble == IMDG_NVT) ? move_atoms_nvt : move_atoms_nve;
   return 0;
 }

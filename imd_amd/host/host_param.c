@@ -36,6 +36,7 @@ char infilename[IMD_STRLEN], outfilename[IMD_STRLEN] = "imd";
 char potfilename[IMD_STRLEN], eam2_emb_E_filename[IMD_STRLEN];
 char eam2_at_rho_filename[IMD_STRLEN];
 int gpu_precision = IMDG_MIXED, gpu_device = 0;
+int gpu_lanes = 0, gpu_autotune = 0;
 int have_potfile = 0, have_eam = 0;
 
 void imd_error(const char *fmt, ...) {
@@ -82,6 +83,8 @@ void init_parameters(void) {
   strcpy(outfilename, "imd");
   gpu_precision = IMDG_MIXED;
   gpu_device = 0;
+  gpu_lanes = 0;
+  gpu_autotune = 0;
   have_potfile = have_eam = 0;
 }
 
@@ -228,6 +231,14 @@ int read_parameters(const char *paramfile, int phase) {
         imd_error("gpu_precision must be mixed or fp64");
     } else if (!strcasecmp(key, "gpu_device")) {
       get_ints(key, &gpu_device, 1);
+    } else if (!strcasecmp(key, "gpu_lanes")) {
+      /* lanes per atom of the neighbour loops (1, 2 or 4; 0: default) */
+      get_ints(key, &gpu_lanes, 1);
+      if (gpu_lanes != 0 && gpu_lanes != 1 && gpu_lanes != 2 && gpu_lanes != 4)
+        imd_error("gpu_lanes must be 0, 1, 2 or 4");
+    } else if (!strcasecmp(key, "gpu_autotune")) {
+      /* time every launch shape once before the main loop (imdg_autotune) */
+      get_ints(key, &gpu_autotune, 1);
     } else if (!strcasecmp(key, "watch_int") || !strcasecmp(key, "stop_int") ||
                !strcasecmp(key, "binary_output") ||
                !strcasecmp(key, "debug_potential")) {

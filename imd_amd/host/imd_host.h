@@ -53,7 +53,7 @@ extern double maxwalltime;
 extern char infilename[IMD_STRLEN], outfilename[IMD_STRLEN];
 extern char potfilename[IMD_STRLEN], eam2_emb_E_filename[IMD_STRLEN];
 extern char eam2_at_rho_filename[IMD_STRLEN];
-extern int gpu_precision, gpu_device;
+extern int gpu_precision, gpu_device, gpu_lanes, gpu_autotune;
 extern int have_potfile, have_eam;
 
 extern pot_table_t pair_pot, embed_pot, rho_h_tab;

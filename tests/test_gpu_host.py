@@ -24,8 +24,12 @@ def _read_eng(path):
     return np.array([[float(v) for v in r] for r in rows])
 
 
-@pytest.mark.parametrize("ensemble", ["nve", "nvt"])
-def test_imd_gpu_eng_matches_oracle(tmp_path, ensemble):
+@pytest.mark.parametrize("ensemble,extra", [
+    ("nve", []), ("nvt", []),
+    # launch shape from the parameter file and the autotuner before the
+    # main loop: same trajectory
+    ("nve", ["gpu_lanes 2", "gpu_autotune 1"])])
+def test_imd_gpu_eng_matches_oracle(tmp_path, ensemble, extra):
     if not os.path.exists(EXE):
         pytest.fail("imd_gpu is not built (build() first)")
     eam = eam_tables()
@@ -57,11 +61,14 @@ def test_imd_gpu_eng_matches_oracle(tmp_path, ensemble):
     ]
     if ensemble == "nvt":
         lines += ["starttemp %.17g" % kT, "tau_eta %.17g" % tau]
+    lines += extra
     param = tmp_path / "run.param"
     param.write_text("\n".join(lines) + "\n")
     r = subprocess.run([EXE, "-p", str(param)], capture_output=True,
                        text=True, timeout=120, cwd=str(tmp_path))
     assert r.returncode == 0, r.stderr[-2000:]
+    if "gpu_autotune 1" in extra:
+        assert "gpu_autotune: density" in r.stdout
     eng = _read_eng(str(tmp_path / "run.eng"))
     assert eng.shape[0] == nsteps // eng_int + 1
 

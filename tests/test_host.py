@@ -102,6 +102,8 @@ core_potential_file a.pt
 embedding_energy_file b.pt
 atomic_e-density_file c.pt
 gpu_precision fp64
+gpu_lanes 2
+gpu_autotune 1
 simulation 2
 maxsteps 999
 """)
@@ -113,6 +115,7 @@ maxsteps 999
     assert g(C.c_int, "eng_int") == 10
     assert abs(g(C.c_double, "isq_tau_eta") - 1 / 9.8227 ** 2) < 1e-15
     assert g(C.c_int, "gpu_precision") == 0
+    assert g(C.c_int, "gpu_lanes") == 2 and g(C.c_int, "gpu_autotune") == 1
     masses = (C.c_double * 8).in_dll(host, "masses")
     assert masses[0] == 63.546 and masses[1] == 58.69
     assert list((C.c_int * 3).in_dll(host, "pbc_dirs")) == [1, 1, 0]
