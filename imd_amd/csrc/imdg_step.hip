@@ -486,10 +486,19 @@ __device__ __forceinline__ void integrate_slot(StepArgs<R>& a,
 // reads for a slot are L consecutive list entries -- mostly neighbouring
 // slots of one cell, one or two cache lines instead of L.  Pass p ends with
 // a sum over the slot's L lanes, and lane p of the slot keeps it: after the
-// L passes lane l holds the sums of chunk slot epi_slot(l), the slot whose
-// per-atom work (embedding, integration, writes) it then does.
+// L passes lane l holds the sums of chunk slot epi_slot(l).  The per-atom
+// work (embedding, integration, writes) then runs with lane = slot, so its
+// loads and stores are contiguous: lane l takes the sums of slot l from
+// lane keep_lane(l) (a cross-lane shuffle per sum).
 __device__ __forceinline__ int epi_slot(int lane, int L) {
   return (IMDG_WAVE / L) * (lane % L) + lane / L;
+}
+__device__ __forceinline__ int keep_lane(int slot, int L) {
+  return (slot % (IMDG_WAVE / L)) * L + slot / (IMDG_WAVE / L);
+}
+template <int L, typename R>
+__device__ __forceinline__ R to_slot_lane(R v, int lane) {
+  return L == 1 ? v : __shfl(v, keep_lane(lane, L), IMDG_WAVE);
 }
 
 // Sum over the L lanes of a slot (aligned lane groups); every lane of the
@@ -1002,7 +1011,8 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
       rho = lane_group_sum<L>(rho);
       if (gs == p) rho_keep = rho;
     }
-    int i = c * IMDG_CHUNK + epi_slot(lane, L);
+    rho_keep = to_slot_lane<L>(rho_keep, lane);
+    int i = c * IMDG_CHUNK + lane;
     double emb = 0.0;
     if (i - Rp.atom_off < Rp.natoms) {
       R4 ui = rec[i];
@@ -1169,7 +1179,12 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
       kz *= p2is;
       kvir *= p2is;
     }
-    int i = c * IMDG_CHUNK + epi_slot(lane, L);
+    kx = to_slot_lane<L>(kx, lane);
+    ky = to_slot_lane<L>(ky, lane);
+    kz = to_slot_lane<L>(kz, lane);
+    kep = to_slot_lane<L>(kep, lane);
+    kvir = to_slot_lane<L>(kvir, lane);
+    int i = c * IMDG_CHUNK + lane;
     double v[PB_N] = {0, 0, 0, 0, 0, 0};
     if (i - Rp.atom_off < Rp.natoms) {
       if (a.store) {
