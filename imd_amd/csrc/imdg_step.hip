@@ -542,6 +542,12 @@ __device__ __forceinline__ const uint32_t* list_words(const uint32_t* nbr,
          s * (IMDG_GROUP / L);
 }
 
+// Marks the four list words as needed here: the memory-counter wait for
+// the load that fills them is placed at this point (an empty statement).
+__device__ __forceinline__ void ready(const uint4& e) {
+  asm volatile("" ::"v"(e.x), "v"(e.y), "v"(e.z), "v"(e.w));
+}
+
 // Walk a slot's list, one body (four entries per lane) at a time, in pairs
 // A = 2h, B = 2h+1.  A and B sit in two registers of their own; each body
 // issues its record gathers, then (through pf/slot) the load of the body two
@@ -566,10 +572,16 @@ __device__ __forceinline__ void list_walk(const uint32_t* w, int nnb, F body) {
                  nbody > 0 ? A.z : 0u, nbody > 0 ? A.w : 0u);
   B = make_uint4(nbody > 0 ? B.x : 0u, nbody > 0 ? B.y : 0u,
                  nbody > 0 ? B.z : 0u, nbody > 0 ? B.w : 0u);
+  ready(A);  // both ways into the loop head then have A's words in hand
   for (int h = 0; 2 * h < nbody; h++) {
     body(A, ListRef<L>{w + (size_t)min(2 * h + 2, blast) * BW}, A);
-    if (L <= 2 || 2 * h + 1 < nbody)
+    if (L <= 2 || 2 * h + 1 < nbody) {
       body(B, ListRef<L>{w + (size_t)min(2 * h + 3, blast) * BW}, B);
+      // take the wait for the next A here, where only B's prefetch is newer
+      // than it (at the loop head the wait counter is merged over the loop
+      // entry and would also wait for B's prefetch)
+      ready(A);
+    }
   }
 }
 
