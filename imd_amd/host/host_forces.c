@@ -8,8 +8,13 @@
  * effect on the globals (tot_pot_energy, virial, tot_kin_energy, eta,
  * nbl_count); the work runs in libimdg through include/imdg.h.
  * main_loop runs IMD's step sequence (src/imd_main_3d.c:155-870:
- * calc_forces, move_atoms, write_eng_file every eng_int steps,
- * check_nblist) with the steps between two outputs fused on the device.
+ * calc_forces, move_atoms, then per step the checkpoint every checkpt_int
+ * steps, the .eng row every eng_int steps, the "write" file test every
+ * watch_int steps, the "stop" file test every stop_int steps and the
+ * maxwalltime test (src/imd_main_3d.c:690-815, 1892-1965), check_nblist)
+ * with the steps between two such output steps fused on the device; the
+ * wall time is tested at the end of each fused run (at most 2048 steps).
+ * A phase ends by writing <outfiles>-final.itr (src/imd_main_3d.c:877-881).
  */
 #define _POSIX_C_SOURCE 199309L
 #include <stdio.h>
@@ -20,6 +25,7 @@
 #include "imd_host.h"
 
 static imdg_ctx *ctx = NULL;
+static double t_start = -1.0;
 
 void (*move_atoms)(void) = NULL;
 
@@ -44,7 +50,10 @@ static imdg_table as_imdg(const pot_table_t *pt) {
   return t;
 }
 
+static double wall(void);
+
 int imd_gpu_init(void) {
+  if (t_start < 0) t_start = wall();
   imdg_config cfg;
   memset(&cfg, 0, sizeof cfg);
   cfg.precision = gpu_precision;
@@ -94,6 +103,17 @@ void imd_gpu_finish(void) {
   ctx = NULL;
 }
 
+void imd_gpu_reconfigure(void) {
+  /* a new phase may change the ensemble, time step, thermostat, list
+     parameters or box: carry the atoms over into a new context */
+  fetch_atoms();
+  if (do_maxwell) maxwell(temperature);
+  do_maxwell = 0;
+  make_box();
+  imd_gpu_finish();
+  imd_gpu_init();
+}
+
 static void pull_thermo(void) {
   imdg_thermo th;
   check(imdg_get_thermo(ctx, 0, &th), "imdg_get_thermo");
@@ -132,7 +152,8 @@ void move_atoms_nvt(void) {
 }
 
 void fetch_atoms(void) {
-  check(imdg_get_atoms(ctx, atom_x, atom_p, NULL, NULL, NULL, NULL),
+  check(imdg_get_atoms(ctx, atom_x, atom_p, NULL, atom_epot,
+                       have_eam ? atom_rho : NULL, NULL),
         "imdg_get_atoms");
 }
 
@@ -142,27 +163,83 @@ static double wall(void) {
   return t.tv_sec + 1e-9 * t.tv_nsec;
 }
 
+/* empty file "write": write <outfiles>-interm.chkpt (src/imd_main_3d.c:1892-1912) */
+static void check_write(int steps) {
+  FILE *t = fopen("write", "r");
+  if (!t) return;
+  int empty = fgetc(t) == EOF;
+  fclose(t);
+  if (!empty) return;
+  remove("write");
+  printf("Write file found after %d steps\n", steps);
+  fetch_atoms();
+  write_config(-2, steps);
+}
+
+/* file "stop": write <outfiles>-final.chkpt and end (src/imd_main_3d.c:1918-1940) */
+static int check_stop(int steps) {
+  FILE *t = fopen("stop", "r");
+  if (!t) return 0;
+  fclose(t);
+  unlink("stop");
+  printf("Stop file found after %d steps\n", steps);
+  fetch_atoms();
+  write_config(-1, steps);
+  steps_max = steps;
+  return 1;
+}
+
+/* src/imd_main_3d.c:1946-1965 */
+static int check_walltime(int steps) {
+  if (wall() - t_start <= maxwalltime) return 0;
+  printf("Maximal allowed walltime reached after %d steps\n", steps);
+  fetch_atoms();
+  write_config(-1, steps);
+  steps_max = steps;
+  return 1;
+}
+
+/* the outputs and tests at the end of step `steps`, in IMD's order, with
+   tot_pot_energy, tot_kin_energy, virial and eta of that step; returns 1
+   when the run is to end */
+static int step_outputs(int steps) {
+  if (checkpt_int > 0 && steps % checkpt_int == 0) {
+    fetch_atoms();
+    write_config(steps / checkpt_int, steps);
+  }
+  if (eng_int > 0 && steps % eng_int == 0) write_eng_file(steps);
+  if (watch_int > 0 && steps % watch_int == 0) check_write(steps);
+  if (stop_int > 0 && steps % stop_int == 0 && check_stop(steps)) return 1;
+  if (maxwalltime > 0 && check_walltime(steps)) return 1;
+  return 0;
+}
+
+static int next_multiple(int steps, int every, int last) {
+  if (every <= 0) return last;
+  int m = ((steps + every - 1) / every) * every;
+  return m < last ? m : last;
+}
+
 int main_loop(void) {
   const char *unfused = getenv("IMD_GPU_UNFUSED");
-  long total = (long)steps_max - steps_min + 1;
+  int finished = 0;
+  long done_steps = 0;
   double t0 = wall();
-  if (total <= 0) return 0;
   if (unfused && atoi(unfused)) {
     /* IMD's own call sequence, one entry point at a time */
     for (int steps = steps_min; steps <= steps_max; steps++) {
       calc_forces(steps);
       move_atoms();
-      if (eng_int > 0 && steps % eng_int == 0) write_eng_file(steps);
-      if (checkpt_int > 0 && steps % checkpt_int == 0) {
-        char fn[IMD_STRLEN + 32];
-        fetch_atoms();
-        snprintf(fn, sizeof fn, "%s.%d.chkpt", outfilename, steps / checkpt_int);
-        write_config(fn);
+      done_steps++;
+      if (step_outputs(steps)) {
+        finished = 1;
+        break;
       }
       check_nblist();
     }
   } else {
     imdg_thermo *hist = malloc(sizeof(imdg_thermo) * 2048);
+    if (!hist) imd_error("cannot allocate the thermo buffer");
     int steps = steps_min;
     long done = 0;
     {
@@ -173,45 +250,32 @@ int main_loop(void) {
     while (steps <= steps_max) {
       /* run through the next output step, at most 2048 steps at a time */
       int last = steps_max;
-      if (eng_int > 0) {
-        int ne = ((steps + eng_int - 1) / eng_int) * eng_int;
-        if (ne < last) last = ne;
-      }
-      if (checkpt_int > 0) {
-        int nc = ((steps + checkpt_int - 1) / checkpt_int) * checkpt_int;
-        if (nc < last) last = nc;
-      }
+      last = next_multiple(steps, eng_int, last);
+      last = next_multiple(steps, checkpt_int, last);
+      last = next_multiple(steps, watch_int, last);
+      last = next_multiple(steps, stop_int, last);
       if (last - steps + 1 > 2048) last = steps + 2047;
       long n = last - steps + 1;
       check(imdg_run(ctx, n), "imdg_run");
       check(imdg_sync(ctx), "imdg_sync");
       check(imdg_get_thermo_history(ctx, done, n, hist), "thermo history");
       for (long k = 0; k < n; k++) {
+        tot_pot_energy = hist[k].epot;
+        tot_kin_energy = hist[k].ekin;
+        virial = hist[k].virial;
+        eta = hist[k].eta;
         int s = steps + (int)k;
-        if (eng_int > 0 && s % eng_int == 0) {
-          tot_pot_energy = hist[k].epot;
-          tot_kin_energy = hist[k].ekin;
-          virial = hist[k].virial;
-          eta = hist[k].eta;
-          write_eng_file(s);
+        /* only the eng interval can fall inside a fused run */
+        if (k < n - 1) {
+          if (eng_int > 0 && s % eng_int == 0) write_eng_file(s);
+        } else if (step_outputs(s)) {
+          finished = 1;
         }
       }
-      tot_pot_energy = hist[n - 1].epot;
-      tot_kin_energy = hist[n - 1].ekin;
-      virial = hist[n - 1].virial;
-      eta = hist[n - 1].eta;
-      if (checkpt_int > 0 && last % checkpt_int == 0) {
-        char fn[IMD_STRLEN + 32];
-        fetch_atoms();
-        snprintf(fn, sizeof fn, "%s.%d.chkpt", outfilename, last / checkpt_int);
-        write_config(fn);
-      }
       done += n;
+      done_steps += n;
       steps = last + 1;
-      if (maxwalltime > 0 && wall() - t0 > maxwalltime) {
-        imd_warning("maxwalltime reached after step %d", last);
-        break;
-      }
+      if (finished) break;
     }
     free(hist);
   }
@@ -220,6 +284,12 @@ int main_loop(void) {
   check(imdg_get_info(ctx, &info), "imdg_get_info");
   nbl_count = (int)info.nbl_count;
   printf("Did %ld steps in %.3f s: %.4e atom-steps/s, %ld list builds\n",
-         total, el, natoms * (double)total / el, (long)nbl_count);
-  return 0;
+         done_steps, el, el > 0 ? natoms * (double)done_steps / el : 0.0,
+         (long)nbl_count);
+  /* the phase is done: the restart flag is cleared and the state is kept
+     for the next phase */
+  imdrestart = 0;
+  write_itr_file(-1, steps_max, "");
+  printf("End of simulation %d\n", simulation);
+  return finished;
 }

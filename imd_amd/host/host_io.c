@@ -15,19 +15,33 @@
  *   in input order.
  * write_eng_file: one line per eng_int steps with time, Epot per atom,
  *   temperature 2 Ekin / nactive, pressure T/(V/N) + virial/(3V), volume per
- *   atom and, for NVT, eta * tau_eta (src/imd_io.c:2262-2431).
+ *   atom and, for NVT, eta * tau_eta (src/imd_io.c:2262-2431); the header
+ *   is written when the file is started, rows are appended.
+ * write_config(n, steps): the checkpoint <outfiles>.%05d.chkpt (n >= 0),
+ *   <outfiles>-final.chkpt (n == -1) or <outfiles>-interm.chkpt (n == -2)
+ *   with IMD's header (#F A 1 1 1 3 3 nd, #C, #X/#Y/#Z, ##PBC, ## lines,
+ *   #E; src/imd_io.c:3152-3300) and rows "number type mass x y z vx vy vz
+ *   Epot [eam_rho]" (src/imd_io_3d.c:696-948), then the iteration file
+ *   (src/imd_io.c:171-181).  Numbers are written with 17 significant
+ *   digits, so a restart reads back the same doubles.
+ * write_itr_file(n, steps, ""): <outfiles>.%05d.itr, -final.itr or
+ *   -interm.itr with "# checkpoint n", startstep = steps + 1, the box
+ *   vectors and, for NVT with tau_eta, starttemp and eta
+ *   (src/imd_io_3d.c:949-1095); it is a parameter file.
  */
 #define _XOPEN_SOURCE 600
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "imd_host.h"
 
 long natoms = 0;
 int nactive = 0;
 double *atom_x = NULL, *atom_p = NULL, *atom_mass = NULL;
+double *atom_epot = NULL, *atom_rho = NULL;
 int *atom_type = NULL;
 long *atom_number = NULL;
 double tot_pot_energy = 0.0, tot_kin_energy = 0.0, virial = 0.0, volume = 0.0;
@@ -41,14 +55,18 @@ static int grow_atoms(long cap) {
   atom_mass = realloc(atom_mass, sizeof(double) * cap);
   atom_type = realloc(atom_type, sizeof(int) * cap);
   atom_number = realloc(atom_number, sizeof(long) * cap);
-  return (atom_x && atom_p && atom_mass && atom_type && atom_number) ? 0 : 1;
+  atom_epot = realloc(atom_epot, sizeof(double) * cap);
+  atom_rho = realloc(atom_rho, sizeof(double) * cap);
+  return (atom_x && atom_p && atom_mass && atom_type && atom_number &&
+          atom_epot && atom_rho) ? 0 : 1;
 }
 
-static double box_volume(void) {
+/* the volume of make_box (src/imd_geom_3d.c:52-104) */
+void make_box(void) {
   double c[3] = {box_y[1] * box_z[2] - box_y[2] * box_z[1],
                  box_y[2] * box_z[0] - box_y[0] * box_z[2],
                  box_y[0] * box_z[1] - box_y[1] * box_z[0]};
-  return fabs(box_x[0] * c[0] + box_x[1] * c[1] + box_x[2] * c[2]);
+  volume = fabs(box_x[0] * c[0] + box_x[1] * c[1] + box_x[2] * c[2]);
 }
 
 int read_atoms(const char *filename) {
@@ -137,7 +155,7 @@ int read_atoms(const char *filename) {
   fclose(f);
   if (missing_vel) do_maxwell = 1;
   nactive = (int)(3 * natoms);
-  volume = box_volume();
+  make_box();
   return 0;
 }
 
@@ -168,7 +186,7 @@ int generate_fcc(void) {
         natoms++;
       }
   nactive = (int)(3 * natoms);
-  volume = box_volume();
+  make_box();
   do_maxwell = 1;
   return 0;
 }
@@ -199,37 +217,85 @@ void maxwell(double temp) {
     for (int c = 0; c < 3; c++) atom_p[3 * i + c] -= tot[c];
 }
 
-int write_config(const char *filename) {
-  FILE *f = fopen(filename, "w");
-  if (!f) return 1;
-  fprintf(f, "#F A 1 1 1 3 3 0\n#C number type mass x y z vx vy vz\n");
-  fprintf(f, "#X %.16e %.16e %.16e\n", box_x[0], box_x[1], box_x[2]);
-  fprintf(f, "#Y %.16e %.16e %.16e\n", box_y[0], box_y[1], box_y[2]);
-  fprintf(f, "#Z %.16e %.16e %.16e\n", box_z[0], box_z[1], box_z[2]);
+/* <outfiles>.%05d.<ext>, <outfiles>-final.<ext> or <outfiles>-interm.<ext> */
+static void out_name(char *dst, size_t cap, int fzhlr, const char *ext) {
+  if (fzhlr >= 0)
+    snprintf(dst, cap, "%s.%05d.%s", outfilename, fzhlr, ext);
+  else if (fzhlr == -1)
+    snprintf(dst, cap, "%s-final.%s", outfilename, ext);
+  else
+    snprintf(dst, cap, "%s-interm.%s", outfilename, ext);
+}
+
+int write_itr_file(int fzhlr, int steps, const char *suffix) {
+  char fname[IMD_STRLEN + 32];
+  char ext[16];
+  snprintf(ext, sizeof ext, "%sitr", suffix ? suffix : "");
+  out_name(fname, sizeof fname, fzhlr, ext);
+  FILE *out = fopen(fname, "w");
+  if (!out) imd_error("Cannot write iteration file.");
+  fprintf(out, "# checkpoint %d\n", fzhlr);
+  fprintf(out, "startstep \t%d\n", steps + 1);
+  fprintf(out, "box_x \t%.16f %.16f %.16f\n", box_x[0], box_x[1], box_x[2]);
+  fprintf(out, "box_y \t%.16f %.16f %.16f\n", box_y[0], box_y[1], box_y[2]);
+  fprintf(out, "box_z \t%.16f %.16f %.16f\n", box_z[0], box_z[1], box_z[2]);
+  if (ensemble == IMDG_NVT && isq_tau_eta > 0) {
+    fprintf(out, "starttemp \t%.17g\n", temperature);
+    fprintf(out, "eta \t%.17g\n", eta);
+  }
+  fclose(out);
+  return 0;
+}
+
+int write_config(int fzhlr, int steps) {
+  char fname[IMD_STRLEN + 32];
+  out_name(fname, sizeof fname, fzhlr, "chkpt");
+  FILE *f = fopen(fname, "w");
+  if (!f) imd_error("Cannot open output file %s", fname);
+  int nd = have_eam ? 2 : 1;
+  time_t now = time(NULL);
+  fprintf(f, "#F A 1 1 1 3 3 %d\n", nd);
+  fprintf(f, "#C number type mass x y z vx vy vz Epot%s\n",
+          have_eam ? " eam_rho" : "");
+  fprintf(f, "#X \t%.16e %.16e %.16e\n", box_x[0], box_x[1], box_x[2]);
+  fprintf(f, "#Y \t%.16e %.16e %.16e\n", box_y[0], box_y[1], box_y[2]);
+  fprintf(f, "#Z \t%.16e %.16e %.16e\n", box_z[0], box_z[1], box_z[2]);
+  fprintf(f, "##PBC %d %d %d\n", pbc_dirs[0], pbc_dirs[1], pbc_dirs[2]);
+  fprintf(f, "## Generated on %s", ctime(&now));
+  fprintf(f, "## by %s after step %d\n", progname, steps);
   fprintf(f, "#E\n");
   for (long i = 0; i < natoms; i++) {
     double m = atom_mass[i];
-    fprintf(f, "%ld %d %.16e %.16e %.16e %.16e %.16e %.16e %.16e\n",
+    fprintf(f, "%ld %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g",
             atom_number[i], atom_type[i], m, atom_x[3 * i], atom_x[3 * i + 1],
             atom_x[3 * i + 2], atom_p[3 * i] / m, atom_p[3 * i + 1] / m,
-            atom_p[3 * i + 2] / m);
+            atom_p[3 * i + 2] / m, atom_epot[i]);
+    if (have_eam) fprintf(f, " %.17g", atom_rho[i]);
+    fputc('\n', f);
   }
-  fclose(f);
+  if (fclose(f)) imd_error("Cannot write output file %s", fname);
+  write_itr_file(fzhlr, steps, "");
   return 0;
 }
 
 int write_eng_file_header(void) {
   char fname[IMD_STRLEN + 8];
   snprintf(fname, sizeof fname, "%s.eng", outfilename);
-  eng_file = fopen(fname, "w");
-  if (!eng_file) imd_error("Cannot open properties file %s", fname);
-  fprintf(eng_file, "# time Epot temperature pressure volume%s\n",
+  FILE *fl = fopen(fname, "w");
+  if (!fl) imd_error("Cannot open properties file.");
+  fprintf(fl, "# time Epot temperature pressure volume%s\n",
           ensemble == IMDG_NVT ? " eta*tau_eta" : "");
+  fclose(fl);
   return 0;
 }
 
 int write_eng_file(int steps) {
-  if (!eng_file) write_eng_file_header();
+  if (!eng_file) {
+    char fname[IMD_STRLEN + 8];
+    snprintf(fname, sizeof fname, "%s.eng", outfilename);
+    eng_file = fopen(fname, "a");
+    if (!eng_file) imd_error("Cannot open properties file.");
+  }
   double Epot = tot_pot_energy / natoms;
   double Temp = 2.0 * tot_kin_energy / nactive;
   double vol = volume / natoms;

@@ -7,6 +7,15 @@
  * (src/imd_param.c:304-318).  Unknown keys are reported and ignored
  * (src/imd_param.c:3507-3509).  Required keys are checked like
  * check_parameters_complete (src/imd_param.c:3525-3547).
+ *
+ * read_parameters(file, phase) follows src/imd_param.c:3880-3979: a later
+ * phase first writes <outfiles>-final.itr, re-reads the parameter file
+ * through its own section and then that iteration file; phase 1 reads the
+ * itrname file (keeping startstep); with -r n the iteration file
+ * <outfiles>.n.itr (or .%05d.itr) is read and the configuration becomes
+ * <outfiles>.n.chkpt (or .%05d.chkpt); a fresh phase 1 deletes the .eng
+ * file it will append to.  read_command_line takes -p file and -r n, both
+ * also written without the blank (src/imd_param.c:3829-3866).
  */
 #define _GNU_SOURCE
 #include <stdarg.h>
@@ -14,6 +23,8 @@
 #include <stdlib.h>
 #include <string.h>
 #include <strings.h>
+
+#include <unistd.h>
 
 #include "imd_host.h"
 
@@ -29,10 +40,14 @@ double temperature = 0.0;
 double eta = 0.0, tau_eta = 0.0, isq_tau_eta = 0.0;
 double nbl_margin = 0.4, nbl_size = 1.1; /* src/globals.h:419-423 */
 int steps_min = 0, steps_max = 0, eng_int = 0, checkpt_int = 0;
+int watch_int = 0, stop_int = 0;
+int imdrestart = 0, simulation = 1;
 int do_maxwell = 0, box_from_header = 0;
 long seed = 0;
 double maxwalltime = 0.0;
 char infilename[IMD_STRLEN], outfilename[IMD_STRLEN] = "imd";
+char itrfilename[IMD_STRLEN], paramfilename[IMD_STRLEN];
+char progname[IMD_STRLEN] = "imd_gpu";
 char potfilename[IMD_STRLEN], eam2_emb_E_filename[IMD_STRLEN];
 char eam2_at_rho_filename[IMD_STRLEN];
 int gpu_precision = IMDG_MIXED, gpu_device = 0;
@@ -75,10 +90,12 @@ void init_parameters(void) {
   nbl_margin = 0.4;
   nbl_size = 1.1;
   steps_min = steps_max = eng_int = checkpt_int = 0;
+  watch_int = stop_int = 0;
   do_maxwell = box_from_header = 0;
   seed = 0;
   maxwalltime = 0.0;
   infilename[0] = potfilename[0] = eam2_emb_E_filename[0] = 0;
+  itrfilename[0] = 0;
   eam2_at_rho_filename[0] = 0;
   strcpy(outfilename, "imd");
   gpu_precision = IMDG_MIXED;
@@ -119,10 +136,13 @@ static void get_reals(const char *key, double *dst, int n) {
   }
 }
 
-int read_parameters(const char *paramfile, int phase) {
+/* one parameter (or iteration) file through phase `phase`; returns 1 when
+   the end of the file was reached (no later phase follows) */
+static int getparamfile(const char *paramfile, int phase) {
   char buf[1024];
+  int finished = 1;
   FILE *pf = fopen(paramfile, "r");
-  if (!pf) imd_error("Cannot open parameter file %s", paramfile);
+  if (!pf) imd_error("Could not open parameter file %s", paramfile);
   curline = 0;
   while (fgets(buf, sizeof buf, pf)) {
     curline++;
@@ -133,7 +153,10 @@ int read_parameters(const char *paramfile, int phase) {
     if (!strcasecmp(key, "simulation")) {
       int ph;
       get_ints(key, &ph, 1);
-      if (phase < ph) break;
+      if (phase < ph) {
+        finished = 0;
+        break;
+      }
     } else if (!strcasecmp(key, "ensemble")) {
       char s[IMD_STRLEN];
       get_str(key, s);
@@ -153,6 +176,12 @@ int read_parameters(const char *paramfile, int phase) {
       get_ints(key, &checkpt_int, 1);
     } else if (!strcasecmp(key, "eng_int")) {
       get_ints(key, &eng_int, 1);
+    } else if (!strcasecmp(key, "watch_int")) {
+      get_ints(key, &watch_int, 1);
+    } else if (!strcasecmp(key, "stop_int")) {
+      get_ints(key, &stop_int, 1);
+    } else if (!strcasecmp(key, "itrname")) {
+      get_str(key, itrfilename);
     } else if (!strcasecmp(key, "coordname")) {
       get_str(key, infilename);
     } else if (!strcasecmp(key, "outfiles")) {
@@ -239,8 +268,7 @@ int read_parameters(const char *paramfile, int phase) {
     } else if (!strcasecmp(key, "gpu_autotune")) {
       /* time every launch shape once before the main loop (imdg_autotune) */
       get_ints(key, &gpu_autotune, 1);
-    } else if (!strcasecmp(key, "watch_int") || !strcasecmp(key, "stop_int") ||
-               !strcasecmp(key, "binary_output") ||
+    } else if (!strcasecmp(key, "binary_output") ||
                !strcasecmp(key, "debug_potential")) {
       /* accepted; the output and steering they control is not part of
          the force path */
@@ -249,6 +277,85 @@ int read_parameters(const char *paramfile, int phase) {
     }
   }
   fclose(pf);
+  return finished;
+}
+
+/* the first of "<out>.<n>.<ext>" and "<out>.%05d.<ext>" that exists */
+static void numbered_file(char *dst, int n, const char *ext) {
+  snprintf(dst, IMD_STRLEN, "%s.%d.%s", outfilename, n, ext);
+  FILE *f = fopen(dst, "r");
+  if (!f) {
+    snprintf(dst, IMD_STRLEN, "%s.%05d.%s", outfilename, n, ext);
+    f = fopen(dst, "r");
+    if (!f) imd_error("file %s not found", dst);
+  }
+  fclose(f);
+}
+
+int read_parameters(const char *paramfile, int phase) {
+  char fname[IMD_STRLEN];
+  /* the iteration file that carries the state into the next phase */
+  if (phase > 1) write_itr_file(-1, steps_max, "");
+  int finished = getparamfile(paramfile, phase);
+  if (phase == 1 && itrfilename[0]) {
+    int keep = steps_min;
+    getparamfile(itrfilename, 1);
+    steps_min = keep;
+  }
+  if (phase > 1) {
+    snprintf(itrfilename, sizeof itrfilename, "%s-final.itr", outfilename);
+    getparamfile(itrfilename, 1);
+  }
+  check_parameters_complete();
+  if (imdrestart) {
+    numbered_file(fname, imdrestart, "itr");
+    getparamfile(fname, 1);
+    numbered_file(infilename, imdrestart, "chkpt");
+    printf("Restarting from %s.\n", infilename);
+  } else if (phase == 1) {
+    /* a fresh run: the .eng file is appended to, so start it anew */
+    snprintf(fname, sizeof fname, "%s.eng", outfilename);
+    unlink(fname);
+  }
+  return finished;
+}
+
+static void usage(void) {
+  fprintf(stderr, "usage: %s [-r nnn] [-p paramfile]\n", progname);
+}
+
+int read_command_line(int argc, char **argv) {
+  strncpy(progname, argv[0], IMD_STRLEN - 1);
+  while (argc > 1 && argv[1][0] == '-') {
+    char opt = argv[1][1];
+    const char *val = NULL;
+    if (opt == 'r' || opt == 'p') {
+      if (argv[1][2] == 0) {
+        if (argc > 2) {
+          val = argv[2];
+          argc--;
+          argv++;
+        }
+      } else {
+        val = argv[1] + 2;
+      }
+      if (val && opt == 'r') imdrestart = atoi(val);
+      if (val && opt == 'p') {
+        strncpy(paramfilename, val, IMD_STRLEN - 1);
+        paramfilename[IMD_STRLEN - 1] = 0;
+      }
+    } else {
+      printf("Illegal option %s \n", argv[1]);
+      usage();
+      exit(-1);
+    }
+    argv++;
+    argc--;
+  }
+  if (!paramfilename[0]) {
+    usage();
+    exit(2);
+  }
   return 0;
 }
 

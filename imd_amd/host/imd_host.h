@@ -47,10 +47,14 @@ extern double temperature;        /* starttemp, k_B T */
 extern double eta, tau_eta, isq_tau_eta;
 extern double nbl_margin, nbl_size;
 extern int steps_min, steps_max, eng_int, checkpt_int;
+extern int watch_int, stop_int;
+extern int imdrestart, simulation;  /* -r n; current phase */
 extern int do_maxwell, box_from_header;
 extern long seed;
 extern double maxwalltime;
 extern char infilename[IMD_STRLEN], outfilename[IMD_STRLEN];
+extern char itrfilename[IMD_STRLEN], paramfilename[IMD_STRLEN];
+extern char progname[IMD_STRLEN];
 extern char potfilename[IMD_STRLEN], eam2_emb_E_filename[IMD_STRLEN];
 extern char eam2_at_rho_filename[IMD_STRLEN];
 extern int gpu_precision, gpu_device, gpu_lanes, gpu_autotune;
@@ -63,6 +67,7 @@ extern double cellsz;
 extern long natoms;
 extern int nactive;
 extern double *atom_x, *atom_p, *atom_mass;
+extern double *atom_epot, *atom_rho; /* per-atom Epot and eam_rho */
 extern int *atom_type;
 extern long *atom_number;
 
@@ -78,8 +83,10 @@ extern void (*move_atoms)(void);
 void imd_error(const char *fmt, ...);
 void imd_warning(const char *fmt, ...);
 
-/* parameters: defaults, then one simulation phase of a tag file */
+/* parameters: defaults, then one simulation phase of a tag file;
+   read_parameters returns 1 when no later phase follows */
 void init_parameters(void);
+int  read_command_line(int argc, char **argv);
 int  read_parameters(const char *paramfile, int phase);
 int  check_parameters_complete(void);
 
@@ -92,14 +99,19 @@ int  setup_potentials(void);
 /* configuration */
 int  read_atoms(const char *filename);
 int  generate_fcc(void);
+void make_box(void);  /* volume from the box vectors */
 void maxwell(double temp);
-int  write_config(const char *filename);
+/* checkpoint number n (-1: final, -2: intermediate) after step `steps`,
+   with its iteration file */
+int  write_config(int fzhlr, int steps);
+int  write_itr_file(int fzhlr, int steps, const char *suffix);
 int  write_eng_file_header(void);
 int  write_eng_file(int steps);
 
 /* force module entry points (IMD names and semantics) */
 int  imd_gpu_init(void);
 void imd_gpu_finish(void);
+void imd_gpu_reconfigure(void);  /* between phases */
 void calc_forces(int steps);
 void make_nblist(void);
 void check_nblist(void);
@@ -108,8 +120,9 @@ void move_atoms_nve(void);
 void move_atoms_nvt(void);
 void fetch_atoms(void);
 
-/* main loop: IMD's per-step sequence, fused on the device between
-   output steps */
+/* main loop of one phase: IMD's per-step sequence, fused on the device
+   between output steps; returns 1 when the simulation is to end (stop
+   file, wall time) */
 int  main_loop(void);
 
 imdg_ctx *imd_gpu_context(void);

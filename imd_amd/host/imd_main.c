@@ -1,8 +1,11 @@
 /*
- * imd_main.c -- "imd_gpu -p paramfile": IMD's program sequence for the
- * neighbour-list force path (src/imd.c:32-467): parameters, potentials,
- * configuration (read or generated), Maxwell velocities if needed, the main
- * loop, the final configuration.
+ * imd_main.c -- "imd_gpu [-r n] -p paramfile": IMD's program sequence for
+ * the neighbour-list force path (src/imd.c:32-467): command line,
+ * parameters, potentials, configuration (read, generated, or the
+ * checkpoint of a restart), Maxwell velocities if needed and not
+ * restarting, the .eng header unless restarting, then the main loop of
+ * each simulation phase until the parameter file has no further phase or a
+ * phase ends the simulation.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -11,33 +14,32 @@
 #include "imd_host.h"
 
 int main(int argc, char **argv) {
-  const char *pfile = NULL;
-  for (int i = 1; i < argc; i++) {
-    if (!strcmp(argv[i], "-p") && i + 1 < argc)
-      pfile = argv[++i];
-    else if (!strncmp(argv[i], "-p", 2) && argv[i][2])
-      pfile = argv[i] + 2;
-  }
-  if (!pfile) {
-    fprintf(stderr, "usage: %s -p paramfile\n", argv[0]);
-    return 2;
-  }
+  read_command_line(argc, argv);
   init_parameters();
-  read_parameters(pfile, 1);
-  check_parameters_complete();
+  int finished = read_parameters(paramfilename, simulation);
   setup_potentials();
   if (infilename[0])
     read_atoms(infilename);
   else
     generate_fcc();
-  if (do_maxwell) maxwell(temperature);
+  if (!imdrestart && do_maxwell) maxwell(temperature);
+  do_maxwell = 0;
   imd_gpu_init();
-  write_eng_file_header();
-  main_loop();
-  fetch_atoms();
-  char fn[IMD_STRLEN + 16];
-  snprintf(fn, sizeof fn, "%s.end.chkpt", outfilename);
-  write_config(fn);
+  if (!imdrestart && eng_int > 0) write_eng_file_header();
+  if (steps_min <= steps_max) {
+    printf("Starting simulation %d\n", simulation);
+    if (main_loop()) finished = 1;
+  }
+  while (!finished) {
+    simulation++;
+    finished = read_parameters(paramfilename, simulation);
+    if (steps_min <= steps_max) {
+      imd_gpu_reconfigure();
+      printf("Starting simulation %d\n", simulation);
+      if (main_loop()) finished = 1;
+    }
+  }
   imd_gpu_finish();
+  printf("Done simulation.\n");
   return 0;
 }

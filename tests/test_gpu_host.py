@@ -1,10 +1,12 @@
 """The IMD-style C host on the GPU: `imd_gpu -p file.param` reads IMD's
 parameter keys, format-2 tables and an ASCII configuration, runs the fused
 main loop through the C ABI and writes the `.eng` file; its rows must match
-the oracle's thermo at the same steps (FP64 path).  The executable runs as
-a child process."""
+the oracle's thermo at the same steps (FP64 path).  Checkpoints, `-r n`
+restarts, phases and the stop/write files must continue the same
+trajectory.  The executable runs as a child process."""
 import dataclasses
 import os
+import shutil
 import subprocess
 
 import numpy as np
@@ -21,7 +23,9 @@ EXE = os.path.join(B.HOST, "imd_gpu")
 
 def _read_eng(path):
     rows = [l.split() for l in open(path) if l.strip() and not l.startswith("#")]
-    return np.array([[float(v) for v in r] for r in rows])
+    # an NVT phase adds the eta*tau_eta column: keep the common columns
+    w = min(len(r) for r in rows)
+    return np.array([[float(v) for v in r[:w]] for r in rows])
 
 
 @pytest.mark.parametrize("ensemble,extra", [
@@ -87,5 +91,163 @@ def test_imd_gpu_eng_matches_oracle(tmp_path, ensemble, extra):
         assert abs(row[1] - epot) <= 1e-9 * abs(epot), (step, row[1], epot)
         # the temperature column is printed with 6 significant digits
         assert abs(row[2] - temp) <= 1e-5 * abs(temp), (step, row[2], temp)
-    # the final configuration is written in IMD's format
-    assert os.path.exists(str(tmp_path / "run.end.chkpt"))
+    # the phase ends with the iteration file of its last step
+    itr = open(str(tmp_path / "run-final.itr")).read().splitlines()
+    assert itr[1] == "startstep \t%d" % (nsteps + 1)
+
+
+def _setup(tmp_path, temp_k=700.0):
+    eam = eam_tables()
+    s = W.eam_crystal(n_cells=5, temp_k=temp_k, seed=21, displace=0.03)
+    s = dataclasses.replace(s, p=(s.p / s.mass[:, None]) * s.mass[:, None])
+    files = eam.write(str(tmp_path / "pot"))
+    cfg = s.write_config(str(tmp_path / "start.chkpt"))
+    return eam, s, files, cfg
+
+
+def _lines(s, files, cfg, out, ensemble, maxsteps, extra=()):
+    lines = [
+        "ensemble %s" % ensemble, "ntypes 1", "masses %.17g" % s.mass[0],
+        "timestep %.17g" % W.FS, "startstep 0", "maxsteps %d" % maxsteps,
+        "eng_int 10", "coordname %s" % cfg, "outfiles %s" % out,
+        "core_potential_file %s" % files["core_potential_file"],
+        "embedding_energy_file %s" % files["embedding_energy_file"],
+        "atomic_e-density_file %s" % files["atomic_e-density_file"],
+        "nbl_margin 0.4", "gpu_precision fp64", "box_from_header 1"]
+    if ensemble == "nvt":
+        lines += ["starttemp %.17g" % W.kT(700.0), "tau_eta %.17g" % (50 * W.FS)]
+    return lines + list(extra)
+
+
+def _run(dirpath, lines, args=()):
+    param = os.path.join(str(dirpath), "run.param")
+    open(param, "w").write("\n".join(lines) + "\n")
+    r = subprocess.run([EXE, *args, "-p", param], capture_output=True,
+                       text=True, timeout=120, cwd=str(dirpath))
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r
+
+
+def _rows(path):
+    """number type mass x y z vx vy vz Epot eam_rho rows of a checkpoint"""
+    lines = open(path).read().splitlines()
+    end = lines.index("#E")
+    return np.array([[float(v) for v in l.split()] for l in lines[end + 1:]])
+
+
+def test_checkpoint_restart_continues_trajectory(tmp_path):
+    """A 40-step NVT run with checkpt_int 20 writes run.00001/2.chkpt and
+    .itr; `imd_gpu -r 1` in a fresh directory with the same parameter file
+    continues from step 21 (startstep, box, starttemp and eta from the
+    iteration file; x and v from the checkpoint) and must reach the same
+    state and .eng rows as the uninterrupted run and the oracle."""
+    eam, s, files, cfg = _setup(tmp_path)
+    a = tmp_path / "a"
+    b = tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    lines = _lines(s, files, cfg, "run", "nvt", 40, ["checkpt_int 20"])
+    _run(a, lines)
+    for k in (1, 2):
+        for ext in ("chkpt", "itr"):
+            assert os.path.exists(str(a / ("run.%05d.%s" % (k, ext))))
+    itr = open(str(a / "run.00001.itr")).read().splitlines()
+    assert itr[0] == "# checkpoint 1" and itr[1] == "startstep \t21"
+    assert [l.split()[0] for l in itr[5:]] == ["starttemp", "eta"]
+    for ext in ("chkpt", "itr"):
+        shutil.copy(str(a / ("run.00001.%s" % ext)), str(b))
+    r = _run(b, lines, ["-r", "1"])
+    assert "Restarting from run.00001.chkpt." in r.stdout
+    # the restart appends to .eng without a header: rows 30 and 40
+    eng_a = _read_eng(str(a / "run.eng"))
+    eng_b = _read_eng(str(b / "run.eng"))
+    assert eng_a.shape[0] == 5 and eng_b.shape[0] == 2
+    assert np.allclose(eng_b[:, 0], eng_a[3:, 0], rtol=1e-12)
+    assert np.allclose(eng_b[:, 1], eng_a[3:, 1], rtol=1e-11, atol=0)
+    assert np.allclose(eng_b[:, 5], eng_a[3:, 5], rtol=1e-5)  # eta*tau, %e
+    ca = _rows(str(a / "run.00002.chkpt"))
+    cb = _rows(str(b / "run.00002.chkpt"))
+    assert np.array_equal(ca[:, :3], cb[:, :3])
+    # positions are wrapped at list rebuilds, which the two runs do at
+    # different steps: compare them modulo the (orthogonal) box
+    L = np.diag(s.box)
+    d = ca[:, 3:6] - cb[:, 3:6]
+    d -= L * np.round(d / L)
+    assert np.max(np.abs(d)) < 1e-10
+    vmax = np.max(np.abs(ca[:, 6:9]))
+    assert np.max(np.abs(ca[:, 6:9] - cb[:, 6:9])) < 1e-10 * vmax
+    assert np.allclose(cb[:, 9:], ca[:, 9:], rtol=1e-10, atol=1e-12)
+    # and the oracle's uninterrupted run
+    import oracle as O
+    kT, tau = W.kT(700.0), 50 * W.FS
+    orc = oracle_eam(s, eam, nbl_margin=0.4, timestep=W.FS,
+                     ensemble=O.ORC_NVT, temperature=kT, eta=0.0,
+                     isq_tau_eta=1.0 / tau ** 2)
+    th = orc.run(41)
+    for row in eng_b:
+        step = int(round(row[0] / W.FS))
+        ep = th[step, 0] / s.n
+        assert abs(row[1] - ep) <= 1e-9 * abs(ep), (step, row[1], ep)
+    # per-atom Epot and eam_rho of the checkpoint (input order): the
+    # oracle's from the force evaluation of step 40
+    o = orc.atoms()
+    assert np.allclose(cb[:, 9], o["epot"], rtol=1e-9, atol=1e-12)
+    assert np.allclose(cb[:, 10], o["rho"], rtol=1e-9, atol=1e-12)
+
+
+def test_stop_and_write_files(tmp_path):
+    """An empty "write" file found at a watch_int step writes
+    run-interm.chkpt and is removed; a "stop" file found at a stop_int step
+    writes run-final.chkpt / .itr and ends the run
+    (src/imd_main_3d.c:1892-1940).  Step 0 is such a step, so files present
+    at the start are found after step 0."""
+    eam, s, files, cfg = _setup(tmp_path)
+    open(str(tmp_path / "write"), "w").close()
+    r = _run(tmp_path, _lines(s, files, cfg, "run", "nve", 40,
+                              ["watch_int 10"]))
+    assert "Write file found after 0 steps" in r.stdout
+    assert not os.path.exists(str(tmp_path / "write"))
+    assert os.path.exists(str(tmp_path / "run-interm.chkpt"))
+    assert _read_eng(str(tmp_path / "run.eng")).shape[0] == 5
+    open(str(tmp_path / "stop"), "w").close()
+    r = _run(tmp_path, _lines(s, files, cfg, "run", "nve", 40,
+                              ["stop_int 20"]))
+    assert "Stop file found after 0 steps" in r.stdout
+    assert "Did 1 steps" in r.stdout
+    assert not os.path.exists(str(tmp_path / "stop"))
+    assert _read_eng(str(tmp_path / "run.eng")).shape[0] == 1
+    itr = open(str(tmp_path / "run-final.itr")).read().splitlines()
+    assert itr[1] == "startstep \t1"
+    c = _rows(str(tmp_path / "run-final.chkpt"))
+    assert c.shape == (s.n, 11)
+    # maxwalltime is tested after every fused run
+    r = _run(tmp_path, _lines(s, files, cfg, "run", "nve", 40,
+                              ["maxwalltime 1e-9"]))
+    assert "Maximal allowed walltime reached after 0 steps" in r.stdout
+
+
+def test_phases_nvt_then_nve(tmp_path):
+    """Phase 1 NVT through step 20, phase 2 NVE through step 40: the state
+    (x, p) carries over, startstep comes from run-final.itr, and the .eng
+    rows of phase 2 match an oracle that switches at the same step."""
+    eam, s, files, cfg = _setup(tmp_path)
+    lines = _lines(s, files, cfg, "run", "nvt", 20) + [
+        "simulation 2", "ensemble nve", "maxsteps 40"]
+    r = _run(tmp_path, lines)
+    assert "Starting simulation 2" in r.stdout
+    eng = _read_eng(str(tmp_path / "run.eng"))
+    assert eng.shape[0] == 5
+    import oracle as O
+    kT, tau = W.kT(700.0), 50 * W.FS
+    o1 = oracle_eam(s, eam, nbl_margin=0.4, timestep=W.FS,
+                    ensemble=O.ORC_NVT, temperature=kT, eta=0.0,
+                    isq_tau_eta=1.0 / tau ** 2)
+    th1 = o1.run(21)
+    st = o1.atoms()
+    s2 = dataclasses.replace(s, x=st["x"], p=st["p"])
+    o2 = oracle_eam(s2, eam, nbl_margin=0.4, timestep=W.FS)
+    th2 = o2.run(20)  # steps 21..40
+    for row in eng:
+        step = int(round(row[0] / W.FS))
+        ep = (th1[step, 0] if step <= 20 else th2[step - 21, 0]) / s.n
+        assert abs(row[1] - ep) <= 1e-9 * abs(ep), (step, row[1], ep)

@@ -104,11 +104,12 @@ atomic_e-density_file c.pt
 gpu_precision fp64
 gpu_lanes 2
 gpu_autotune 1
+outfiles %s
 simulation 2
 maxsteps 999
-""")
+""" % str(tmp_path / "run"))
     host.init_parameters()
-    host.read_parameters(str(p).encode(), 1)
+    assert host.read_parameters(str(p).encode(), 1) == 0  # phase 2 follows
     g = lambda t, n: t.in_dll(host, n).value
     assert g(C.c_int, "ensemble") == 1 and g(C.c_int, "ntypes") == 2
     assert g(C.c_int, "steps_max") == 100          # phase 2 not read
@@ -147,10 +148,202 @@ def test_config_reader(host, tmp_path):
 def test_missing_ensemble_is_fatal(tmp_path):
     p = tmp_path / "bad.param"
     p.write_text("ntypes 1\ntimestep 0.1\ncore_potential_file x.pt\n")
-    exe = os.path.join(ROOT, "imd_amd", "host", "imd_gpu")
-    if not os.path.exists(exe):
-        B.build_host()
-    r = subprocess.run([exe, "-p", str(p)], capture_output=True, text=True,
+    r = subprocess.run([_exe(), "-p", str(p)], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 2
     assert "ensemble" in r.stderr
+
+
+# ---------------------------------------------------------------------------
+# checkpoints, iteration files, restart and phases
+# ---------------------------------------------------------------------------
+
+def _set_str(host, name, val):
+    (C.c_char * 256).in_dll(host, name).value = val.encode()
+
+
+def _get_str(host, name):
+    return (C.c_char * 256).in_dll(host, name).value.decode()
+
+
+def _param(tmp_path, out, extra=()):
+    p = tmp_path / "run.param"
+    p.write_text("\n".join([
+        "ensemble nvt", "ntypes 1", "masses 63.546", "timestep 0.098227",
+        "startstep 0", "maxsteps 100", "starttemp 0.05", "tau_eta 9.8227",
+        "core_potential_file a.pt", "embedding_energy_file b.pt",
+        "atomic_e-density_file c.pt", "outfiles %s" % out, *extra]) + "\n")
+    return str(p).encode()
+
+
+def _nvt_state(host, out):
+    """Globals a checkpoint carries, with values that need all 17 digits."""
+    host.init_parameters()
+    _set_str(host, "outfilename", out)
+    C.c_int.in_dll(host, "ensemble").value = 1
+    C.c_double.in_dll(host, "temperature").value = 0.1 / 3
+    C.c_double.in_dll(host, "eta").value = -2.0 / 7
+    C.c_double.in_dll(host, "isq_tau_eta").value = 1 / 9.8227 ** 2
+    box = {"box_x": [289.2 / 3, 0.0, 0.0], "box_y": [1e-3 / 3, 96.4, 0.0],
+           "box_z": [0.0, 0.0, 100.0 / 7]}
+    for k, v in box.items():
+        (C.c_double * 3).in_dll(host, k)[:] = v
+    return box
+
+
+def _box_back(host):
+    return {k: list((C.c_double * 3).in_dll(host, k))
+            for k in ("box_x", "box_y", "box_z")}
+
+
+def test_itr_file_format_and_restart_lookup(host, tmp_path, capfd):
+    """write_itr_file (src/imd_io_3d.c:949-1095) writes a parameter file with
+    startstep = step + 1, the box and, for NVT, starttemp and eta; -r n
+    (src/imd_param.c:3907-3941) reads it and takes <out>.%05d.chkpt as the
+    configuration."""
+    out = str(tmp_path / "run")
+    box = _nvt_state(host, out)
+    host.write_itr_file.argtypes = [C.c_int, C.c_int, C.c_char_p]
+    assert host.write_itr_file(3, 41, b"") == 0
+    text = open(out + ".00003.itr").read().splitlines()
+    assert text[0] == "# checkpoint 3"
+    assert text[1] == "startstep \t42"
+    assert [l.split()[0] for l in text[2:]] == ["box_x", "box_y", "box_z",
+                                                "starttemp", "eta"]
+    open(out + ".00003.chkpt", "w").write("")
+    param = _param(tmp_path, out)
+    host.init_parameters()
+    C.c_int.in_dll(host, "imdrestart").value = 3
+    try:
+        host.read_parameters(param, 1)
+    finally:
+        C.c_int.in_dll(host, "imdrestart").value = 0
+    g = lambda t, n: t.in_dll(host, n).value
+    assert g(C.c_int, "steps_min") == 42 and g(C.c_int, "steps_max") == 100
+    assert g(C.c_double, "temperature") == 0.1 / 3
+    assert g(C.c_double, "eta") == -2.0 / 7
+    # %.16f keeps 16 decimals of the box: within 1e-16 absolute
+    for k, v in _box_back(host).items():
+        assert np.allclose(v, box[k], rtol=0, atol=1e-15), k
+    assert _get_str(host, "infilename") == out + ".00003.chkpt"
+    C.CDLL(None).fflush(None)  # the C library's stdout buffer
+    assert "Restarting from %s.00003.chkpt." % out in capfd.readouterr().out
+
+
+def test_restart_prefers_unpadded_names(host, tmp_path):
+    out = str(tmp_path / "run")
+    _nvt_state(host, out)
+    host.write_itr_file.argtypes = [C.c_int, C.c_int, C.c_char_p]
+    host.write_itr_file(7, 69, b"")
+    os.rename(out + ".00007.itr", out + ".7.itr")
+    for name in (".7.chkpt", ".00007.chkpt"):
+        open(out + name, "w").write("")
+    host.init_parameters()
+    C.c_int.in_dll(host, "imdrestart").value = 7
+    try:
+        host.read_parameters(_param(tmp_path, out), 1)
+    finally:
+        C.c_int.in_dll(host, "imdrestart").value = 0
+    assert C.c_int.in_dll(host, "steps_min").value == 70
+    assert _get_str(host, "infilename") == out + ".7.chkpt"
+
+
+def test_checkpoint_round_trip(host, tmp_path):
+    """write_config(n, steps) writes IMD's checkpoint (header
+    src/imd_io.c:3152-3300, rows src/imd_io_3d.c:696-948) and its
+    iteration file; read_atoms reads the same positions back bit for bit
+    and p = (p/m) m to the last bit or so."""
+    s = W.fcc_system((3, 3, 3), 3.615, [63.546, 58.69], W.kT(300), 5,
+                     type_fractions=(0.5, 0.5), displace=0.05)
+    path = str(tmp_path / "c.cfg")
+    s.write_config(path)
+    out = str(tmp_path / "ck")
+    host.init_parameters()
+    C.c_int.in_dll(host, "ntypes").value = 2
+    C.c_int.in_dll(host, "ensemble").value = 0
+    C.c_int.in_dll(host, "have_eam").value = 1
+    assert host.read_atoms(path.encode()) == 0
+    n = s.n
+    ep = np.ctypeslib.as_array(C.POINTER(C.c_double).in_dll(host, "atom_epot"), (n,))
+    rh = np.ctypeslib.as_array(C.POINTER(C.c_double).in_dll(host, "atom_rho"), (n,))
+    rng = np.random.default_rng(3)
+    ep[:] = -3.5 + rng.random(n) / 3
+    rh[:] = 0.25 + rng.random(n) / 7
+    epot, rho = ep.copy(), rh.copy()
+    _set_str(host, "outfilename", out)
+    host.write_config.argtypes = [C.c_int, C.c_int]
+    assert host.write_config(2, 20) == 0
+    lines = open(out + ".00002.chkpt").read().splitlines()
+    assert lines[0] == "#F A 1 1 1 3 3 2"
+    assert lines[1] == "#C number type mass x y z vx vy vz Epot eam_rho"
+    assert lines[2].startswith("#X") and lines[5] == "##PBC 1 1 1"
+    end = lines.index("#E")
+    rows = np.array([[float(v) for v in l.split()] for l in lines[end + 1:]])
+    assert rows.shape == (n, 11)
+    assert np.array_equal(rows[:, 0], s.id) and np.array_equal(rows[:, 1], s.type)
+    assert np.array_equal(rows[:, 9], epot) and np.array_equal(rows[:, 10], rho)
+    assert open(out + ".00002.itr").read().splitlines()[1] == "startstep \t21"
+    for name, fz in (("-final", -1), ("-interm", -2)):
+        host.write_config(fz, 20)
+        assert os.path.exists(out + name + ".chkpt")
+        assert os.path.exists(out + name + ".itr")
+    # read the checkpoint back
+    host.init_parameters()
+    C.c_int.in_dll(host, "ntypes").value = 2
+    assert host.read_atoms((out + ".00002.chkpt").encode()) == 0
+    assert C.c_long.in_dll(host, "natoms").value == n
+    assert C.c_int.in_dll(host, "do_maxwell").value == 0  # velocities given
+    x = np.ctypeslib.as_array(C.POINTER(C.c_double).in_dll(host, "atom_x"), (n, 3))
+    p = np.ctypeslib.as_array(C.POINTER(C.c_double).in_dll(host, "atom_p"), (n, 3))
+    assert np.array_equal(x, s.x)
+    assert np.allclose(p, s.p, rtol=4e-16, atol=0)
+    assert list((C.c_double * 3).in_dll(host, "box_x")) == list(s.box[0])
+
+
+def test_phases_carry_state(host, tmp_path):
+    """read_parameters(file, n > 1) (src/imd_param.c:3880-3903): writes
+    <out>-final.itr of the phase that ended, reads the file through phase n
+    and then that iteration file; returns 1 at the last phase."""
+    out = str(tmp_path / "ph")
+    p = tmp_path / "ph.param"
+    p.write_text("\n".join([
+        "ensemble nvt", "ntypes 1", "masses 63.546", "timestep 0.098227",
+        "maxsteps 20", "starttemp 0.05", "tau_eta 9.8227",
+        "core_potential_file a.pt", "embedding_energy_file b.pt",
+        "atomic_e-density_file c.pt", "outfiles %s" % out,
+        "simulation 2", "ensemble nve", "maxsteps 40", "timestep 0.05",
+        "simulation 3", "maxsteps 60"]) + "\n")
+    g = lambda t, n: t.in_dll(host, n).value
+    host.init_parameters()
+    assert host.read_parameters(str(p).encode(), 1) == 0
+    assert g(C.c_int, "ensemble") == 1 and g(C.c_int, "steps_max") == 20
+    C.c_double.in_dll(host, "eta").value = 0.125  # as left by phase 1
+    assert host.read_parameters(str(p).encode(), 2) == 0
+    assert os.path.exists(out + "-final.itr")
+    assert g(C.c_int, "ensemble") == 0
+    assert g(C.c_int, "steps_min") == 21 and g(C.c_int, "steps_max") == 40
+    assert g(C.c_double, "timestep") == 0.05
+    assert host.read_parameters(str(p).encode(), 3) == 1
+    assert g(C.c_int, "steps_min") == 41 and g(C.c_int, "steps_max") == 60
+
+
+def _exe():
+    exe = os.path.join(ROOT, "imd_amd", "host", "imd_gpu")
+    if not os.path.exists(exe):
+        B.build_host()
+    return exe
+
+
+def test_command_line(tmp_path):
+    """-x is an illegal option (exit -1); -r n without its iteration file is
+    a fatal error, raised before any device work."""
+    r = subprocess.run([_exe(), "-x"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 255 and "Illegal option -x" in r.stdout
+    out = str(tmp_path / "run")
+    param = _param(tmp_path, out).decode()
+    for args in (["-r", "7", "-p", param], ["-r7", "-p" + param]):
+        r = subprocess.run([_exe(), *args], capture_output=True, text=True,
+                           timeout=60, cwd=str(tmp_path))
+        assert r.returncode == 2, r.stderr
+        assert "run.00007.itr not found" in r.stderr
