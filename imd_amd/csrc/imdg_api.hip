@@ -38,6 +38,9 @@ template <typename R>
 hipError_t launch_finalize(const StepArgs<R>&, bool, bool, hipStream_t);
 template <typename R>
 hipError_t launch_copyrec(const StepArgs<R>&, int, hipStream_t);
+template <typename R>
+hipError_t launch_stress(const StepArgs<R>&, const TabDev<R>*, double*,
+                         double*, int, hipStream_t);
 void launch_build(const BuildArgs&, int, hipStream_t);
 void launch_check(const RepDev*, Ctl*, const int*, const double4*,
                   const double4*, const double4*, double*, int, hipStream_t);
@@ -138,6 +141,12 @@ struct imdg_ctx {
   // release fence in every block of the launch costs more than the launch)
   bool final_inline = false;
   hipGraphExec_t step_graph = nullptr;
+  // per-atom stress tensor (imdg_set_stress): computed after the last step
+  // of every imdg_run and after every imdg_move_atoms while enabled
+  bool stress_on = false;
+  bool stress_valid = false;     // d_stress holds the last step's tensor
+  double* d_stress = nullptr;    // 6 per slot: xx yy zz yz zx xy
+  double* d_spart = nullptr;     // 6 per IMDG_BLOCK block
   bool atoms_set = false;
   long target_steps = 0;
   // profiling
@@ -203,6 +212,9 @@ void free_all(imdg_ctx* c) {
   c->bufs.clear();
   c->d_nbr = nullptr;
   c->nbr_bytes = 0;
+  c->d_stress = nullptr;
+  c->d_spart = nullptr;
+  c->stress_valid = false;
 }
 
 void vec_prod(const double* u, const double* v, double* w) {
@@ -677,6 +689,33 @@ int enqueue_move(imdg_ctx* c) {
   else
     HIPCHK(c, enqueue_move_t<double>(c));
   HIPCHK(c, hipGetLastError());
+  return IMDG_OK;
+}
+
+template <typename R>
+hipError_t enqueue_stress_t(imdg_ctx* c) {
+  StepArgs<R> a = step_args<R>(c);
+  return launch_stress<R>(a, (const TabDev<R>*)c->d_tabs, c->d_stress,
+                          c->d_spart, c->nblk, c->stream);
+}
+
+// The stress tensor of the step just enqueued (never part of a captured
+// step: it runs after the step's last kernel).
+int enqueue_stress(imdg_ctx* c) {
+  int rc;
+  if (!c->d_stress) {
+    if ((rc = dalloc(c, (void**)&c->d_stress,
+                     sizeof(double) * 6 * (size_t)c->nslots)))
+      return rc;
+    if ((rc = dalloc(c, (void**)&c->d_spart,
+                     sizeof(double) * 6 * (size_t)c->nblk)))
+      return rc;
+  }
+  if (c->mixed)
+    HIPCHK(c, enqueue_stress_t<float>(c));
+  else
+    HIPCHK(c, enqueue_stress_t<double>(c));
+  c->stress_valid = true;
   return IMDG_OK;
 }
 
@@ -1224,6 +1263,7 @@ int imdg_calc_forces(imdg_ctx* c, int steps) {
   if (!c) return IMDG_ERR_ARG;
   int rc = ensure_ready(c);
   if (rc) return rc;
+  c->stress_valid = false;
   if ((rc = build_until_fits(c))) return rc;
   if ((rc = enqueue_forces(c, false))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1236,6 +1276,7 @@ int imdg_move_atoms(imdg_ctx* c) {
   int rc = ensure_ready(c);
   if (rc) return rc;
   if ((rc = enqueue_move(c))) return rc;
+  if (c->stress_on && (rc = enqueue_stress(c))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   collect_profile(c);
   c->target_steps += 1;
@@ -1292,6 +1333,7 @@ int imdg_make_nblist(imdg_ctx* c) {
   if (!c) return IMDG_ERR_ARG;
   int rc = ensure_ready(c);
   if (rc) return rc;
+  c->stress_valid = false;  // the rebuild renumbers the slots
   if ((rc = flag_all_for_rebuild(c))) return rc;
   if ((rc = build_until_fits(c))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1346,7 +1388,63 @@ int imdg_run(imdg_ctx* c, long nsteps) {
       if (c->split_move && (rc = enqueue_move(c))) return rc;
     }
   }
+  if (nsteps > 0) {
+    c->stress_valid = false;
+    if (c->stress_on && (rc = enqueue_stress(c))) return rc;
+  }
   c->target_steps += nsteps;
+  return IMDG_OK;
+}
+
+int imdg_set_stress(imdg_ctx* c, int enable) {
+  if (!c) return IMDG_ERR_ARG;
+  c->stress_on = enable != 0;
+  if (!c->stress_on) c->stress_valid = false;
+  return IMDG_OK;
+}
+
+int imdg_get_stress(imdg_ctx* c, double* per_atom, double* total) {
+  if (!c || !c->atoms_set) return IMDG_ERR_ARG;
+  hipSetDevice(c->dev);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!c->stress_valid)
+    return fail(c, IMDG_ERR_STATE,
+                "no stress tensor of the last step (imdg_set_stress, then "
+                "imdg_run or imdg_move_atoms)");
+  SlotView v;
+  int rc = read_slots(c, v, false);
+  if (rc) return rc;
+  if (v.ctl.frozen)
+    return fail(c, IMDG_ERR_STATE, "list overflow pending (imdg_sync first)");
+  if (total) {
+    std::vector<double> sp((size_t)6 * c->nblk);
+    HIPCHK(c, hipMemcpy(sp.data(), c->d_spart, sizeof(double) * sp.size(),
+                        hipMemcpyDeviceToHost));
+    for (int r = 0; r < c->nrep; r++) {
+      const RepDev& R = c->rep[r];
+      for (int q = 0; q < 6; q++) {
+        double t = 0.0;
+        for (int b = 0; b < R.nblk; b++) t += sp[(size_t)(R.blk_off + b) * 6 + q];
+        total[(size_t)6 * r + q] = t;
+      }
+    }
+  }
+  if (per_atom) {
+    std::vector<double> st((size_t)6 * c->nslots);
+    HIPCHK(c, hipMemcpy(st.data(), c->d_stress, sizeof(double) * st.size(),
+                        hipMemcpyDeviceToHost));
+    for (int r = 0; r < c->nrep; r++) {
+      const RepDev& R = c->rep[r];
+      for (int k = 0; k < R.natoms; k++) {
+        int s = R.atom_off + k;
+        long in = v.aid[s];
+        if (in < 0 || in >= c->natoms_tot)
+          return fail(c, IMDG_ERR_STATE, "slot %d has no atom", s);
+        for (int q = 0; q < 6; q++)
+          per_atom[(size_t)6 * in + q] = st[(size_t)6 * s + q];
+      }
+    }
+  }
   return IMDG_OK;
 }
 

@@ -1263,6 +1263,125 @@ __global__ void __launch_bounds__(256) k_copyrec(StepArgs<R> a, int nslots) {
   if (i < nslots) a.rec_b[i] = a.rec_a[i];
 }
 
+// Per-atom stress tensor of the step just done (STRESS_TENS:
+// src/imd_forces_nbl.c:558-581, 1283-1318, src/imd_integrate.c:411-431,
+// 1085-1094): S_i = -1/2 sum_j g_ij d d^T over the full list, with d = x_j -
+// x_i and g_ij the pair gradient of the force loop (pair term plus
+// 1/2 (2F'_i rho'_j + 2F'_j rho'_i)), plus p_i p_i^T / m_i with the
+// momenta after the step; components xx yy zz yz zx xy.  It gathers rec_b,
+// which still holds the (u, 2F') records the step's force loop read (the
+// integration rewrites rec_a only), on the generic knot tables.  Launched
+// only on steps whose stress is read.  One thread per slot in
+// IMDG_BLOCK-thread blocks (one replica each); each block also writes the
+// sums of its atoms' six components to spart (FP64).
+template <typename R>
+__global__ void __launch_bounds__(IMDG_BLOCK)
+    k_stress(StepArgs<R> a, const TabDev<R>* __restrict__ tabs,
+             double* __restrict__ stress, double* __restrict__ spart) {
+  typedef typename V4<R>::T R4;
+  typedef typename V2<R>::T R2;
+  __shared__ R4 off[IMDG_NCODES];
+  __shared__ double red[IMDG_NWAVE][6];
+  if (a.ctl->frozen) return;
+  const int b = blockIdx.x;
+  const RepDev& Rp = a.rep[a.blk_rep[b]];
+  if (threadIdx.x < IMDG_NCODES) {
+    R4 o;
+    o.x = Rp.offs[threadIdx.x][0];
+    o.y = Rp.offs[threadIdx.x][1];
+    o.z = Rp.offs[threadIdx.x][2];
+    o.w = 0;
+    off[threadIdx.x] = o;
+  }
+  __syncthreads();
+  const TabDev<R>& tp = tabs[0];
+  const TabDev<R>& tr = tabs[1];
+  const bool hp = a.has_pair != 0, he = a.has_eam != 0;
+  const int nt = a.ntypes;
+  const R2* __restrict__ K = a.knots;
+  const R4* __restrict__ rec = a.rec_b;
+  const int i = b * IMDG_BLOCK + threadIdx.x;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  if (i - Rp.atom_off < Rp.natoms) {
+    const R4 ui = rec[i];
+    const int it = a.atype[i];
+    const R dFi = he ? ui.w : (R)0;
+    R t[6] = {0, 0, 0, 0, 0, 0};
+    const uint4* lw = list_of(a.nbr, i, a.kmax);
+    const int ngroup = (a.nnb[i] + IMDG_GROUP - 1) / IMDG_GROUP;
+    for (int gq = 0; gq < ngroup; gq++) {
+      uint4 e4 = lw[(size_t)gq * IMDG_WAVE];
+      uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        // entries past the list's end are padding: a far-away image,
+        // outside every table
+        const int j = en[u] & IMDG_IDX_MASK;
+        const R4 uj = rec[j];
+        const R4 o = off[en[u] >> IMDG_IDX_BITS];
+        const R dx = (uj.x - ui.x) + o.x;
+        const R dy = (uj.y - ui.y) + o.y;
+        const R dz = (uj.z - ui.z) + o.z;
+        const R r2 = dx * dx + dy * dy + dz * dz;
+        const int jt = a.atype[j];
+        const int col = it * nt + jt, colT = jt * nt + it;
+        R g = (R)0;
+        if (hp && r2 <= tp.end[col]) {
+          R chi;
+          int k = seg_locate(r2, tp.begin[col], tp.end[col], tp.invstep[col],
+                             tp.len[col], chi);
+          int k0 = tp.off[col] + k;
+          g += knot_grad<R>(K[k0], K[k0 + 1], chi, tp.twoistep[col]);
+        }
+        if (he && (r2 < tr.end[colT] || r2 < tr.end[col])) {
+          // rho'_i from column jt*nt+it, rho'_j from it*nt+jt
+          R chi;
+          int k = seg_locate(r2, tr.begin[colT], tr.end[colT],
+                             tr.invstep[colT], tr.len[colT], chi);
+          int k0 = tr.off[colT] + k;
+          R rpi = knot_grad<R>(K[k0], K[k0 + 1], chi, tr.twoistep[colT]);
+          R rpj = rpi;
+          if (colT != col) {
+            k = seg_locate(r2, tr.begin[col], tr.end[col], tr.invstep[col],
+                           tr.len[col], chi);
+            k0 = tr.off[col] + k;
+            rpj = knot_grad<R>(K[k0], K[k0 + 1], chi, tr.twoistep[col]);
+          }
+          g += (R)0.5 * (dFi * rpj + uj.w * rpi);
+        }
+        t[0] = fma(dx * dx, g, t[0]);
+        t[1] = fma(dy * dy, g, t[1]);
+        t[2] = fma(dz * dz, g, t[2]);
+        t[3] = fma(dy * dz, g, t[3]);
+        t[4] = fma(dz * dx, g, t[4]);
+        t[5] = fma(dx * dy, g, t[5]);
+      }
+    }
+    const double4 p = a.mom[a.ctl->pos_cur][i];
+    const double m = p.w;
+    s[0] = -0.5 * (double)t[0] + p.x * p.x / m;
+    s[1] = -0.5 * (double)t[1] + p.y * p.y / m;
+    s[2] = -0.5 * (double)t[2] + p.z * p.z / m;
+    s[3] = -0.5 * (double)t[3] + p.y * p.z / m;
+    s[4] = -0.5 * (double)t[4] + p.z * p.x / m;
+    s[5] = -0.5 * (double)t[5] + p.x * p.y / m;
+#pragma unroll
+    for (int q = 0; q < 6; q++) stress[(size_t)i * 6 + q] = s[q];
+  }
+  const int w = threadIdx.x / IMDG_WAVE;
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    double v = wave_sum(s[q]);
+    if ((threadIdx.x & (IMDG_WAVE - 1)) == 0) red[w][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double v = 0.0;
+    for (int k = 0; k < IMDG_NWAVE; k++) v += red[k][threadIdx.x];
+    spart[(size_t)b * 6 + threadIdx.x] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launch plans and launchers
 // ---------------------------------------------------------------------------
@@ -1440,6 +1559,17 @@ hipError_t launch_copyrec(const StepArgs<R>& a, int nslots, hipStream_t st) {
   return hipGetLastError();
 }
 
+// nblk: IMDG_BLOCK-slot blocks of all replicas; stress holds 6 doubles per
+// slot, spart 6 per block.
+template <typename R>
+hipError_t launch_stress(const StepArgs<R>& a, const TabDev<R>* tabs,
+                         double* stress, double* spart, int nblk,
+                         hipStream_t st) {
+  hipLaunchKernelGGL((k_stress<R>), dim3(nblk), dim3(IMDG_BLOCK), 0, st, a,
+                     tabs, stress, spart);
+  return hipGetLastError();
+}
+
 #define IMDG_INSTANTIATE(R)                                                   \
   template int step_grid_density<R>(bool, bool, size_t, int, int, int);      \
   template int step_grid_force<R>(int, bool, bool, bool, bool, size_t, int,  \
@@ -1453,6 +1583,8 @@ hipError_t launch_copyrec(const StepArgs<R>& a, int nslots, hipStream_t st) {
                                       int, bool, int, hipStream_t);           \
   template hipError_t launch_move<R>(const StepArgs<R>&, int, hipStream_t);   \
   template hipError_t launch_copyrec<R>(const StepArgs<R>&, int, hipStream_t); \
+  template hipError_t launch_stress<R>(const StepArgs<R>&, const TabDev<R>*,  \
+                                       double*, double*, int, hipStream_t);   \
   template hipError_t launch_finalize<R>(const StepArgs<R>&, bool, bool,     \
                                          hipStream_t);
 

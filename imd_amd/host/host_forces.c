@@ -9,7 +9,8 @@
  * nbl_count); the work runs in libimdg through include/imdg.h.
  * main_loop runs IMD's step sequence (src/imd_main_3d.c:155-870:
  * calc_forces, move_atoms, then per step the checkpoint every checkpt_int
- * steps, the .eng row every eng_int steps, the "write" file test every
+ * steps, the .eng row every eng_int steps, the .press file every press_int
+ * steps, the "write" file test every
  * watch_int steps, the "stop" file test every stop_int steps and the
  * maxwalltime test (src/imd_main_3d.c:690-815, 1892-1965), check_nblist)
  * with the steps between two such output steps fused on the device; the
@@ -94,6 +95,7 @@ int imd_gpu_init(void) {
            "threads\n", t.density_lanes, t.density_threads, t.force_lanes,
            t.force_threads);
   }
+  if (stress_tens) check(imdg_set_stress(ctx, 1), "imdg_set_stress");
   move_atoms = (ensemble == IMDG_NVT) ? move_atoms_nvt : move_atoms_nve;
   return 0;
 }
@@ -203,11 +205,21 @@ static int check_walltime(int steps) {
    tot_pot_energy, tot_kin_energy, virial and eta of that step; returns 1
    when the run is to end */
 static int step_outputs(int steps) {
+  int eng = eng_int > 0 && steps % eng_int == 0;
+  int press = press_int > 0 && steps % press_int == 0;
   if (checkpt_int > 0 && steps % checkpt_int == 0) {
     fetch_atoms();
     write_config(steps / checkpt_int, steps);
   }
-  if (eng_int > 0 && steps % eng_int == 0) write_eng_file(steps);
+  if (stress_tens && eng)
+    check(imdg_get_stress(ctx, NULL, tot_presstens), "imdg_get_stress");
+  if (eng) write_eng_file(steps);
+  if (press) {
+    /* src/imd_main_3d.c:733-738 */
+    fetch_atoms();
+    check(imdg_get_stress(ctx, atom_press, tot_presstens), "imdg_get_stress");
+    write_config_press(steps / press_int);
+  }
   if (watch_int > 0 && steps % watch_int == 0) check_write(steps);
   if (stop_int > 0 && steps % stop_int == 0 && check_stop(steps)) return 1;
   if (maxwalltime > 0 && check_walltime(steps)) return 1;
@@ -238,8 +250,7 @@ int main_loop(void) {
       check_nblist();
     }
   } else {
-    imdg_thermo *hist = malloc(sizeof(imdg_thermo) * 2048);
-    if (!hist) imd_error("cannot allocate the thermo buffer");
+    imdg_thermo hist[1];
     int steps = steps_min;
     long done = 0;
     {
@@ -254,30 +265,25 @@ int main_loop(void) {
       last = next_multiple(steps, checkpt_int, last);
       last = next_multiple(steps, watch_int, last);
       last = next_multiple(steps, stop_int, last);
+      last = next_multiple(steps, press_int, last);
       if (last - steps + 1 > 2048) last = steps + 2047;
       long n = last - steps + 1;
       check(imdg_run(ctx, n), "imdg_run");
       check(imdg_sync(ctx), "imdg_sync");
-      check(imdg_get_thermo_history(ctx, done, n, hist), "thermo history");
-      for (long k = 0; k < n; k++) {
-        tot_pot_energy = hist[k].epot;
-        tot_kin_energy = hist[k].ekin;
-        virial = hist[k].virial;
-        eta = hist[k].eta;
-        int s = steps + (int)k;
-        /* only the eng interval can fall inside a fused run */
-        if (k < n - 1) {
-          if (eng_int > 0 && s % eng_int == 0) write_eng_file(s);
-        } else if (step_outputs(s)) {
-          finished = 1;
-        }
-      }
+      /* every output interval ends a fused run: the outputs all belong to
+         its last step */
+      check(imdg_get_thermo_history(ctx, done + n - 1, 1, hist),
+            "thermo history");
+      tot_pot_energy = hist[0].epot;
+      tot_kin_energy = hist[0].ekin;
+      virial = hist[0].virial;
+      eta = hist[0].eta;
+      if (step_outputs(last)) finished = 1;
       done += n;
       done_steps += n;
       steps = last + 1;
       if (finished) break;
     }
-    free(hist);
   }
   double el = wall() - t0;
   imdg_info info;

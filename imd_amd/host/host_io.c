@@ -15,8 +15,9 @@
  *   in input order.
  * write_eng_file: one line per eng_int steps with time, Epot per atom,
  *   temperature 2 Ekin / nactive, pressure T/(V/N) + virial/(3V), volume per
- *   atom and, for NVT, eta * tau_eta (src/imd_io.c:2262-2431); the header
- *   is written when the file is started, rows are appended.
+ *   atom, for NVT eta * tau_eta, and with the stress tensor on the six
+ *   components of tot_presstens / volume (src/imd_io.c:2262-2431); the
+ *   header is written when the file is started, rows are appended.
  * write_config(n, steps): the checkpoint <outfiles>.%05d.chkpt (n >= 0),
  *   <outfiles>-final.chkpt (n == -1) or <outfiles>-interm.chkpt (n == -2)
  *   with IMD's header (#F A 1 1 1 3 3 nd, #C, #X/#Y/#Z, ##PBC, ## lines,
@@ -24,6 +25,8 @@
  *   Epot [eam_rho]" (src/imd_io_3d.c:696-948), then the iteration file
  *   (src/imd_io.c:171-181).  Numbers are written with 17 significant
  *   digits, so a restart reads back the same doubles.
+ * write_config_press(n): <outfiles>.%05d.press with rows "number type mass
+ *   x y z P_xx P_yy P_zz P_yz P_zx P_xy" (src/imd_io.c:1047-1240).
  * write_itr_file(n, steps, ""): <outfiles>.%05d.itr, -final.itr or
  *   -interm.itr with "# checkpoint n", startstep = steps + 1, the box
  *   vectors and, for NVT with tau_eta, starttemp and eta
@@ -41,13 +44,16 @@
 long natoms = 0;
 int nactive = 0;
 double *atom_x = NULL, *atom_p = NULL, *atom_mass = NULL;
-double *atom_epot = NULL, *atom_rho = NULL;
+double *atom_epot = NULL, *atom_rho = NULL, *atom_press = NULL;
+double tot_presstens[6];
 int *atom_type = NULL;
 long *atom_number = NULL;
 double tot_pot_energy = 0.0, tot_kin_energy = 0.0, virial = 0.0, volume = 0.0;
 int nbl_count = 0;
 
 static FILE *eng_file = NULL;
+
+static void write_box_header(FILE *f, int steps);
 
 static int grow_atoms(long cap) {
   atom_x = realloc(atom_x, sizeof(double) * 3 * cap);
@@ -57,8 +63,9 @@ static int grow_atoms(long cap) {
   atom_number = realloc(atom_number, sizeof(long) * cap);
   atom_epot = realloc(atom_epot, sizeof(double) * cap);
   atom_rho = realloc(atom_rho, sizeof(double) * cap);
+  atom_press = realloc(atom_press, sizeof(double) * 6 * cap);
   return (atom_x && atom_p && atom_mass && atom_type && atom_number &&
-          atom_epot && atom_rho) ? 0 : 1;
+          atom_epot && atom_rho && atom_press) ? 0 : 1;
 }
 
 /* the volume of make_box (src/imd_geom_3d.c:52-104) */
@@ -253,17 +260,10 @@ int write_config(int fzhlr, int steps) {
   FILE *f = fopen(fname, "w");
   if (!f) imd_error("Cannot open output file %s", fname);
   int nd = have_eam ? 2 : 1;
-  time_t now = time(NULL);
   fprintf(f, "#F A 1 1 1 3 3 %d\n", nd);
   fprintf(f, "#C number type mass x y z vx vy vz Epot%s\n",
           have_eam ? " eam_rho" : "");
-  fprintf(f, "#X \t%.16e %.16e %.16e\n", box_x[0], box_x[1], box_x[2]);
-  fprintf(f, "#Y \t%.16e %.16e %.16e\n", box_y[0], box_y[1], box_y[2]);
-  fprintf(f, "#Z \t%.16e %.16e %.16e\n", box_z[0], box_z[1], box_z[2]);
-  fprintf(f, "##PBC %d %d %d\n", pbc_dirs[0], pbc_dirs[1], pbc_dirs[2]);
-  fprintf(f, "## Generated on %s", ctime(&now));
-  fprintf(f, "## by %s after step %d\n", progname, steps);
-  fprintf(f, "#E\n");
+  write_box_header(f, steps);
   for (long i = 0; i < natoms; i++) {
     double m = atom_mass[i];
     fprintf(f, "%ld %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g",
@@ -278,13 +278,46 @@ int write_config(int fzhlr, int steps) {
   return 0;
 }
 
+/* the header lines after #F and #C of IMD's configuration-like files */
+static void write_box_header(FILE *f, int steps) {
+  time_t now = time(NULL);
+  fprintf(f, "#X \t%.16e %.16e %.16e\n", box_x[0], box_x[1], box_x[2]);
+  fprintf(f, "#Y \t%.16e %.16e %.16e\n", box_y[0], box_y[1], box_y[2]);
+  fprintf(f, "#Z \t%.16e %.16e %.16e\n", box_z[0], box_z[1], box_z[2]);
+  fprintf(f, "##PBC %d %d %d\n", pbc_dirs[0], pbc_dirs[1], pbc_dirs[2]);
+  fprintf(f, "## Generated on %s", ctime(&now));
+  fprintf(f, "## by %s after step %d\n", progname, steps);
+  fprintf(f, "#E\n");
+}
+
+int write_config_press(int nr) {
+  char fname[IMD_STRLEN + 32];
+  snprintf(fname, sizeof fname, "%s.%05d.press", outfilename, nr);
+  FILE *f = fopen(fname, "w");
+  if (!f) imd_error("Cannot open output file %s", fname);
+  fprintf(f, "#F A 1 1 1 3 0 6\n");
+  fprintf(f, "#C number type mass x y z P_xx P_yy P_zz P_yz P_zx P_xy\n");
+  write_box_header(f, nr * press_int);
+  for (long i = 0; i < natoms; i++) {
+    const double *P = atom_press + 6 * i;
+    fprintf(f, "%ld %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g "
+            "%.17g\n", atom_number[i], atom_type[i], atom_mass[i],
+            atom_x[3 * i], atom_x[3 * i + 1], atom_x[3 * i + 2], P[0], P[1],
+            P[2], P[3], P[4], P[5]);
+  }
+  if (fclose(f)) imd_error("Cannot write output file %s", fname);
+  return 0;
+}
+
 int write_eng_file_header(void) {
   char fname[IMD_STRLEN + 8];
   snprintf(fname, sizeof fname, "%s.eng", outfilename);
   FILE *fl = fopen(fname, "w");
   if (!fl) imd_error("Cannot open properties file.");
-  fprintf(fl, "# time Epot temperature pressure volume%s\n",
-          ensemble == IMDG_NVT ? " eta*tau_eta" : "");
+  fprintf(fl, "# time Epot temperature pressure volume%s%s\n",
+          ensemble == IMDG_NVT ? " eta*tau_eta" : "",
+          stress_tens ? " Press_xx Press_yy Press_zz Press_yz Press_xz Press_xy"
+                      : "");
   fclose(fl);
   return 0;
 }
@@ -303,6 +336,9 @@ int write_eng_file(int steps) {
   fprintf(eng_file, "%e %.18e %e %e %e", steps * timestep, Epot, Temp,
           pressure, vol);
   if (ensemble == IMDG_NVT) fprintf(eng_file, " %e", eta * tau_eta);
+  if (stress_tens)
+    for (int k = 0; k < 6; k++)
+      fprintf(eng_file, " %e", tot_presstens[k] / volume);
   fprintf(eng_file, "\n");
   fflush(eng_file);
   return 0;

@@ -41,6 +41,7 @@ double eta = 0.0, tau_eta = 0.0, isq_tau_eta = 0.0;
 double nbl_margin = 0.4, nbl_size = 1.1; /* src/globals.h:419-423 */
 int steps_min = 0, steps_max = 0, eng_int = 0, checkpt_int = 0;
 int watch_int = 0, stop_int = 0;
+int press_int = 0, stress_tens = 0;
 int imdrestart = 0, simulation = 1;
 int do_maxwell = 0, box_from_header = 0;
 long seed = 0;
@@ -91,6 +92,7 @@ void init_parameters(void) {
   nbl_size = 1.1;
   steps_min = steps_max = eng_int = checkpt_int = 0;
   watch_int = stop_int = 0;
+  press_int = stress_tens = 0;
   do_maxwell = box_from_header = 0;
   seed = 0;
   maxwalltime = 0.0;
@@ -180,6 +182,13 @@ static int getparamfile(const char *paramfile, int phase) {
       get_ints(key, &watch_int, 1);
     } else if (!strcasecmp(key, "stop_int")) {
       get_ints(key, &stop_int, 1);
+    } else if (!strcasecmp(key, "press_int")) {
+      /* .press files of the per-atom stress tensor (src/imd_param.c:2224) */
+      get_ints(key, &press_int, 1);
+    } else if (!strcasecmp(key, "stress_tens")) {
+      /* what IMD's STRESS_TENS build option turns on: the stress tensor on
+         output steps and its six columns in the .eng file */
+      get_ints(key, &stress_tens, 1);
     } else if (!strcasecmp(key, "itrname")) {
       get_str(key, itrfilename);
     } else if (!strcasecmp(key, "coordname")) {
@@ -368,6 +377,7 @@ int check_parameters_complete(void) {
     imd_error("EAM needs embedding_energy_file and atomic_e-density_file");
   if (ensemble == IMDG_NVT && isq_tau_eta == 0.0)
     imd_warning("NVT without tau_eta: eta stays constant");
+  if (press_int > 0) stress_tens = 1;
   for (int t = 0; t < ntypes; t++)
     if (!(masses[t] > 0.0) && !infilename[0])
       imd_error("masses are missing");

@@ -48,6 +48,7 @@ extern double eta, tau_eta, isq_tau_eta;
 extern double nbl_margin, nbl_size;
 extern int steps_min, steps_max, eng_int, checkpt_int;
 extern int watch_int, stop_int;
+extern int press_int, stress_tens;  /* .press interval; STRESS_TENS on */
 extern int imdrestart, simulation;  /* -r n; current phase */
 extern int do_maxwell, box_from_header;
 extern long seed;
@@ -68,6 +69,8 @@ extern long natoms;
 extern int nactive;
 extern double *atom_x, *atom_p, *atom_mass;
 extern double *atom_epot, *atom_rho; /* per-atom Epot and eam_rho */
+extern double *atom_press;           /* 6 per atom: xx yy zz yz zx xy */
+extern double tot_presstens[6];      /* sum over the atoms */
 extern int *atom_type;
 extern long *atom_number;
 
@@ -105,6 +108,8 @@ void maxwell(double temp);
    with its iteration file */
 int  write_config(int fzhlr, int steps);
 int  write_itr_file(int fzhlr, int steps, const char *suffix);
+/* <outfiles>.%05d.press: per-atom stress tensor (atom_press) */
+int  write_config_press(int nr);
 int  write_eng_file_header(void);
 int  write_eng_file(int steps);
 

@@ -29,7 +29,7 @@ SYMBOLS = [
     "imdg_get_thermo", "imdg_get_thermo_history", "imdg_get_cells",
     "imdg_get_pairs", "imdg_get_grid", "imdg_get_info", "imdg_profile",
     "imdg_kernel_times", "imdg_struct_size", "imdg_get_tuning",
-    "imdg_set_tuning", "imdg_autotune",
+    "imdg_set_tuning", "imdg_autotune", "imdg_set_stress", "imdg_get_stress",
 ]
 
 
@@ -112,6 +112,8 @@ def lib():
         L.imdg_get_cells.argtypes = [vp, ip]
         L.imdg_get_pairs.argtypes = [vp, lp, C.c_long, lp]
         L.imdg_get_grid.argtypes = [vp, C.c_int, ip]
+        L.imdg_set_stress.argtypes = [vp, C.c_int]
+        L.imdg_get_stress.argtypes = [vp, dp, dp]
         L.imdg_get_info.argtypes = [vp, C.POINTER(Info)]
         L.imdg_profile.argtypes = [vp, C.c_int]
         L.imdg_kernel_times.argtypes = [vp, C.POINTER(C.c_char_p), dp, lp,
@@ -337,6 +339,19 @@ class Context:
         g = (C.c_int * 3)()
         self._chk(lib().imdg_get_grid(self._h, rep, g), "imdg_get_grid")
         return tuple(g)
+
+    def set_stress(self, enable=True):
+        self._chk(lib().imdg_set_stress(self._h, int(enable)),
+                  "imdg_set_stress")
+
+    def stress(self):
+        """Per-atom stress tensor (n, 6) in input order and the per-replica
+        sums (nrep, 6); components xx yy zz yz zx xy."""
+        per = np.zeros((sum(self.natoms), 6))
+        tot = np.zeros((self.nrep, 6))
+        self._chk(lib().imdg_get_stress(self._h, _dp(per), _dp(tot)),
+                  "imdg_get_stress")
+        return per, tot
 
     def info(self):
         i = Info()

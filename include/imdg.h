@@ -186,6 +186,21 @@ int  imdg_get_cells(imdg_ctx *ctx, int *cell3);
 int  imdg_get_pairs(imdg_ctx *ctx, long *out, long cap, long *npairs);
 /* Grid dimensions (3 ints) of a replica. */
 int  imdg_get_grid(imdg_ctx *ctx, int rep, int *gdim3);
+
+/* Per-atom stress tensor, IMD's STRESS_TENS with do_press_calc on the
+   steps whose tensor is read (PRESSTENS: src/imd_forces_nbl.c:346-352,
+   558-581, 1283-1318; src/imd_integrate.c:411-431, 1085-1094; summed over
+   the atoms like calc_tot_presstens, src/imd_main_3d.c:2069-2100).  While
+   enabled, the tensor of the last step of every imdg_run and of every
+   imdg_move_atoms is computed after that step: per atom -1/2 sum_j
+   (d d^T) g over its neighbours, g the pair's force/distance factor, plus
+   p p^T / m with the momenta after the step.  Energy units (divide the sum
+   by the volume for a pressure tensor). */
+int  imdg_set_stress(imdg_ctx *ctx, int enable);
+/* 6 doubles per atom (input order) and 6 per replica (the sums), in the
+   order xx yy zz yz zx xy; either pointer may be NULL.  IMDG_ERR_STATE if
+   no tensor of the current state was computed. */
+int  imdg_get_stress(imdg_ctx *ctx, double *per_atom, double *total);
 int  imdg_get_info(imdg_ctx *ctx, imdg_info *info);
 
 /* Kernel timing with HIP events on the context's stream.  enable=1 starts

@@ -70,6 +70,8 @@ def lib():
         L.orc_get_scalar.restype = C.c_double
         L.orc_get_grid.argtypes = [C.c_void_p, ip]
         L.orc_get_cells.argtypes = [C.c_void_p, ip]
+        L.orc_set_press.argtypes = [C.c_void_p, C.c_int]
+        L.orc_get_press.argtypes = [C.c_void_p, dp]
         L.orc_get_pairs.argtypes = [C.c_void_p, lp, C.c_long]
         L.orc_get_pairs.restype = C.c_long
         _lib = L
@@ -210,6 +212,15 @@ class OracleSystem:
         lib().orc_get_atoms(self._s, _dp(out["x"]), _dp(out["p"]),
                             _dp(out["f"]), _dp(out["epot"]), _dp(out["rho"]),
                             _dp(out["df"]))
+        return out
+
+    def set_press(self, on=True):
+        lib().orc_set_press(self._s, int(on))
+
+    def press(self):
+        """Per-atom stress tensor (n, 6): xx yy zz yz zx xy."""
+        out = np.zeros((self.n, 6))
+        lib().orc_get_press(self._s, _dp(out))
         return out
 
     def scalar(self, which):

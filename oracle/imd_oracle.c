@@ -275,6 +275,9 @@ struct orc_sys {
   int *stype, *ssrc, *sroot, *sshift; /* sshift: 3 per slot */
   double *svec; /* shift vector added when the slot was copied, 3 per slot */
   double *sf, *sepot, *srho, *sdf;
+  /* per-slot stress tensor xx yy zz yz zx xy, when do_press is set */
+  double *spress;
+  int do_press;
   /* cells */
   ivec *cells;
   int *inner;      /* inner cell indices, IMD order */
@@ -395,6 +398,7 @@ void orc_free(orc_sys *s) {
   free(s->svec);
   free(s->sf);
   free(s->sepot);
+  free(s->spress);
   free(s->srho);
   free(s->sdf);
   free(s->tl);
@@ -520,6 +524,7 @@ static int slot_reserve(orc_sys *s, int need) {
   GROW(s->sepot, double, 1);
   GROW(s->srho, double, 1);
   GROW(s->sdf, double, 1);
+  GROW(s->spress, double, 6);
 #undef GROW
   s->slotcap = nc;
   return ORC_OK;
@@ -670,6 +675,39 @@ static int make_nblist(orc_sys *s) {
 }
 
 /* calc_forces, PAIR + EAM2 branches (src/imd_forces_nbl.c:281-1323) */
+/* STRESS_TENS of one pair (src/imd_forces_nbl.c:558-581, 1283-1318): half
+   of the pair force, h = fc / 2, gives -d_a h_b to the neighbour's tensor
+   and to the atom's running sum ps, in the order xx yy zz yz zx xy. */
+static void press_pair(double *pj, double *ps, const double *d,
+                       const double *fc) {
+  double h[3] = {fc[0] * 0.5, fc[1] * 0.5, fc[2] * 0.5};
+  double t[6] = {d[0] * h[0], d[1] * h[1], d[2] * h[2],
+                 d[1] * h[2], d[2] * h[0], d[0] * h[1]};
+  for (int k = 0; k < 6; k++) {
+    ps[k] -= t[k];
+    pj[k] -= t[k];
+  }
+}
+
+/* the kinetic part p (x) p / m with the new momenta
+   (src/imd_integrate.c:411-431, 1085-1094) */
+static void press_kin(double *pi, const double *p, double m) {
+  pi[0] += p[0] * p[0] / m;
+  pi[1] += p[1] * p[1] / m;
+  pi[2] += p[2] * p[2] / m;
+  pi[3] += p[1] * p[2] / m;
+  pi[4] += p[2] * p[0] / m;
+  pi[5] += p[0] * p[1] / m;
+}
+
+void orc_set_press(orc_sys *s, int on) { s->do_press = on; }
+
+void orc_get_press(const orc_sys *s, double *out) {
+  for (int i = 0; i < s->n; i++)
+    for (int k = 0; k < 6; k++)
+      out[6 * i + k] = s->spress ? s->spress[6 * i + k] : 0.0;
+}
+
 int orc_calc_forces(orc_sys *s, int steps) {
   (void)steps;
   int rc, nt = s->ntypes, inc = nt * nt;
@@ -687,6 +725,8 @@ int orc_calc_forces(orc_sys *s, int steps) {
     s->sf[3 * q] = s->sf[3 * q + 1] = s->sf[3 * q + 2] = 0.0;
     s->sepot[q] = 0.0;
     s->srho[q] = 0.0;
+    if (s->do_press)
+      for (int k = 0; k < 6; k++) s->spress[6 * q + k] = 0.0;
   }
   const orc_table *pp = s->pair, *rt = s->rho_t, *et = s->embed;
   int is_short = 0;
@@ -695,6 +735,7 @@ int orc_calc_forces(orc_sys *s, int steps) {
     int ia = s->tl_atom[n];
     const double *d1 = s->sx + 3 * ia;
     double ff[3] = {0, 0, 0}, ee = 0.0, eam_r = 0.0;
+    double ps[6] = {0, 0, 0, 0, 0, 0};
     int it = s->stype[ia];
     for (int m = s->tl[n]; m < s->tl[n + 1]; m++) {
       int js = s->tb[m];
@@ -718,6 +759,7 @@ int orc_calc_forces(orc_sys *s, int steps) {
         s->sepot[js] += pot;
         s->virial -= r2 * grad;
         s->virial_abs += fabs(r2 * grad);
+        if (s->do_press) press_pair(s->spress + 6 * js, ps, d, fc);
       }
       if (rt) {
         double rho_h = 0.0;
@@ -738,6 +780,8 @@ int orc_calc_forces(orc_sys *s, int steps) {
     s->sf[3 * ia + 2] += ff[2];
     s->sepot[ia] += ee;
     s->srho[ia] += eam_r;
+    if (s->do_press)
+      for (int k = 0; k < 6; k++) s->spress[6 * ia + k] += ps[k];
   }
   if (rt) {
     /* add_rho: buffer copies back to their sources (x, then y, then z) */
@@ -762,6 +806,7 @@ int orc_calc_forces(orc_sys *s, int steps) {
       int ia = s->tl_atom[n];
       const double *d1 = s->sx + 3 * ia;
       double ff[3] = {0, 0, 0};
+      double ps[6] = {0, 0, 0, 0, 0, 0};
       int it = s->stype[ia];
       for (int m = s->tl[n]; m < s->tl[n + 1]; m++) {
         int js = s->tb[m];
@@ -784,11 +829,14 @@ int orc_calc_forces(orc_sys *s, int steps) {
           ff[2] += fc[2];
           s->virial -= d[0] * fc[0] + d[1] * fc[1] + d[2] * fc[2];
           s->virial_abs += fabs(d[0] * fc[0] + d[1] * fc[1] + d[2] * fc[2]);
+          if (s->do_press) press_pair(s->spress + 6 * js, ps, d, fc);
         }
       }
       s->sf[3 * ia] += ff[0];
       s->sf[3 * ia + 1] += ff[1];
       s->sf[3 * ia + 2] += ff[2];
+      if (s->do_press)
+        for (int k = 0; k < 6; k++) s->spress[6 * ia + k] += ps[k];
     }
   }
   /* add_forces: buffer copies back to their sources (x, then y, then z) */
@@ -798,6 +846,9 @@ int orc_calc_forces(orc_sys *s, int steps) {
     s->sf[3 * src + 1] += s->sf[3 * q + 1];
     s->sf[3 * src + 2] += s->sf[3 * q + 2];
     s->sepot[src] += s->sepot[q];
+    /* add_forces carries the stress too (src/imd_comm_force_3d.c:916-921) */
+    if (s->do_press)
+      for (int k = 0; k < 6; k++) s->spress[6 * src + k] += s->spress[6 * q + k];
   }
   s->is_short |= is_short;
   return ORC_OK;
@@ -824,6 +875,7 @@ void orc_move_atoms(orc_sys *s) {
         x[0] += tmp * p[0];
         x[1] += tmp * p[1];
         x[2] += tmp * p[2];
+        if (s->do_press) press_kin(s->spress + 6 * i, p, s->mass[i]);
       }
     }
   } else {
@@ -844,6 +896,7 @@ void orc_move_atoms(orc_sys *s) {
         x[0] += tmp * p[0];
         x[1] += tmp * p[1];
         x[2] += tmp * p[2];
+        if (s->do_press) press_kin(s->spress + 6 * i, p, s->mass[i]);
       }
     }
     s->tot_kin_energy = (e1 + e2) / 4.0;

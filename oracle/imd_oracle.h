@@ -99,6 +99,12 @@ int  orc_set_atoms(orc_sys *s, int n, const long *id, const int *type,
                    const double *mass, const double *x, const double *p);
 
 int  orc_calc_forces(orc_sys *s, int steps);
+/* do_press_calc: per-atom stress tensor in calc_forces and move_atoms
+   (STRESS_TENS, src/imd_forces_nbl.c:346-352, 558-581, 1283-1318;
+   src/imd_integrate.c:411-431, 1085-1094); orc_get_press gives 6 per atom
+   (xx yy zz yz zx xy, PRESSTENS), caller order */
+void orc_set_press(orc_sys *s, int on);
+void orc_get_press(const orc_sys *s, double *out);
 void orc_move_atoms(orc_sys *s);
 void orc_check_nblist(orc_sys *s);
 void orc_invalidate_nblist(orc_sys *s);

@@ -251,3 +251,37 @@ def test_phases_nvt_then_nve(tmp_path):
         step = int(round(row[0] / W.FS))
         ep = (th1[step, 0] if step <= 20 else th2[step - 21, 0]) / s.n
         assert abs(row[1] - ep) <= 1e-9 * abs(ep), (step, row[1], ep)
+
+
+def test_stress_outputs_match_oracle(tmp_path):
+    """press_int 10: the .eng rows carry tot_presstens / volume (6 columns,
+    IMD's STRESS_TENS layout) and run.%05d.press the per-atom tensors, both
+    matching the oracle's PRESSTENS at the same steps (FP64)."""
+    eam, s, files, cfg = _setup(tmp_path)
+    _run(tmp_path, _lines(s, files, cfg, "run", "nve", 20, ["press_int 10"]))
+    hdr = open(str(tmp_path / "run.eng")).readline().split()
+    assert hdr[-6:] == ["Press_xx", "Press_yy", "Press_zz", "Press_yz",
+                        "Press_xz", "Press_xy"]
+    eng = _read_eng(str(tmp_path / "run.eng"))
+    assert eng.shape == (3, 11)
+    orc = oracle_eam(s, eam, nbl_margin=0.4, timestep=W.FS)
+    orc.set_press(True)
+    vol = abs(np.linalg.det(s.box))
+    sums = {}
+    for step in range(21):
+        orc.calc_forces(step)
+        orc.move_atoms()
+        orc.check_nblist()
+        if step % 10 == 0:
+            sums[step] = orc.press().sum(0) / vol
+    scale = max(np.abs(v).max() for v in sums.values())
+    for row in eng:
+        step = int(round(row[0] / W.FS))
+        # printed with %e: 7 significant digits
+        assert np.all(np.abs(row[5:] - sums[step]) <= 2e-6 * scale), step
+    for k in range(3):
+        assert os.path.exists(str(tmp_path / ("run.%05d.press" % k)))
+    rows = _rows(str(tmp_path / "run.00002.press"))
+    ref = orc.press()
+    assert rows.shape == (s.n, 12)
+    assert np.abs(rows[:, 6:] - ref).max() <= 1e-9 * np.abs(ref).max()

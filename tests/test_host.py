@@ -347,3 +347,35 @@ def test_command_line(tmp_path):
                            timeout=60, cwd=str(tmp_path))
         assert r.returncode == 2, r.stderr
         assert "run.00007.itr not found" in r.stderr
+
+
+def test_press_keys_and_file_format(host, tmp_path):
+    """press_int turns the stress tensor on (STRESS_TENS); write_config_press
+    writes <out>.%05d.press with IMD's header and rows "number type mass
+    x y z P_xx P_yy P_zz P_yz P_zx P_xy" (src/imd_io.c:1047-1240)."""
+    out = str(tmp_path / "run")
+    host.init_parameters()
+    host.read_parameters(_param(tmp_path, out, ["press_int 10"]), 1)
+    assert C.c_int.in_dll(host, "press_int").value == 10
+    assert C.c_int.in_dll(host, "stress_tens").value == 1
+    s = W.fcc_system((2, 2, 2), 3.615, [63.546], W.kT(300), 5, displace=0.05)
+    path = str(tmp_path / "c.cfg")
+    s.write_config(path)
+    C.c_int.in_dll(host, "ntypes").value = 1
+    assert host.read_atoms(path.encode()) == 0
+    n = s.n
+    ap = np.ctypeslib.as_array(C.POINTER(C.c_double).in_dll(host, "atom_press"),
+                               (n, 6))
+    ap[:] = np.random.default_rng(4).normal(size=(n, 6)) / 3
+    want = ap.copy()
+    _set_str(host, "outfilename", out)
+    host.write_config_press.argtypes = [C.c_int]
+    assert host.write_config_press(3) == 0
+    lines = open(out + ".00003.press").read().splitlines()
+    assert lines[0] == "#F A 1 1 1 3 0 6"
+    assert lines[1] == "#C number type mass x y z P_xx P_yy P_zz P_yz P_zx P_xy"
+    end = lines.index("#E")
+    rows = np.array([[float(v) for v in l.split()] for l in lines[end + 1:]])
+    assert rows.shape == (n, 12)
+    assert np.array_equal(rows[:, 3:6], s.x)
+    assert np.array_equal(rows[:, 6:], want)

@@ -65,14 +65,19 @@ def analytic(x, types, box, A, B, C, D):
     epot_i = 0.5 * phi.sum(1) + F
     epot = epot_i.sum()
     virial = -0.5 * np.sum(r2 * g)
+    # per-atom stress tensor: -1/2 sum_j g d_a d_b, xx yy zz yz zx xy
+    ab = [(0, 0), (1, 1), (2, 2), (1, 2), (2, 0), (0, 1)]
+    press = np.stack([-0.5 * np.einsum("ij,ij,ij->i", g, d[:, :, a], d[:, :, b])
+                      for a, b in ab], axis=1)
     return dict(f=f, rho=rho, df=2 * Fp, epot_i=epot_i, epot=epot,
-                virial=virial)
+                virial=virial, press=press)
 
 
-def run_oracle(s, tabs, nt):
+def run_oracle(s, tabs, nt, press=False):
     pair, rho, emb = [O.OracleTable.from_table(t) for t in tabs]
     orc = O.OracleSystem(s, pair, rho, emb, nbl_margin=0.3, timestep=0.001,
                          ntypes=nt)
+    orc.set_press(press)
     orc.calc_forces()
     return orc, (pair, rho, emb)
 
@@ -175,3 +180,81 @@ def test_virial_is_volume_derivative():
         es.append(_energy(t, tabs)[0])
     dEdl = (es[0] - es[1]) / (2 * h)
     assert abs(-dEdl - vir) < 1e-5 * abs(vir)
+
+
+# ---------------------------------------------------------------------------
+# per-atom stress tensor (STRESS_TENS)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("nt", [1, 2])
+def test_stress_closed_form(nt):
+    """PRESSTENS after calc_forces: -1/2 sum_j g_ij d_a d_b over all
+    neighbours (src/imd_forces_nbl.c:558-581, 1283-1318, buffer copies added
+    back), against the brute-force closed form; its trace summed over the
+    atoms is the virial."""
+    s = W.fcc_system((4, 4, 4), 3.25, [1.0] * nt, 0.0, 9, displace=0.15,
+                     type_fractions=None if nt == 1 else (0.6, 0.4))
+    A = [0.05, 0.08][:nt]
+    B = [0.02, 0.035][:nt]
+    C = [0.3, 0.2][:nt]
+    D = [-1.0, -0.7][:nt]
+    ref = analytic(s.x, s.type, s.box, A, B, C, D)
+    tabs = poly_tables(nt, A, B, C, D, rho_max=1.3 * ref["rho"].max())
+    orc, _ = run_oracle(s, tabs, nt, press=True)
+    P = orc.press()
+    scale = np.abs(ref["press"]).max()
+    assert np.max(np.abs(P - ref["press"])) < 1e-10 * scale
+    # off-diagonal terms are not small here (displaced lattice)
+    assert np.abs(ref["press"][:, 3:]).max() > 1e-2 * scale
+    tr = P[:, :3].sum()
+    assert abs(tr - orc.scalar("virial")) < 1e-10 * abs(orc.scalar("virial"))
+
+
+def test_stress_is_strain_derivative():
+    """sum_i S_aa = -dE/d eps_aa for the stretch x_a -> (1 + eps) x_a of
+    positions and box (Johnson EAM, finite differences)."""
+    eam = W.johnson_eam()
+    tabs = (eam.pair, eam.rho, eam.embed)
+    s = W.eam_crystal(n_cells=4, temp_k=0.0, seed=6, displace=0.05)
+    orc, _ = run_oracle(s, tabs, 1, press=True)
+    tot = orc.press().sum(0)
+    h = 1e-6
+    for a in range(3):
+        es = []
+        for sign in (1, -1):
+            t = W.System(**{**s.__dict__})
+            t.x = s.x.copy()
+            t.x[:, a] *= 1 + sign * h
+            t.box = s.box.copy()
+            t.box[a, a] *= 1 + sign * h
+            es.append(_energy(t, tabs)[0])
+        dE = (es[0] - es[1]) / (2 * h)
+        assert abs(-dE - tot[a]) < 1e-5 * abs(tot[a]), (a, -dE, tot[a])
+
+
+@pytest.mark.parametrize("ensemble", ["nve", "nvt"])
+def test_stress_kinetic_part(ensemble):
+    """move_atoms adds p (x) p / m with the momenta after the step
+    (src/imd_integrate.c:411-431, 1085-1094)."""
+    eam = W.johnson_eam()
+    s = W.eam_crystal(n_cells=4, temp_k=600.0, seed=8, displace=0.03)
+    pair, rho, emb = [O.OracleTable.from_table(t)
+                      for t in (eam.pair, eam.rho, eam.embed)]
+    kw = {}
+    if ensemble == "nvt":
+        kw = dict(ensemble=O.ORC_NVT, temperature=W.kT(300.0), eta=0.01,
+                  isq_tau_eta=1e-2)
+    orc = O.OracleSystem(s, pair, rho, emb, nbl_margin=0.3, timestep=W.FS,
+                         ntypes=1, **kw)
+    orc.set_press(True)
+    orc.calc_forces()
+    pot = orc.press()
+    orc.move_atoms()
+    p = orc.atoms()["p"]
+    m = s.mass[:, None]
+    kin = np.stack([p[:, 0] * p[:, 0], p[:, 1] * p[:, 1], p[:, 2] * p[:, 2],
+                    p[:, 1] * p[:, 2], p[:, 2] * p[:, 0], p[:, 0] * p[:, 1]],
+                   axis=1) / m
+    assert np.allclose(orc.press() - pot, kin, rtol=1e-12, atol=1e-15)
+    # the diagonal carries 2 Ekin of the new momenta
+    assert abs(kin[:, :3].sum() - (p * p / m).sum()) < 1e-12 * kin[:, :3].sum()
