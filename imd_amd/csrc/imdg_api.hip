@@ -18,10 +18,10 @@
 namespace imdg {
 template <typename R>
 int step_grid_density(bool nt1, bool lt, size_t lds, int ncu, int lpa,
-                      int threads);
+                      int threads, bool cub);
 template <typename R>
 int step_grid_force(int mode, bool integrate, bool lt, bool hp, bool he,
-                    size_t lds, int ncu, int lpa, int threads);
+                    size_t lds, int ncu, int lpa, int threads, bool cub);
 template <typename R>
 int force_threads(int mode);
 template <typename R>
@@ -76,6 +76,7 @@ struct imdg_ctx {
   int dev = 0;
   hipStream_t stream = nullptr;
   bool mixed = false;
+  bool cubic = false;   // 4-point or spline tables (cubic segments)
   int xcd_remap = 1;
   // tables
   HostTable tab[3];  // pair, rho, embed
@@ -241,6 +242,10 @@ int copy_table(imdg_ctx* c, HostTable& h, const imdg_table* t, int ncols,
   h.begin.assign(t->begin, t->begin + ncols);
   h.end.assign(t->end, t->end + ncols);
   h.invstep.assign(t->invstep, t->invstep + ncols);
+  // the spline coefficients use step itself (PAIR_INT_SP reads pt.step)
+  h.step.resize(ncols);
+  for (int k = 0; k < ncols; k++)
+    h.step[k] = t->step ? t->step[k] : 1.0 / t->invstep[k];
   h.len.assign(t->len, t->len + ncols);
   for (int k = 0; k < ncols; k++)
     if (h.len[k] < 3 || h.len[k] > h.maxsteps)
@@ -249,6 +254,74 @@ int copy_table(imdg_ctx* c, HostTable& h, const imdg_table* t, int ncols,
   h.table.assign(t->table, t->table + (size_t)(h.maxsteps + 2) * ncols);
   h.present = true;
   return IMDG_OK;
+}
+
+// Cubic segments k = 0..len-1 of column col for the 4-point or spline
+// interpolation, as (c0, c1, c2, c3) per segment: value c0 + x (c1 + x (c2
+// + x c3)) at fraction x of segment k, in FP64.
+//  * 4-point (PAIR_INT3, src/potaccess.h:365-412): the Lagrange cubic
+//    through rows k-1 .. k+2 at x = -1, 0, 1, 2; segment 0 is segment 1's
+//    cubic continued (the reference reads it with k = 1 and x - 1).  The two
+//    guard rows continue the last cubic (init_fourpoint,
+//    src/imd_potential.c:1171-1187).
+//  * spline (PAIR_INT_SP, src/potaccess.h:414-457): a p_k + b p_k+1 +
+//    (a (a^2-1) y2_k + b (b^2-1) y2_k+1) h^2/6 with b = x, a = 1 - x, and the
+//    second derivatives y2 of init_spline (src/imd_potential.c:1197-1246):
+//    natural at the start; at the end zero slope (radial) or natural.
+// Returns false if the column is too short (4 rows).
+bool cubic_segments(const HostTable& h, int col, bool radial, int interp,
+                    std::vector<double>& seg) {
+  const int nc = h.ncols, n = h.len[col];
+  if (n < 4) return false;
+  std::vector<double> y(n + 2);
+  for (int k = 0; k < n; k++) y[k] = h.table[(size_t)k * nc + col];
+  seg.assign((size_t)4 * n, 0.0);
+  if (interp == IMDG_INTERP_4POINT) {
+    y[n] = 4 * y[n - 1] - 6 * y[n - 2] + 4 * y[n - 3] - y[n - 4];
+    y[n + 1] = 10 * y[n - 1] - 20 * y[n - 2] + 15 * y[n - 3] - 4 * y[n - 4];
+    for (int k = 1; k < n; k++) {
+      double p0 = y[k - 1], p1 = y[k], p2 = y[k + 1], p3 = y[k + 2];
+      double* q = &seg[(size_t)4 * k];
+      q[0] = p1;
+      q[1] = -p0 / 3 - p1 / 2 + p2 - p3 / 6;
+      q[2] = (p0 + p2) / 2 - p1;
+      q[3] = (p3 - p0) / 6 + (p1 - p2) / 2;
+    }
+    // segment 0: P1(x - 1) expanded in x
+    const double* q1 = &seg[4];
+    seg[0] = q1[0] - q1[1] + q1[2] - q1[3];
+    seg[1] = q1[1] - 2 * q1[2] + 3 * q1[3];
+    seg[2] = q1[2] - 3 * q1[3];
+    seg[3] = q1[3];
+    return true;
+  }
+  // spline: the tridiagonal sweep of init_spline, uniform step
+  const double step = h.step[col];
+  std::vector<double> y2(n + 1, 0.0), u(n, 0.0);
+  for (int i = 1; i < n - 1; i++) {
+    double p = 0.5 * y2[i - 1] + 2.0;
+    y2[i] = -0.5 / p;
+    double d = (y[i + 1] - 2 * y[i] + y[i - 1]) / step;
+    u[i] = (6.0 * d / (2 * step) - 0.5 * u[i - 1]) / p;
+  }
+  double qn = 0.0, un = 0.0;
+  if (radial) {
+    qn = 0.5;
+    un = (3.0 / step) * (y[n - 2] - y[n - 1]) / step;
+  }
+  y2[n - 1] = (un - qn * u[n - 2]) / (qn * y2[n - 2] + 1.0);
+  for (int k = n - 2; k >= 0; k--) y2[k] = y2[k] * y2[k + 1] + u[k];
+  y[n] = 2 * y[n - 1] - y[n - 2] + step * step * y2[n - 1];
+  y2[n] = 2 * y2[n - 1] - y2[n - 2];
+  const double h6 = step * step / 6;
+  for (int k = 0; k < n; k++) {
+    double* q = &seg[(size_t)4 * k];
+    q[0] = y[k];
+    q[1] = (y[k + 1] - y[k]) - h6 * (2 * y2[k] + y2[k + 1]);
+    q[2] = 3 * h6 * y2[k];
+    q[3] = h6 * (y2[k + 1] - y2[k]);
+  }
+  return true;
 }
 
 // Knots of IMD's 3-point interpolation: for rows k = 0..len of a column,
@@ -275,6 +348,20 @@ int upload_tables(imdg_ctx* c) {
       td[t].end[col] = (R)h.end[col];
       td[t].invstep[col] = (R)h.invstep[col];
       td[t].twoistep[col] = (R)(2 * h.invstep[col]);
+      if (c->cubic) {
+        std::vector<double> seg;
+        if (!cubic_segments(h, col, t != 2, c->cfg.interpolation, seg))
+          return fail(c, IMDG_ERR_NOTABLE,
+                      "column %d of table %d is too short for the "
+                      "interpolation", col, t);
+        for (size_t k = 0; k < seg.size(); k += 2) {
+          R2 q;
+          q.x = (R)seg[k];
+          q.y = (R)seg[k + 1];
+          kn.push_back(q);
+        }
+        continue;
+      }
       int nc = h.ncols;
       for (int k = 0; k <= h.len[col]; k++) {
         double p0 = h.table[(size_t)k * nc + col];
@@ -303,7 +390,7 @@ int upload_tables(imdg_ctx* c) {
   //         replaces the cutoff tests.
   c->knots_joint = -1;
   const HostTable &hp = c->tab[0], &hr = c->tab[1];
-  bool same = c->cfg.ntypes == 1 && hp.present && hr.present &&
+  bool same = !c->cubic && c->cfg.ntypes == 1 && hp.present && hr.present &&
               hp.begin[0] == hr.begin[0] && hp.end[0] == hr.end[0] &&
               hp.invstep[0] == hr.invstep[0] && hp.len[0] == hr.len[0];
   if (const char* e = getenv("IMDG_JOINT_TABLE"))
@@ -361,7 +448,7 @@ int upload_tables(imdg_ctx* c) {
   // (p0, A, B, 0) (value p0 + chi (A + B chi), A = dv - d2v/2, B = d2v/2),
   // segment len-1 zero, like the force loop's segment table
   c->knots_dpoly = -1;
-  if (sizeof(R) == 4 && c->cfg.ntypes == 1 && hr.present) {
+  if (sizeof(R) == 4 && !c->cubic && c->cfg.ntypes == 1 && hr.present) {
     if (kn.size() & 1) kn.push_back(kn.back());
     c->knots_dpoly = (int)kn.size();
     const int len = hr.len[0];
@@ -404,6 +491,7 @@ int plan_steps(imdg_ctx* c) {
   }
   bool nt1 = c->cfg.ntypes == 1;
   bool hp = c->tab[0].present, he = c->tab[1].present;
+  const bool cub = c->cubic;
   c->force_mode = !nt1 ? 2 : (c->knots_joint >= 0 ? 1 : 0);
   int mode = c->force_mode;
   // default shapes (what imdg_autotune picks on the 2M-atom EAM crystal):
@@ -417,6 +505,7 @@ int plan_steps(imdg_ctx* c) {
   int td = c->thr_den, tf = c->thr_frc;
   int no_lds = 0;
   if (const char* e = getenv("IMDG_TABLES_IN_LDS")) no_lds = (atoi(e) == 0);
+  if (cub) no_lds = 1;  // cubic segments: the generic loops on HBM/L2
   // density: the rho knots
   c->lds_den_lo = c->knots_rho;
   c->lds_den_n = he ? c->knots_embed - c->knots_rho : 0;
@@ -427,11 +516,11 @@ int plan_steps(imdg_ctx* c) {
   c->grid_den = 0;
   if (he && !no_lds)
     c->grid_den = step_grid_density<R>(nt1, true, c->lds_den_n * sizeof(R2),
-                                       c->ncu, c->lpa_den, td);
+                                       c->ncu, c->lpa_den, td, cub);
   if (he && c->grid_den == 0) {
     c->lds_den_n = 0;
     c->grid_den =
-        step_grid_density<R>(nt1, false, 0, c->ncu, c->lpa_den, td);
+        step_grid_density<R>(nt1, false, 0, c->ncu, c->lpa_den, td, cub);
   }
   // force: pair and rho knots (or the interleaved table)
   if (mode == 1) {
@@ -447,14 +536,14 @@ int plan_steps(imdg_ctx* c) {
     if (!no_lds)
       g = step_grid_force<R>(mode, integ, true, hp, he,
                              c->lds_frc_n * sizeof(R2), c->ncu, c->lpa_frc,
-                             tf);
+                             tf, cub);
     if (g == 0) {
       c->lds_frc_n = 0;
       g = step_grid_force<R>(mode, integ, false, hp, he, 0, c->ncu,
-                             c->lpa_frc, tf);
+                             c->lpa_frc, tf, cub);
       if (integ == 1)  // keep both variants on the same table placement
         c->grid_frc[0] = step_grid_force<R>(mode, 0, false, hp, he, 0,
-                                            c->ncu, c->lpa_frc, tf);
+                                            c->ncu, c->lpa_frc, tf, cub);
     }
     c->grid_frc[integ] = g;
   }
@@ -496,6 +585,7 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.knots = (const typename V2<R>::T*)c->d_knots;
   a.joint = c->knots_joint;
   a.dpoly = c->knots_dpoly;
+  a.cubic = c->cubic ? 1 : 0;
   a.final_inline = c->final_inline ? 1 : 0;
   a.lpa_den = c->lpa_den;
   a.lpa_frc = c->lpa_frc;
@@ -941,8 +1031,12 @@ int imdg_ctx_create(const imdg_config* cfg, imdg_ctx** out) {
     return IMDG_ERR_ARG;
   if (cfg->ntypes < 1 || cfg->ntypes > IMDG_MAXTYPES) return IMDG_ERR_ARG;
   if (!(cfg->timestep > 0) || !(cfg->nbl_margin >= 0)) return IMDG_ERR_ARG;
+  if (cfg->interpolation < IMDG_INTERP_3POINT ||
+      cfg->interpolation > IMDG_INTERP_SPLINE)
+    return IMDG_ERR_ARG;
   imdg_ctx* c = new imdg_ctx();
   c->cfg = *cfg;
+  c->cubic = cfg->interpolation != IMDG_INTERP_3POINT;
   if (!(c->cfg.nbl_size > 0)) c->cfg.nbl_size = 1.1;
   c->mixed = cfg->precision == IMDG_MIXED;
   const char* remap = getenv("IMDG_XCD_REMAP");

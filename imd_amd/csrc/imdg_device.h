@@ -84,6 +84,34 @@ __device__ __forceinline__ R knot_grad(R2 a, R2 b, R chi, R twoistep) {
   return twoistep * (a.y + (chi - (R)0.5) * d2v);
 }
 
+// The 4-point and spline interpolations (src/potaccess.h:365-457) are cubic
+// in the fraction x of segment k; their tables hold two knots per segment,
+// (c0, c1) and (c2, c3): value c0 + x (c1 + x (c2 + x c3)), twice the
+// derivative 2 istep (c1 + x (2 c2 + 3 x c3)).  tab_* pick the segment's
+// first knot and evaluate it in either form (CUB: cubic segments; else the
+// 3-point knots k, k+1).
+template <typename R, typename R2>
+__device__ __forceinline__ R cub_val(R2 a, R2 b, R x) {
+  return a.x + x * (a.y + x * (b.x + x * b.y));
+}
+template <typename R, typename R2>
+__device__ __forceinline__ R cub_grad(R2 a, R2 b, R x, R twoistep) {
+  return twoistep * (a.y + x * ((R)2 * b.x + (R)3 * x * b.y));
+}
+template <bool CUB>
+__device__ __forceinline__ int tab_knot(int k) {
+  return CUB ? 2 * k : k;
+}
+template <bool CUB, typename R, typename R2>
+__device__ __forceinline__ R tab_val(R2 a, R2 b, R x) {
+  if (CUB) return cub_val<R>(a, b, x);
+  return knot_val<R>(a, b, x);
+}
+template <bool CUB, typename R, typename R2>
+__device__ __forceinline__ R tab_grad(R2 a, R2 b, R x, R twoistep) {
+  if (CUB) return cub_grad<R>(a, b, x, twoistep);
+  return knot_grad<R>(a, b, x, twoistep);
+}
 // Entry m of slot i in the grouped list layout (4 entries per lane).
 __device__ __forceinline__ size_t nbr_index(int i, int m, int kmax) {
   return (((size_t)(i >> 6) * (kmax >> 2) + (m >> 2)) * IMDG_WAVE + (i & 63)) *

@@ -210,6 +210,8 @@ struct StepArgs {
                           // (pair, rho) table, -1 if there is none
   int dpoly;              // first knot of the FP32 rho segment polynomials
                           // (one species), -1 if there are none
+  int cubic;              // tables hold cubic segments, two knots each
+                          // (4-point or spline interpolation)
   int lpa_den, lpa_frc;   // lanes per slot of the density / force loops
   int thr_den, thr_frc;   // threads per block of the density / force loops
 };

@@ -604,7 +604,8 @@ struct ForceCols {
 // Four neighbours of the density loop: records and offsets, the prefetch,
 // r^2 and segments, all knot loads together, accumulation with mask
 // multiplies (no branches around the loads).
-template <typename R, bool NT1, typename R4, typename R2, typename PF>
+template <typename R, bool NT1, bool CUB, typename R4, typename R2,
+          typename PF>
 __device__ __forceinline__ void density_group(
     uint4 e4, PF pf, uint4& slot, const R4* __restrict__ rec,
     const R4* off, const R4& ui, int it, int nt, const R2* __restrict__ K,
@@ -642,14 +643,14 @@ __device__ __forceinline__ void density_group(
     R dz = (uj[u].z - ui.z) + o[u].z;
     r2[u] = dx * dx + dy * dy + dz * dz;
     if (NT1) {
-      kk[u] = c0.kb + seg_locate(r2[u], c0.begin, c0.end, c0.invstep, c0.len,
-                                 chi[u]);
+      kk[u] = c0.kb + tab_knot<CUB>(seg_locate(r2[u], c0.begin, c0.end,
+                                               c0.invstep, c0.len, chi[u]));
       cb[u] = c0.begin;
       ce[u] = c0.end;
     } else {
       ColPar<R> cp = cpr[it * nt + (int)uj[u].w];
-      kk[u] = cp.off +
-              seg_locate(r2[u], cp.begin, cp.end, cp.invstep, cp.len, chi[u]);
+      kk[u] = cp.off + tab_knot<CUB>(seg_locate(r2[u], cp.begin, cp.end,
+                                                cp.invstep, cp.len, chi[u]));
       cb[u] = cp.begin;
       ce[u] = cp.end;
     }
@@ -663,7 +664,7 @@ __device__ __forceinline__ void density_group(
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     bool in = r2[u] < ce[u];
-    rho = fma(in ? (R)1 : (R)0, knot_val<R>(p[u], q[u], chi[u]), rho);
+    rho = fma(in ? (R)1 : (R)0, tab_val<CUB, R>(p[u], q[u], chi[u]), rho);
     if (NT1)
       r2min = r2[u] < r2min ? r2[u] : r2min;  // r2 < begin implies in range
     else
@@ -718,8 +719,8 @@ __device__ __forceinline__ void density_group_poly(
 }
 
 // Four neighbours of the force loop (same phases as density_group).
-template <typename R, bool NT1, bool HP, bool HE, typename R4, typename R2,
-          typename PF>
+template <typename R, bool NT1, bool HP, bool HE, bool CUB, typename R4,
+          typename R2, typename PF>
 __device__ __forceinline__ void force_group(
     uint4 e4, PF pf, uint4& slot, const R4* __restrict__ rec,
     const signed char* __restrict__ atype, const R4* off, const R4& ui, R dFi,
@@ -751,8 +752,12 @@ __device__ __forceinline__ void force_group(
     int kp[4], kr[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      if (HP) kp[u] = c.pk + seg_locate(r2[u], c.pb, c.pe, c.pis, c.plen, cp[u]);
-      if (HE) kr[u] = c.rk + seg_locate(r2[u], c.rb, c.re, c.ris, c.rlen, cr[u]);
+      if (HP)
+        kp[u] = c.pk + tab_knot<CUB>(
+                           seg_locate(r2[u], c.pb, c.pe, c.pis, c.plen, cp[u]));
+      if (HE)
+        kr[u] = c.rk + tab_knot<CUB>(
+                           seg_locate(r2[u], c.rb, c.re, c.ris, c.rlen, cr[u]));
     }
     R2 p0[4], p1[4], q0[4], q1[4];
 #pragma unroll
@@ -772,15 +777,15 @@ __device__ __forceinline__ void force_group(
       if (HP) {
         bool in = r2[u] <= c.pe;
         R m = in ? (R)1 : (R)0;
-        g1 = m * knot_grad<R>(p0[u], p1[u], cp[u], c.p2is);
-        ep = fma(m, knot_val<R>(p0[u], p1[u], cp[u]), ep);
+        g1 = m * tab_grad<CUB, R>(p0[u], p1[u], cp[u], c.p2is);
+        ep = fma(m, tab_val<CUB, R>(p0[u], p1[u], cp[u]), ep);
         is_short |= (in && r2[u] < c.pb);
       }
       if (HE) {
         bool in = r2[u] < c.re;
         R m = in ? (R)0.5 : (R)0;
-        g1 = fma(m * (dFi + uj[u].w), knot_grad<R>(q0[u], q1[u], cr[u], c.r2is),
-                 g1);
+        g1 = fma(m * (dFi + uj[u].w),
+                 tab_grad<CUB, R>(q0[u], q1[u], cr[u], c.r2is), g1);
         is_short |= (in && r2[u] < c.rb);
       }
       gg[u] = g1;
@@ -794,10 +799,11 @@ __device__ __forceinline__ void force_group(
         ColPar<R> cp = cpp[col];
         if (r2[u] <= cp.end) {
           R chi;
-          int k = seg_locate(r2[u], cp.begin, cp.end, cp.invstep, cp.len, chi);
+          int k = tab_knot<CUB>(
+              seg_locate(r2[u], cp.begin, cp.end, cp.invstep, cp.len, chi));
           R2 p = K[cp.off + k], q = K[cp.off + k + 1];
-          g1 += knot_grad<R>(p, q, chi, cp.twoistep);
-          ep += knot_val<R>(p, q, chi);
+          g1 += tab_grad<CUB, R>(p, q, chi, cp.twoistep);
+          ep += tab_val<CUB, R>(p, q, chi);
           is_short |= (r2[u] < cp.begin);
         }
       }
@@ -807,16 +813,18 @@ __device__ __forceinline__ void force_group(
         ColPar<R> ci = cpr[colT], cj = cpr[col];
         if (r2[u] < ci.end || r2[u] < cj.end) {
           R chi;
-          int k = seg_locate(r2[u], ci.begin, ci.end, ci.invstep, ci.len, chi);
+          int k = tab_knot<CUB>(
+              seg_locate(r2[u], ci.begin, ci.end, ci.invstep, ci.len, chi));
           is_short |= (r2[u] < ci.begin);
-          R rpi = knot_grad<R>(K[ci.off + k], K[ci.off + k + 1], chi,
-                               ci.twoistep);
+          R rpi = tab_grad<CUB, R>(K[ci.off + k], K[ci.off + k + 1], chi,
+                                   ci.twoistep);
           R rpj = rpi;
           if (colT != col) {
-            k = seg_locate(r2[u], cj.begin, cj.end, cj.invstep, cj.len, chi);
+            k = tab_knot<CUB>(
+                seg_locate(r2[u], cj.begin, cj.end, cj.invstep, cj.len, chi));
             is_short |= (r2[u] < cj.begin);
-            rpj = knot_grad<R>(K[cj.off + k], K[cj.off + k + 1], chi,
-                               cj.twoistep);
+            rpj = tab_grad<CUB, R>(K[cj.off + k], K[cj.off + k + 1], chi,
+                                   cj.twoistep);
           }
           g1 += (R)0.5 * (dFi * rpj + uj[u].w * rpi);
         }
@@ -954,8 +962,9 @@ __device__ __forceinline__ void force_group_sgf(
 // Host density, embedding energy and 2F'(rho) (src/imd_forces_nbl.c:586-611,
 // 1079-1112): rho_i = sum_j rho_{type(i) type(j)}(r_ij^2) over the full
 // list, then F and 2F' from the embedding column of type(i).  Writes the
-// (u, 2F') records the force loop gathers.  L lanes per slot.
-template <typename R, bool NT1, bool LT, int L>
+// (u, 2F') records the force loop gathers.  L lanes per slot; CUB: the
+// tables are cubic segments (4-point or spline interpolation).
+template <typename R, bool NT1, bool LT, int L, bool CUB>
 __global__ void __launch_bounds__(IMDG_STEP_THREADS)
     k_density(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
   typedef typename V4<R>::T R4;
@@ -1001,7 +1010,7 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
         DensityCol<R> col0 = {kb0, len0, b0, e0, is0};
         R r2min = (R)3.0e38;
         const uint32_t* nw = list_words<L>(a.nbr, i, a.kmax, gs);
-        if constexpr (sizeof(R) == 4 && NT1) {
+        if constexpr (sizeof(R) == 4 && NT1 && !CUB) {
           const float4* __restrict__ DQ =
               reinterpret_cast<const float4*>(K + (a.dpoly - klo));
           const float tc0 = -b0 * is0, tz = (float)len0 - 0.5f;
@@ -1013,9 +1022,9 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
         } else {
           list_walk<L>(nw, a.nnb[i],
                        [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                         density_group<R, NT1>(e4, pf, slot, rec, off, ui, it,
-                                               nt, K, cpr, col0, rho,
-                                               is_short, r2min);
+                         density_group<R, NT1, CUB>(e4, pf, slot, rec, off,
+                                                    ui, it, nt, K, cpr, col0,
+                                                    rho, is_short, r2min);
                        });
         }
         if (NT1) is_short |= (r2min < b0);
@@ -1030,15 +1039,15 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
       R4 ui = rec[i];
       int it = NT1 ? 0 : (int)ui.w;
       R rho = rho_keep;
-      // embedding: PAIR_INT2 on the embedding column of the type, argument
+      // embedding: PAIR_INT on the embedding column of the type, argument
       // rho (knots in HBM; one lookup per atom)
       const R2* __restrict__ ke = a.knots + te.off[it];
       R chi;
-      int k = seg_locate(rho, te.begin[it], te.end[it], te.invstep[it],
-                         te.len[it], chi);
+      int k = tab_knot<CUB>(seg_locate(rho, te.begin[it], te.end[it],
+                                       te.invstep[it], te.len[it], chi));
       R2 p = ke[k], q = ke[k + 1];
-      R F = knot_val<R>(p, q, chi);
-      R dF = knot_grad<R>(p, q, chi, te.twoistep[it]);
+      R F = tab_val<CUB, R>(p, q, chi);
+      R dF = tab_grad<CUB, R>(p, q, chi, te.twoistep[it]);
       R4 rb;
       rb.x = ui.x;
       rb.y = ui.y;
@@ -1075,7 +1084,7 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
 // (calc_forces, move_atoms and check_nblist of one main_loop iteration in
 // one launch).  L lanes per slot.
 template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT,
-          int L>
+          int L, bool CUB>
 __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
     k_force(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
   constexpr bool NT1 = MODE != FM_NTN;
@@ -1132,7 +1141,7 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
         int it = NT1 ? 0 : (int)atype[i];
         R dFi = HE ? ui.w : (R)0;
         const uint32_t* nw = list_words<L>(a.nbr, i, a.kmax, gs);
-        if (MODE == FM_SG) {
+        if constexpr (MODE == FM_SG && !CUB) {
           // one grid: sums in units of 1/(2 istep), scaled once per slot
           R r2min = (R)3.0e38;
           if constexpr (sizeof(R) == 4) {
@@ -1166,7 +1175,7 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
                              rk0, rlen, rb, re, ris, r2is};
           list_walk<L>(nw, a.nnb[i],
                        [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                         force_group<R, NT1, HP, HE>(
+                         force_group<R, NT1, HP, HE, CUB>(
                              e4, pf, slot, rec, atype, off, ui, dFi, it, nt,
                              K, cpp, cpr, fc, fx, fy, fz, ep, vir, is_short);
                        });
@@ -1185,7 +1194,7 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
         kvir = vir;
       }
     }
-    if (MODE == FM_SG) {
+    if (MODE == FM_SG && !CUB) {
       kx *= p2is;
       ky *= p2is;
       kz *= p2is;
@@ -1300,6 +1309,14 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
   const int nt = a.ntypes;
   const R2* __restrict__ K = a.knots;
   const R4* __restrict__ rec = a.rec_b;
+  const bool cub = a.cubic != 0;
+  // twice the derivative of a table at segment knot k0 (3-point knots or
+  // cubic segments)
+  auto grad_at = [&](int k0, R chi, R twoistep) {
+    return cub ? cub_grad<R>(K[k0], K[k0 + 1], chi, twoistep)
+               : knot_grad<R>(K[k0], K[k0 + 1], chi, twoistep);
+  };
+  const int kmul = cub ? 2 : 1;
   const int i = b * IMDG_BLOCK + threadIdx.x;
   double s[6] = {0, 0, 0, 0, 0, 0};
   if (i - Rp.atom_off < Rp.natoms) {
@@ -1330,22 +1347,19 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
           R chi;
           int k = seg_locate(r2, tp.begin[col], tp.end[col], tp.invstep[col],
                              tp.len[col], chi);
-          int k0 = tp.off[col] + k;
-          g += knot_grad<R>(K[k0], K[k0 + 1], chi, tp.twoistep[col]);
+          g += grad_at(tp.off[col] + kmul * k, chi, tp.twoistep[col]);
         }
         if (he && (r2 < tr.end[colT] || r2 < tr.end[col])) {
           // rho'_i from column jt*nt+it, rho'_j from it*nt+jt
           R chi;
           int k = seg_locate(r2, tr.begin[colT], tr.end[colT],
                              tr.invstep[colT], tr.len[colT], chi);
-          int k0 = tr.off[colT] + k;
-          R rpi = knot_grad<R>(K[k0], K[k0 + 1], chi, tr.twoistep[colT]);
+          R rpi = grad_at(tr.off[colT] + kmul * k, chi, tr.twoistep[colT]);
           R rpj = rpi;
           if (colT != col) {
             k = seg_locate(r2, tr.begin[col], tr.end[col], tr.invstep[col],
                            tr.len[col], chi);
-            k0 = tr.off[col] + k;
-            rpj = knot_grad<R>(K[k0], K[k0 + 1], chi, tr.twoistep[col]);
+            rpj = grad_at(tr.off[col] + kmul * k, chi, tr.twoistep[col]);
           }
           g += (R)0.5 * (dFi * rpj + uj.w * rpi);
         }
@@ -1386,48 +1400,66 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
 // launch plans and launchers
 // ---------------------------------------------------------------------------
 
-template <typename R, bool NT1, bool LT>
+template <typename R, bool NT1, bool LT, bool CUB>
 const void* density_fn(int lpa) {
-  if (lpa == 4) return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 4>);
-  if (lpa == 2) return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 2>);
-  return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 1>);
+  if (lpa == 4)
+    return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 4, CUB>);
+  if (lpa == 2)
+    return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 2, CUB>);
+  return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 1, CUB>);
 }
 
-template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT>
+template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT,
+          bool CUB>
 const void* force_fn3(int lpa) {
   if (lpa == 4)
     return reinterpret_cast<const void*>(
-        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 4>);
+        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 4, CUB>);
   if (lpa == 2)
     return reinterpret_cast<const void*>(
-        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 2>);
+        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 2, CUB>);
   return reinterpret_cast<const void*>(
-      &k_force<R, MODE, INTEGRATE, HP, HE, LT, 1>);
+      &k_force<R, MODE, INTEGRATE, HP, HE, LT, 1, CUB>);
 }
 
-template <typename R, int MODE, bool INTEGRATE, bool LT>
+template <typename R, int MODE, bool INTEGRATE, bool LT, bool CUB>
 const void* force_fn2(bool hp, bool he, int lpa) {
   if (MODE == FM_SG)
-    return force_fn3<R, MODE, INTEGRATE, true, true, LT>(lpa);
-  if (hp && he) return force_fn3<R, MODE, INTEGRATE, true, true, LT>(lpa);
-  if (hp) return force_fn3<R, MODE, INTEGRATE, true, false, LT>(lpa);
-  return force_fn3<R, MODE, INTEGRATE, false, true, LT>(lpa);
+    return force_fn3<R, MODE, INTEGRATE, true, true, LT, CUB>(lpa);
+  if (hp && he) return force_fn3<R, MODE, INTEGRATE, true, true, LT, CUB>(lpa);
+  if (hp) return force_fn3<R, MODE, INTEGRATE, true, false, LT, CUB>(lpa);
+  return force_fn3<R, MODE, INTEGRATE, false, true, LT, CUB>(lpa);
 }
 
 template <typename R, int MODE>
 const void* force_fn1(bool integrate, bool lt, bool hp, bool he, int lpa) {
   if (integrate)
-    return lt ? force_fn2<R, MODE, true, true>(hp, he, lpa)
-              : force_fn2<R, MODE, true, false>(hp, he, lpa);
-  return lt ? force_fn2<R, MODE, false, true>(hp, he, lpa)
-            : force_fn2<R, MODE, false, false>(hp, he, lpa);
+    return lt ? force_fn2<R, MODE, true, true, false>(hp, he, lpa)
+              : force_fn2<R, MODE, true, false, false>(hp, he, lpa);
+  return lt ? force_fn2<R, MODE, false, true, false>(hp, he, lpa)
+            : force_fn2<R, MODE, false, false, false>(hp, he, lpa);
+}
+
+// The cubic-segment tables (4-point, spline) take the generic loops with
+// the tables in HBM/L2 (two knots per segment: twice the 3-point size).
+template <typename R, int MODE>
+const void* force_fn1_cub(bool integrate, bool hp, bool he, int lpa) {
+  return integrate ? force_fn2<R, MODE, true, false, true>(hp, he, lpa)
+                   : force_fn2<R, MODE, false, false, true>(hp, he, lpa);
 }
 
 // mode: 0 one species, 1 one species on a shared grid (needs hp and he),
-// 2 several species; lpa: lanes per slot (1, 2 or 4)
+// 2 several species; lpa: lanes per slot (1, 2 or 4); cub: cubic-segment
+// tables (modes 0 and 2 only, lt false)
 template <typename R>
 const void* force_fn(int mode, bool integrate, bool lt, bool hp, bool he,
-                     int lpa) {
+                     int lpa, bool cub) {
+  if (cub) {
+    if (lt || mode == FM_SG) return nullptr;
+    if (mode == FM_NT1)
+      return force_fn1_cub<R, FM_NT1>(integrate, hp, he, lpa);
+    return force_fn1_cub<R, FM_NTN>(integrate, hp, he, lpa);
+  }
   if (mode == FM_SG)
     return force_fn1<R, FM_SG>(integrate, lt, true, true, lpa);
   if (mode == FM_NT1) return force_fn1<R, FM_NT1>(integrate, lt, hp, he, lpa);
@@ -1435,12 +1467,17 @@ const void* force_fn(int mode, bool integrate, bool lt, bool hp, bool he,
 }
 
 template <typename R>
-const void* density_fn(bool nt1, bool lt, int lpa) {
+const void* density_fn(bool nt1, bool lt, int lpa, bool cub) {
+  if (cub) {
+    if (lt) return nullptr;
+    return nt1 ? density_fn<R, true, false, true>(lpa)
+               : density_fn<R, false, false, true>(lpa);
+  }
   if (nt1)
-    return lt ? density_fn<R, true, true>(lpa)
-              : density_fn<R, true, false>(lpa);
-  return lt ? density_fn<R, false, true>(lpa)
-            : density_fn<R, false, false>(lpa);
+    return lt ? density_fn<R, true, true, false>(lpa)
+              : density_fn<R, true, false, false>(lpa);
+  return lt ? density_fn<R, false, true, false>(lpa)
+            : density_fn<R, false, false, false>(lpa);
 }
 
 // Persistent grid of a kernel: resident blocks per CU (from the occupancy
@@ -1448,6 +1485,7 @@ const void* density_fn(bool nt1, bool lt, int lpa) {
 // down to a multiple of the 8 XCDs; 0 if the kernel cannot be launched with
 // that much dynamic LDS.
 static int persistent_grid(const void* fn, int threads, size_t lds, int ncu) {
+  if (!fn) return 0;  // no such variant
   hipFuncAttributes fa;
   if (hipFuncGetAttributes(&fa, fn) != hipSuccess) {
     (void)hipGetLastError();
@@ -1481,8 +1519,8 @@ static int persistent_grid(const void* fn, int threads, size_t lds, int ncu) {
 
 template <typename R>
 int step_grid_density(bool nt1, bool lt, size_t lds, int ncu, int lpa,
-                      int threads) {
-  return persistent_grid(density_fn<R>(nt1, lt, lpa), threads, lds, ncu);
+                      int threads, bool cub) {
+  return persistent_grid(density_fn<R>(nt1, lt, lpa, cub), threads, lds, ncu);
 }
 
 template <typename R>
@@ -1494,9 +1532,9 @@ int force_threads(int mode) {
 
 template <typename R>
 int step_grid_force(int mode, bool integrate, bool lt, bool hp, bool he,
-                    size_t lds, int ncu, int lpa, int threads) {
+                    size_t lds, int ncu, int lpa, int threads, bool cub) {
   if (threads > force_threads<R>(mode)) return 0;
-  return persistent_grid(force_fn<R>(mode, integrate, lt, hp, he, lpa),
+  return persistent_grid(force_fn<R>(mode, integrate, lt, hp, he, lpa, cub),
                          threads, lds, ncu);
 }
 
@@ -1516,8 +1554,9 @@ hipError_t launch_density(const StepArgs<R>& a, const TabDev<R>* tabs,
     const TabDev<R>* t;
   } pk = {a, tabs};
   void* kargs[] = {&pk.a, &pk.t};
-  return hipLaunchKernel(density_fn<R>(nt1, lt, a.lpa_den), dim3(grid),
-                         dim3(a.thr_den), kargs, lds, st);
+  const void* fn = density_fn<R>(nt1, lt, a.lpa_den, a.cubic != 0);
+  if (!fn) return hipErrorInvalidDeviceFunction;
+  return hipLaunchKernel(fn, dim3(grid), dim3(a.thr_den), kargs, lds, st);
 }
 
 template <typename R>
@@ -1528,10 +1567,10 @@ hipError_t launch_force(const StepArgs<R>& a, const TabDev<R>* tabs,
   StepArgs<R> aa = a;
   const TabDev<R>* tt = tabs;
   void* kargs[] = {&aa, &tt};
-  return hipLaunchKernel(
-      force_fn<R>(mode, integrate, lt, a.has_pair != 0, a.has_eam != 0,
-                  a.lpa_frc),
-      dim3(grid), dim3(a.thr_frc), kargs, lds, st);
+  const void* fn = force_fn<R>(mode, integrate, lt, a.has_pair != 0,
+                               a.has_eam != 0, a.lpa_frc, a.cubic != 0);
+  if (!fn) return hipErrorInvalidDeviceFunction;
+  return hipLaunchKernel(fn, dim3(grid), dim3(a.thr_frc), kargs, lds, st);
 }
 
 template <typename R>
@@ -1571,9 +1610,10 @@ hipError_t launch_stress(const StepArgs<R>& a, const TabDev<R>* tabs,
 }
 
 #define IMDG_INSTANTIATE(R)                                                   \
-  template int step_grid_density<R>(bool, bool, size_t, int, int, int);      \
+  template int step_grid_density<R>(bool, bool, size_t, int, int, int,     \
+                                    bool);                                    \
   template int step_grid_force<R>(int, bool, bool, bool, bool, size_t, int,  \
-                                  int, int);                                  \
+                                  int, int, bool);                            \
   template int force_threads<R>(int);                                         \
   template int step_grid_move<R>(int);                                        \
   template hipError_t launch_density<R>(const StepArgs<R>&,                   \

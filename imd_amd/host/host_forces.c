@@ -65,6 +65,7 @@ int imd_gpu_init(void) {
   cfg.nbl_size = nbl_size;
   cfg.device = gpu_device;
   cfg.lanes_per_atom = gpu_lanes;
+  cfg.interpolation = interpolation;
   check(imdg_ctx_create(&cfg, &ctx), "imdg_ctx_create");
   imdg_table tp, tr, te;
   if (have_potfile) tp = as_imdg(&pair_pot);

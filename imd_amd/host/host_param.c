@@ -53,6 +53,7 @@ char potfilename[IMD_STRLEN], eam2_emb_E_filename[IMD_STRLEN];
 char eam2_at_rho_filename[IMD_STRLEN];
 int gpu_precision = IMDG_MIXED, gpu_device = 0;
 int gpu_lanes = 0, gpu_autotune = 0;
+int interpolation = IMDG_INTERP_3POINT;
 int have_potfile = 0, have_eam = 0;
 
 void imd_error(const char *fmt, ...) {
@@ -104,6 +105,7 @@ void init_parameters(void) {
   gpu_device = 0;
   gpu_lanes = 0;
   gpu_autotune = 0;
+  interpolation = IMDG_INTERP_3POINT;
   have_potfile = have_eam = 0;
 }
 
@@ -274,6 +276,19 @@ static int getparamfile(const char *paramfile, int phase) {
       get_ints(key, &gpu_lanes, 1);
       if (gpu_lanes != 0 && gpu_lanes != 1 && gpu_lanes != 2 && gpu_lanes != 4)
         imd_error("gpu_lanes must be 0, 1, 2 or 4");
+    } else if (!strcasecmp(key, "interpolation")) {
+      /* what IMD's FOURPOINT / SPLINE build options select
+         (src/potaccess.h:24-36): threepoint (default), fourpoint, spline */
+      char v[IMD_STRLEN];
+      get_str(key, v);
+      if (!strcasecmp(v, "threepoint"))
+        interpolation = IMDG_INTERP_3POINT;
+      else if (!strcasecmp(v, "fourpoint"))
+        interpolation = IMDG_INTERP_4POINT;
+      else if (!strcasecmp(v, "spline"))
+        interpolation = IMDG_INTERP_SPLINE;
+      else
+        imd_error("interpolation must be threepoint, fourpoint or spline");
     } else if (!strcasecmp(key, "gpu_autotune")) {
       /* time every launch shape once before the main loop (imdg_autotune) */
       get_ints(key, &gpu_autotune, 1);

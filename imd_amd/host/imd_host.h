@@ -59,6 +59,7 @@ extern char progname[IMD_STRLEN];
 extern char potfilename[IMD_STRLEN], eam2_emb_E_filename[IMD_STRLEN];
 extern char eam2_at_rho_filename[IMD_STRLEN];
 extern int gpu_precision, gpu_device, gpu_lanes, gpu_autotune;
+extern int interpolation;  /* IMDG_INTERP_* */
 extern int have_potfile, have_eam;
 
 extern pot_table_t pair_pot, embed_pot, rho_h_tab;

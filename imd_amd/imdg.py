@@ -37,7 +37,12 @@ class Config(C.Structure):
     _fields_ = [("precision", C.c_int), ("ensemble", C.c_int),
                 ("ntypes", C.c_int), ("timestep", C.c_double),
                 ("nbl_margin", C.c_double), ("nbl_size", C.c_double),
-                ("device", C.c_int), ("lanes_per_atom", C.c_int)]
+                ("device", C.c_int), ("lanes_per_atom", C.c_int),
+                ("interpolation", C.c_int)]
+
+
+# table interpolation (IMD's build options: 3-point, FOURPOINT, SPLINE)
+INTERP_3POINT, INTERP_4POINT, INTERP_SPLINE = 0, 1, 2
 
 
 class Table(C.Structure):
@@ -188,10 +193,10 @@ class Context:
 
     def __init__(self, precision=MIXED, ensemble=NVE, ntypes=1,
                  timestep=0.001, nbl_margin=0.4, nbl_size=1.1, device=0,
-                 lanes_per_atom=0):
+                 lanes_per_atom=0, interpolation=INTERP_3POINT):
         L = lib()
         cfg = Config(precision, ensemble, ntypes, timestep, nbl_margin,
-                     nbl_size, device, lanes_per_atom)
+                     nbl_size, device, lanes_per_atom, interpolation)
         h = C.c_void_p()
         rc = L.imdg_ctx_create(C.byref(cfg), C.byref(h))
         if rc != 0:

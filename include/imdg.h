@@ -67,7 +67,21 @@ typedef struct {
   int    device;         /* HIP device ordinal */
   int    lanes_per_atom; /* lanes per atom of both neighbour loops: 1, 2
                             or 4; 0 = the default (see imdg_tuning) */
+  int    interpolation;  /* IMDG_INTERP_*: IMD's table interpolation, a
+                            build option there (src/potaccess.h:24-36) */
 } imdg_config;
+
+/* Table interpolation.  3-point is IMD's default (PAIR_INT2, VAL_FUNC2,
+   DERIV_FUNC2); FOURPOINT uses PAIR_INT3 and friends
+   (src/potaccess.h:365-412), SPLINE PAIR_INT_SP and friends
+   (src/potaccess.h:414-457).  For the last two the library derives the
+   guard rows (init_fourpoint, src/imd_potential.c:1171-1187) and the spline
+   second derivatives (init_spline, src/imd_potential.c:1197-1246; radial
+   tables with zero slope at the end, the embedding function natural) from
+   the first len rows of each column, so the tables are passed as read. */
+#define IMDG_INTERP_3POINT 0
+#define IMDG_INTERP_4POINT 1
+#define IMDG_INTERP_SPLINE 2
 
 /* A read IMD function table (pot_table_t, src/types.h:416-428), after
    read_pot_table has shifted radial columns and appended the two guard rows

@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 ORC_NVE, ORC_NVT = 0, 1
+INTERP_3POINT, INTERP_4POINT, INTERP_SPLINE = 0, 1, 2
 
 
 def build(force: bool = True) -> str:
@@ -30,7 +31,8 @@ class _Table(C.Structure):
     _fields_ = [("ncols", C.c_int), ("maxsteps", C.c_int),
                 ("begin", C.POINTER(C.c_double)), ("end", C.POINTER(C.c_double)),
                 ("step", C.POINTER(C.c_double)), ("invstep", C.POINTER(C.c_double)),
-                ("len", C.POINTER(C.c_int)), ("table", C.POINTER(C.c_double))]
+                ("len", C.POINTER(C.c_int)), ("table", C.POINTER(C.c_double)),
+                ("interp", C.c_int), ("table2", C.POINTER(C.c_double))]
 
 
 _lib = None
@@ -46,6 +48,7 @@ def lib():
         L.orc_table_read.argtypes = [C.POINTER(_Table), C.c_char_p, C.c_int,
                                      C.c_int, C.c_int, dp]
         L.orc_table_free.argtypes = [C.POINTER(_Table)]
+        L.orc_table_set_interp.argtypes = [C.POINTER(_Table), C.c_int, C.c_int]
         L.orc_pair_int2.argtypes = [C.POINTER(_Table), C.c_int, C.c_int,
                                     C.c_double, dp, dp, ip]
         L.orc_val_func2.argtypes = [C.POINTER(_Table), C.c_int, C.c_int,
@@ -86,7 +89,7 @@ class OracleTable:
     """An IMD function table read through read_pot_table semantics."""
 
     def __init__(self, path: str, ncols: int, radial: bool,
-                 default_format: int = 2):
+                 default_format: int = 2, interp: int = INTERP_3POINT):
         self._t = _Table()
         self.cellsz = C.c_double(0.0)
         rc = lib().orc_table_read(C.byref(self._t), path.encode(), ncols,
@@ -95,16 +98,22 @@ class OracleTable:
         if rc != 0:
             raise ValueError("orc_table_read(%s) failed with %d" % (path, rc))
         self.ncols = ncols
+        if interp != INTERP_3POINT:
+            rc = lib().orc_table_set_interp(C.byref(self._t), interp,
+                                            int(radial))
+            if rc != 0:
+                raise ValueError("orc_table_set_interp failed with %d" % rc)
 
     @classmethod
-    def from_table(cls, table, ncols=None, radial=None):
+    def from_table(cls, table, ncols=None, radial=None, interp=INTERP_3POINT):
         """Write a workloads.Table to a temporary file and read it back."""
         fd, path = tempfile.mkstemp(suffix=".pt")
         os.close(fd)
         try:
             table.write(path)
             return cls(path, ncols or table.ncols,
-                       table.radial if radial is None else radial)
+                       table.radial if radial is None else radial,
+                       interp=interp)
         finally:
             os.unlink(path)
 
@@ -124,7 +133,10 @@ class OracleTable:
                     len=np.ctypeslib.as_array(t.len, (n,)).copy(),
                     maxsteps=t.maxsteps,
                     table=np.ctypeslib.as_array(
-                        t.table, ((t.maxsteps + 2) * n,)).copy().reshape(-1, n))
+                        t.table, ((t.maxsteps + 2) * n,)).copy().reshape(-1, n),
+                    table2=(np.ctypeslib.as_array(
+                        t.table2, ((t.maxsteps + 2) * n,)).copy().reshape(-1, n)
+                        if t.table2 else None))
 
     def pair_int2(self, col, r2):
         pot, grad, short = C.c_double(), C.c_double(), C.c_int(0)

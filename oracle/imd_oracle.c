@@ -40,6 +40,7 @@ void orc_table_free(orc_table *pt) {
   free(pt->invstep);
   free(pt->len);
   free(pt->table);
+  free(pt->table2);
   memset(pt, 0, sizeof(*pt));
 }
 
@@ -183,54 +184,182 @@ int orc_table_read(orc_table *pt, const char *path, int ncols, int radial,
   return rc;
 }
 
-/* Common argument handling of the three access macros: clamp r2 into
-   [begin, end], index, fraction, three consecutive rows. */
-static inline void tab_locate(const orc_table *pt, int col, int inc, double r2,
-                              int *is_short, double *chi, double *istep_out,
-                              double *p0, double *p1, double *p2) {
+/* init_fourpoint (src/imd_potential.c:1171-1187): continue the last
+   4-point polynomial by two rows. */
+static int init_fourpoint(orc_table *pt) {
+  int nc = pt->ncols;
+  for (int col = 0; col < nc; col++) {
+    double *y = pt->table + col;
+    int n = pt->len[col];
+    if (n < 4) return ORC_ERR_FORMAT;
+    y[n * nc] = 4 * y[(n - 1) * nc] - 6 * y[(n - 2) * nc] +
+                4 * y[(n - 3) * nc] - y[(n - 4) * nc];
+    y[(n + 1) * nc] = 10 * y[(n - 1) * nc] - 20 * y[(n - 2) * nc] +
+                      15 * y[(n - 3) * nc] - 4 * y[(n - 4) * nc];
+  }
+  return ORC_OK;
+}
+
+/* init_spline (src/imd_potential.c:1197-1246): second derivatives of the
+   cubic spline through the len rows of each column (tridiagonal sweep,
+   natural at the start; at the end first derivative zero for radial
+   tables, natural otherwise), then one guard row of values and of second
+   derivatives. */
+static int init_spline(orc_table *pt, int radial) {
+  int nc = pt->ncols, size = pt->maxsteps + 2;
+  free(pt->table2);
+  pt->table2 = calloc((size_t)nc * size, sizeof(double));
+  double *u = calloc(size, sizeof(double));
+  if (!pt->table2 || !u) {
+    free(u);
+    return ORC_ERR_NOMEM;
+  }
+  for (int col = 0; col < nc; col++) {
+    double *y2 = pt->table2 + col, *y = pt->table + col;
+    int n = pt->len[col];
+    double step = pt->step[col], p, qn, un;
+    if (n < 3) {
+      free(u);
+      return ORC_ERR_FORMAT;
+    }
+    y2[0] = u[0] = 0;
+    for (int i = 1; i < n - 1; i++) {
+      p = 0.5 * y2[(i - 1) * nc] + 2.0;
+      y2[i * nc] = -0.5 / p;
+      u[i] = (y[(i + 1) * nc] - 2 * y[i * nc] + y[(i - 1) * nc]) / step;
+      u[i] = (6.0 * u[i] / (2 * step) - 0.5 * u[i - 1]) / p;
+    }
+    if (radial) {
+      qn = 0.5;
+      un = (3.0 / step) * (y[(n - 2) * nc] - y[(n - 1) * nc]) / step;
+    } else {
+      qn = un = 0.0;
+    }
+    y2[(n - 1) * nc] = (un - qn * u[n - 2]) / (qn * y2[(n - 2) * nc] + 1.0);
+    for (int k = n - 2; k >= 0; k--)
+      y2[k * nc] = y2[k * nc] * y2[(k + 1) * nc] + u[k];
+    y[n * nc] = 2 * y[(n - 1) * nc] - y[(n - 2) * nc] +
+                step * step * y2[(n - 1) * nc];
+    y2[n * nc] = 2 * y2[(n - 1) * nc] - y2[(n - 2) * nc];
+  }
+  free(u);
+  return ORC_OK;
+}
+
+int orc_table_set_interp(orc_table *pt, int interp, int radial) {
+  int rc;
+  if (interp == ORC_INTERP_4POINT)
+    rc = init_fourpoint(pt);
+  else if (interp == ORC_INTERP_SPLINE)
+    rc = init_spline(pt, radial);
+  else if (interp == ORC_INTERP_3POINT)
+    rc = init_threepoint(pt);
+  else
+    return ORC_ERR_ARG;
+  if (rc == ORC_OK) pt->interp = interp;
+  return rc;
+}
+
+/* Common argument handling of the access macros: r2 clamped to end, then
+   to begin (is_short), scaled to the table index t >= 0. */
+static inline double tab_arg(const orc_table *pt, int col, double r2,
+                             int *is_short) {
   double r2a = (r2 < pt->end[col]) ? r2 : pt->end[col];
   r2a = r2a - pt->begin[col];
   if (r2a < 0) {
     r2a = 0;
     *is_short = 1;
   }
-  double istep = pt->invstep[col];
-  r2a = r2a * istep;
-  int k = (int)r2a;
-  *chi = r2a - k;
+  return r2a * pt->invstep[col];
+}
+
+/* PAIR_INT2 / VAL_FUNC2 / DERIV_FUNC2 (src/potaccess.h:323-354, 465-495,
+   591-621): 3-point on rows k, k+1, k+2. */
+static void int2(const orc_table *pt, int col, int inc, double t, double *pot,
+                 double *grad) {
+  int k = (int)t;
+  double chi = t - k;
   const double *ptr = pt->table + (long)k * inc + col;
-  *p0 = ptr[0];
-  *p1 = ptr[inc];
-  *p2 = ptr[2 * inc];
-  *istep_out = istep;
+  double p0 = ptr[0], p1 = ptr[inc], p2 = ptr[2 * inc];
+  double dv = p1 - p0;
+  double d2v = p2 - 2 * p1 + p0;
+  if (pot) *pot = p0 + chi * dv + 0.5 * chi * (chi - 1) * d2v;
+  if (grad) *grad = 2 * pt->invstep[col] * (dv + (chi - 0.5) * d2v);
+}
+
+/* PAIR_INT3 / VAL_FUNC3 / DERIV_FUNC3 (src/potaccess.h:365-412, 502-532,
+   628-660): 4-point on rows k-1 .. k+2, k = max(trunc(t), 1). */
+static void int3(const orc_table *pt, int col, int inc, double t, double *pot,
+                 double *grad) {
+  int k = (int)t;
+  if (k < 1) k = 1;
+  double chi = t - k;
+  const double *ptr = pt->table + (long)(k - 1) * inc + col;
+  double p0 = ptr[0], p1 = ptr[inc], p2 = ptr[2 * inc], p3 = ptr[3 * inc];
+  if (pot) {
+    double fac0 = -(1.0 / 6.0) * chi * (chi - 1.0) * (chi - 2.0);
+    double fac1 = 0.5 * (chi * chi - 1.0) * (chi - 2.0);
+    double fac2 = -0.5 * chi * (chi + 1.0) * (chi - 2.0);
+    double fac3 = (1.0 / 6.0) * chi * (chi * chi - 1.0);
+    *pot = fac0 * p0 + fac1 * p1 + fac2 * p2 + fac3 * p3;
+  }
+  if (grad) {
+    double dfac0 = -(1.0 / 6.0) * ((3.0 * chi - 6.0) * chi + 2.0);
+    double dfac1 = 0.5 * ((3.0 * chi - 4.0) * chi - 1.0);
+    double dfac2 = -0.5 * ((3.0 * chi - 2.0) * chi - 2.0);
+    double dfac3 = 1.0 / 6.0 * (3.0 * chi * chi - 1.0);
+    *grad = 2 * pt->invstep[col] *
+            (dfac0 * p0 + dfac1 * p1 + dfac2 * p2 + dfac3 * p3);
+  }
+}
+
+/* PAIR_INT_SP / VAL_FUNC_SP / DERIV_FUNC_SP (src/potaccess.h:414-457,
+   540-576, 667-702): cubic spline on rows k, k+1 and their second
+   derivatives. */
+static void int_sp(const orc_table *pt, int col, int inc, double t,
+                   double *pot, double *grad) {
+  int k = (int)t;
+  double b = t - k, a = 1.0 - b;
+  double step = pt->step[col], istep = pt->invstep[col];
+  long q = (long)k * inc + col;
+  double p1 = pt->table[q], d21 = pt->table2[q];
+  q += inc;
+  double p2 = pt->table[q], d22 = pt->table2[q];
+  double a2 = a * a - 1, b2 = b * b - 1, st6 = step / 6;
+  if (pot) *pot = a * p1 + b * p2 + (a * a2 * d21 + b * b2 * d22) * st6 * step;
+  if (grad)
+    *grad = 2 * ((p2 - p1) * istep +
+                 ((3 * b2 + 2) * d22 - (3 * a2 + 2) * d21) * st6);
+}
+
+static void tab_eval(const orc_table *pt, int col, int inc, double r2,
+                     int *is_short, double *pot, double *grad) {
+  double t = tab_arg(pt, col, r2, is_short);
+  if (pt->interp == ORC_INTERP_4POINT)
+    int3(pt, col, inc, t, pot, grad);
+  else if (pt->interp == ORC_INTERP_SPLINE)
+    int_sp(pt, col, inc, t, pot, grad);
+  else
+    int2(pt, col, inc, t, pot, grad);
 }
 
 void orc_pair_int2(const orc_table *pt, int col, int inc, double r2,
                    double *pot, double *grad, int *is_short) {
-  double chi, istep, p0, p1, p2;
-  tab_locate(pt, col, inc, r2, is_short, &chi, &istep, &p0, &p1, &p2);
-  double dv = p1 - p0;
-  double d2v = p2 - 2 * p1 + p0;
-  *pot = p0 + chi * dv + 0.5 * chi * (chi - 1) * d2v;
-  *grad = 2 * istep * (dv + (chi - 0.5) * d2v);
+  tab_eval(pt, col, inc, r2, is_short, pot, grad);
 }
 
 double orc_val_func2(const orc_table *pt, int col, int inc, double r2,
                      int *is_short) {
-  double chi, istep, p0, p1, p2;
-  tab_locate(pt, col, inc, r2, is_short, &chi, &istep, &p0, &p1, &p2);
-  double dv = p1 - p0;
-  double d2v = p2 - 2 * p1 + p0;
-  return p0 + chi * dv + 0.5 * chi * (chi - 1) * d2v;
+  double v;
+  tab_eval(pt, col, inc, r2, is_short, &v, NULL);
+  return v;
 }
 
 double orc_deriv_func2(const orc_table *pt, int col, int inc, double r2,
                        int *is_short) {
-  double chi, istep, p0, p1, p2;
-  tab_locate(pt, col, inc, r2, is_short, &chi, &istep, &p0, &p1, &p2);
-  double dv = p1 - p0;
-  double d2v = p2 - 2 * p1 + p0;
-  return 2 * istep * (dv + (chi - 0.5) * d2v);
+  double g;
+  tab_eval(pt, col, inc, r2, is_short, NULL, &g);
+  return g;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -55,7 +55,16 @@ typedef struct {
   double *begin, *end, *step, *invstep;
   int    *len;
   double *table;
+  int     interp;  /* ORC_INTERP_* */
+  double *table2;  /* spline second derivatives (pot_table_t.table2) */
 } orc_table;
+
+/* IMD's table interpolation, a build option there (src/potaccess.h:24-36):
+   3-point (default, PAIR_INT2), FOURPOINT (PAIR_INT3), SPLINE
+   (PAIR_INT_SP). */
+#define ORC_INTERP_3POINT 0
+#define ORC_INTERP_4POINT 1
+#define ORC_INTERP_SPLINE 2
 
 /* Table ingest.  default_format is used for header-less files
    (IMD: 2 in EAM2 builds, 1 otherwise; src/config.h:58-63).  radial != 0
@@ -65,8 +74,15 @@ int  orc_table_read(orc_table *pt, const char *path, int ncols, int radial,
                     int default_format, double *cellsz);
 void orc_table_free(orc_table *pt);
 int  orc_table_nrows(const orc_table *pt);
+/* Switch a read table to another interpolation: the guard rows of
+   init_fourpoint (src/imd_potential.c:1171-1187) or the second derivatives
+   and guard row of init_spline (src/imd_potential.c:1197-1246; radial:
+   first derivative zero at the end, otherwise natural), or back to
+   init_threepoint. */
+int  orc_table_set_interp(orc_table *pt, int interp, int radial);
 
-/* Table access (src/potaccess.h).  inc is the column stride (ncols). */
+/* Table access (src/potaccess.h) in the table's interpolation: PAIR_INT,
+   VAL_FUNC, DERIV_FUNC.  inc is the column stride (ncols). */
 void   orc_pair_int2(const orc_table *pt, int col, int inc, double r2,
                      double *pot, double *grad, int *is_short);
 double orc_val_func2(const orc_table *pt, int col, int inc, double r2,

@@ -23,23 +23,24 @@ def lj():
     return W.lj_table()
 
 
-def oracle_tables(eam: "W.EamTables"):
+def oracle_tables(eam: "W.EamTables", interp=0):
     import oracle as O
-    pair = O.OracleTable.from_table(eam.pair)
-    rho = O.OracleTable.from_table(eam.rho)
-    emb = O.OracleTable.from_table(eam.embed, radial=False)
+    pair = O.OracleTable.from_table(eam.pair, interp=interp)
+    rho = O.OracleTable.from_table(eam.rho, interp=interp)
+    emb = O.OracleTable.from_table(eam.embed, radial=False, interp=interp)
     return pair, rho, emb
 
 
-def oracle_eam(system, eam, **kw):
+def oracle_eam(system, eam, interp=0, **kw):
+    """interp: oracle.INTERP_* (the same numbers as imdg.INTERP_*)"""
     import oracle as O
-    pair, rho, emb = oracle_tables(eam)
+    pair, rho, emb = oracle_tables(eam, interp)
     return O.OracleSystem(system, pair, rho, emb, ntypes=len(eam.masses), **kw)
 
 
-def oracle_lj(system, **kw):
+def oracle_lj(system, interp=0, **kw):
     import oracle as O
-    pair = O.OracleTable.from_table(lj())
+    pair = O.OracleTable.from_table(lj(), interp=interp)
     return O.OracleSystem(system, pair, None, None, ntypes=1, **kw)
 
 

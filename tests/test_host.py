@@ -11,6 +11,7 @@ import pytest
 
 import oracle as O
 from imd_amd import build as B
+from imd_amd import imdg as G
 from imd_amd import workloads as W
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -104,6 +105,7 @@ atomic_e-density_file c.pt
 gpu_precision fp64
 gpu_lanes 2
 gpu_autotune 1
+interpolation spline
 outfiles %s
 simulation 2
 maxsteps 999
@@ -117,6 +119,7 @@ maxsteps 999
     assert abs(g(C.c_double, "isq_tau_eta") - 1 / 9.8227 ** 2) < 1e-15
     assert g(C.c_int, "gpu_precision") == 0
     assert g(C.c_int, "gpu_lanes") == 2 and g(C.c_int, "gpu_autotune") == 1
+    assert g(C.c_int, "interpolation") == G.INTERP_SPLINE
     masses = (C.c_double * 8).in_dll(host, "masses")
     assert masses[0] == 63.546 and masses[1] == 58.69
     assert list((C.c_int * 3).in_dll(host, "pbc_dirs")) == [1, 1, 0]

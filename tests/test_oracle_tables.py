@@ -155,3 +155,84 @@ def test_library_table_relayout_matches_oracle_reader():
         assert ta.maxsteps == info["maxsteps"]
         tab2 = ta.table.reshape(-1, ta.ncols)
         assert np.array_equal(tab2, info["table"])
+
+
+# ---------------------------------------------------------------------------
+# 4-point and spline interpolation (IMD's FOURPOINT / SPLINE build options)
+# ---------------------------------------------------------------------------
+
+def cubic_table(c, begin, end, npts, radial=False):
+    step, xs = W._grid(begin, end, npts)
+    vals = c[0] + xs * (c[1] + xs * (c[2] + xs * c[3]))
+    return W.Table(np.array([begin]), np.array([end]), np.array([step]),
+                   [vals], radial)
+
+
+@pytest.mark.parametrize("c", [(0.5, -0.2, 0.03, -0.001), (1.0, 0.0, 0.0, 0.002)])
+def test_fourpoint_exact_on_cubics(c):
+    """PAIR_INT3 is the Lagrange polynomial through rows k-1..k+2 with
+    k = max(trunc(t), 1), and init_fourpoint continues the last cubic: both
+    exact on cubics, including the first segment (read through k = 1) and
+    the end (guard rows)."""
+    t = cubic_table(c, 1.0, 25.0, 400)
+    ot = O.OracleTable.from_table(t, interp=O.INTERP_4POINT)
+    info = ot.info()
+    f = lambda x: c[0] + x * (c[1] + x * (c[2] + x * c[3]))
+    df = lambda x: c[1] + x * (2 * c[2] + 3 * x * c[3])
+    step = info["step"][0]
+    for k in (400, 401):
+        x = 1.0 + k * step
+        assert abs(info["table"][k, 0] - f(x)) < 1e-10 * max(1.0, abs(f(x)))
+    rng = np.random.default_rng(2)
+    xs = np.concatenate([rng.uniform(1.0, 25.0, 200),
+                         [1.0, 1.0 + 0.3 * step, 25.0 - 0.2 * step, 25.0]])
+    for x in xs:
+        pot, grad, short = ot.pair_int2(0, float(x))
+        assert abs(pot - f(x)) < 1e-10 * max(1.0, abs(f(x))), x
+        assert abs(grad - 2 * df(x)) < 1e-8 * max(1.0, abs(df(x))), x
+        assert ot.val_func2(0, float(x)) == pot
+        assert ot.deriv_func2(0, float(x)) == grad
+
+
+@pytest.mark.parametrize("radial", [False, True])
+def test_spline_matches_scipy(radial):
+    """init_spline + PAIR_INT_SP against scipy's cubic spline through the
+    same rows: natural at the start; at the end the first derivative is
+    zero for radial tables and the spline is natural otherwise."""
+    from scipy.interpolate import CubicSpline
+    begin, end, n = 1.0, 16.0, 300
+    step, xs = W._grid(begin, end, n)
+    g = lambda x: np.exp(-0.3 * x) * np.cos(0.7 * x) + 0.1 * x
+    t = W.Table(np.array([begin]), np.array([end]), np.array([step]),
+                [g(xs)], radial)
+    ot = O.OracleTable.from_table(t, interp=O.INTERP_SPLINE)
+    info = ot.info()
+    x = begin + step * np.arange(n)
+    y = info["table"][:n, 0]               # after the radial shift
+    bc = ((2, 0.0), (1, 0.0)) if radial else "natural"
+    cs = CubicSpline(x, y, bc_type=bc)
+    assert np.allclose(info["table2"][:n, 0], cs(x, 2), rtol=1e-8,
+                       atol=1e-10 * np.abs(cs(x, 2)).max())
+    rng = np.random.default_rng(3)
+    for r2 in np.concatenate([rng.uniform(begin, end, 300), [begin, end]]):
+        pot, grad, short = ot.pair_int2(0, float(r2))
+        want_v = float(cs(r2))
+        want_g = 2 * float(cs(r2, 1))
+        assert abs(pot - want_v) < 1e-11 * max(1.0, abs(want_v)), r2
+        assert abs(grad - want_g) < 1e-9 * max(1.0, abs(want_g)), r2
+
+
+def test_spline_and_fourpoint_differ_from_threepoint():
+    """The three interpolations are distinct on a function that is not a
+    polynomial of low degree (a mode that silently fell back would pass the
+    closed forms above only for its own polynomial degree)."""
+    begin, end, n = 1.0, 16.0, 60
+    step, xs = W._grid(begin, end, n)
+    t = W.Table(np.array([begin]), np.array([end]), np.array([step]),
+                [np.sin(xs)], False)
+    vals = []
+    for m in (O.INTERP_3POINT, O.INTERP_4POINT, O.INTERP_SPLINE):
+        ot = O.OracleTable.from_table(t, interp=m)
+        vals.append(ot.pair_int2(0, 5.37)[0])
+    assert len({round(v, 12) for v in vals}) == 3
+    assert all(abs(v - np.sin(5.37)) < 5e-3 for v in vals)
