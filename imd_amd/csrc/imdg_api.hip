@@ -18,10 +18,11 @@
 namespace imdg {
 template <typename R>
 int step_grid_density(bool nt1, bool lt, size_t lds, int ncu, int lpa,
-                      int threads, bool cub);
+                      int threads, bool cub, bool ee);
 template <typename R>
 int step_grid_force(int mode, bool integrate, bool lt, bool hp, bool he,
-                    size_t lds, int ncu, int lpa, int threads, bool cub);
+                    size_t lds, int ncu, int lpa, int threads, bool cub,
+                    bool ee);
 template <typename R>
 int force_threads(int mode);
 template <typename R>
@@ -77,9 +78,10 @@ struct imdg_ctx {
   hipStream_t stream = nullptr;
   bool mixed = false;
   bool cubic = false;   // 4-point or spline tables (cubic segments)
+  bool eeam = false;    // EEAM energy term present (tab[3])
   int xcd_remap = 1;
   // tables
-  HostTable tab[3];  // pair, rho, embed
+  HostTable tab[4];  // pair, rho, embed, EEAM energy term (emod)
   bool tables_set = false;
   double cellsz_tab = 0.0;
   // geometry
@@ -111,9 +113,10 @@ struct imdg_ctx {
   int* d_nnb = nullptr;
   void *d_reca = nullptr, *d_recb = nullptr;
   void *d_rho = nullptr, *d_epot = nullptr, *d_frc = nullptr;
+  void *d_eeP = nullptr, *d_eedm = nullptr;  // EEAM: P and 2E'(P) per slot
   double *d_partA = nullptr, *d_partB = nullptr, *d_maxd = nullptr;
   ThermoRec* d_ring = nullptr;
-  void* d_tabs = nullptr;        // TabDev<R>[3]
+  void* d_tabs = nullptr;        // TabDev<R>[4]
   void* d_knots = nullptr;       // knots of all tables
   // launch plan of the persistent step kernels
   int ncu = 0;
@@ -333,10 +336,10 @@ template <typename R>
 int upload_tables(imdg_ctx* c) {
   typedef typename V2<R>::T R2;
   std::vector<R2> kn;
-  TabDev<R> td[3];
+  TabDev<R> td[4];
   memset(td, 0, sizeof td);
-  int first[4] = {0, 0, 0, 0};
-  for (int t = 0; t < 3; t++) {
+  int first[5] = {0, 0, 0, 0, 0};
+  for (int t = 0; t < 4; t++) {
     HostTable& h = c->tab[t];
     first[t] = (int)kn.size();
     td[t].ncols = h.present ? h.ncols : 0;
@@ -350,7 +353,7 @@ int upload_tables(imdg_ctx* c) {
       td[t].twoistep[col] = (R)(2 * h.invstep[col]);
       if (c->cubic) {
         std::vector<double> seg;
-        if (!cubic_segments(h, col, t != 2, c->cfg.interpolation, seg))
+        if (!cubic_segments(h, col, t < 2, c->cfg.interpolation, seg))
           return fail(c, IMDG_ERR_NOTABLE,
                       "column %d of table %d is too short for the "
                       "interpolation", col, t);
@@ -373,7 +376,7 @@ int upload_tables(imdg_ctx* c) {
       }
     }
   }
-  first[3] = (int)kn.size();
+  first[4] = (int)kn.size();
   c->knots_pair = first[0];
   c->knots_rho = first[1];
   c->knots_embed = first[2];
@@ -390,7 +393,8 @@ int upload_tables(imdg_ctx* c) {
   //         replaces the cutoff tests.
   c->knots_joint = -1;
   const HostTable &hp = c->tab[0], &hr = c->tab[1];
-  bool same = !c->cubic && c->cfg.ntypes == 1 && hp.present && hr.present &&
+  bool same = !c->cubic && !c->eeam && c->cfg.ntypes == 1 && hp.present &&
+              hr.present &&
               hp.begin[0] == hr.begin[0] && hp.end[0] == hr.end[0] &&
               hp.invstep[0] == hr.invstep[0] && hp.len[0] == hr.len[0];
   if (const char* e = getenv("IMDG_JOINT_TABLE"))
@@ -448,7 +452,8 @@ int upload_tables(imdg_ctx* c) {
   // (p0, A, B, 0) (value p0 + chi (A + B chi), A = dv - d2v/2, B = d2v/2),
   // segment len-1 zero, like the force loop's segment table
   c->knots_dpoly = -1;
-  if (sizeof(R) == 4 && !c->cubic && c->cfg.ntypes == 1 && hr.present) {
+  if (sizeof(R) == 4 && !c->cubic && !c->eeam && c->cfg.ntypes == 1 &&
+      hr.present) {
     if (kn.size() & 1) kn.push_back(kn.back());
     c->knots_dpoly = (int)kn.size();
     const int len = hr.len[0];
@@ -505,7 +510,8 @@ int plan_steps(imdg_ctx* c) {
   int td = c->thr_den, tf = c->thr_frc;
   int no_lds = 0;
   if (const char* e = getenv("IMDG_TABLES_IN_LDS")) no_lds = (atoi(e) == 0);
-  if (cub) no_lds = 1;  // cubic segments: the generic loops on HBM/L2
+  // cubic segments, EEAM: the generic loops with the tables in HBM/L2
+  if (cub || c->eeam) no_lds = 1;
   // density: the rho knots
   c->lds_den_lo = c->knots_rho;
   c->lds_den_n = he ? c->knots_embed - c->knots_rho : 0;
@@ -516,11 +522,12 @@ int plan_steps(imdg_ctx* c) {
   c->grid_den = 0;
   if (he && !no_lds)
     c->grid_den = step_grid_density<R>(nt1, true, c->lds_den_n * sizeof(R2),
-                                       c->ncu, c->lpa_den, td, cub);
+                                       c->ncu, c->lpa_den, td, cub, c->eeam);
   if (he && c->grid_den == 0) {
     c->lds_den_n = 0;
     c->grid_den =
-        step_grid_density<R>(nt1, false, 0, c->ncu, c->lpa_den, td, cub);
+        step_grid_density<R>(nt1, false, 0, c->ncu, c->lpa_den, td, cub,
+                             c->eeam);
   }
   // force: pair and rho knots (or the interleaved table)
   if (mode == 1) {
@@ -536,14 +543,15 @@ int plan_steps(imdg_ctx* c) {
     if (!no_lds)
       g = step_grid_force<R>(mode, integ, true, hp, he,
                              c->lds_frc_n * sizeof(R2), c->ncu, c->lpa_frc,
-                             tf, cub);
+                             tf, cub, c->eeam);
     if (g == 0) {
       c->lds_frc_n = 0;
       g = step_grid_force<R>(mode, integ, false, hp, he, 0, c->ncu,
-                             c->lpa_frc, tf, cub);
+                             c->lpa_frc, tf, cub, c->eeam);
       if (integ == 1)  // keep both variants on the same table placement
         c->grid_frc[0] = step_grid_force<R>(mode, 0, false, hp, he, 0,
-                                            c->ncu, c->lpa_frc, tf, cub);
+                                            c->ncu, c->lpa_frc, tf, cub,
+                                            c->eeam);
     }
     c->grid_frc[integ] = g;
   }
@@ -586,6 +594,9 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.joint = c->knots_joint;
   a.dpoly = c->knots_dpoly;
   a.cubic = c->cubic ? 1 : 0;
+  a.eeam = c->eeam ? 1 : 0;
+  a.eeP = (R*)c->d_eeP;
+  a.eedm = (R*)c->d_eedm;
   a.final_inline = c->final_inline ? 1 : 0;
   a.lpa_den = c->lpa_den;
   a.lpa_frc = c->lpa_frc;
@@ -1101,6 +1112,14 @@ int imdg_set_tables(imdg_ctx* c, const imdg_table* pair, const imdg_table* rho,
   return IMDG_OK;
 }
 
+int imdg_set_eeam_table(imdg_ctx* c, const imdg_table* emod) {
+  if (!c) return IMDG_ERR_ARG;
+  if (c->atoms_set) return fail(c, IMDG_ERR_STATE, "set tables before atoms");
+  if (emod && !c->tab[1].present)
+    return fail(c, IMDG_ERR_NOTABLE, "EEAM needs the EAM tables");
+  return copy_table(c, c->tab[3], emod, c->cfg.ntypes, "eeam");
+}
+
 int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
                    const int* natoms, const double* x, const double* p,
                    const int* type, const double* mass) {
@@ -1231,6 +1250,12 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
   ALLOC(c->d_recb, rs * 4 * ns);
   ALLOC(c->d_rho, rs * ns);
   ALLOC(c->d_epot, rs * ns);
+  c->eeam = c->tab[3].present;
+  c->d_eeP = c->d_eedm = nullptr;
+  if (c->eeam) {
+    ALLOC(c->d_eeP, rs * ns);
+    ALLOC(c->d_eedm, rs * ns);
+  }
   ALLOC(c->d_frc, rs * 4 * ns);
   ALLOC(c->d_maxd, sizeof(double) * c->nblk);
   ALLOC(c->d_ring, sizeof(ThermoRec) * IMDG_RING * nrep);
@@ -1330,6 +1355,10 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
   HIPCHK(c, hipMemsetAsync(c->d_recb, 0, rs * 4 * ns, st));
   HIPCHK(c, hipMemsetAsync(c->d_rho, 0, rs * ns, st));
   HIPCHK(c, hipMemsetAsync(c->d_epot, 0, rs * ns, st));
+  if (c->eeam) {
+    HIPCHK(c, hipMemsetAsync(c->d_eeP, 0, rs * ns, st));
+    HIPCHK(c, hipMemsetAsync(c->d_eedm, 0, rs * ns, st));
+  }
   HIPCHK(c, hipMemsetAsync(c->d_frc, 0, rs * 4 * ns, st));
   HIPCHK(c, hipMemsetAsync(c->d_ring, 0, sizeof(ThermoRec) * IMDG_RING * nrep,
                            st));
@@ -1487,6 +1516,40 @@ int imdg_run(imdg_ctx* c, long nsteps) {
     if (c->stress_on && (rc = enqueue_stress(c))) return rc;
   }
   c->target_steps += nsteps;
+  return IMDG_OK;
+}
+
+int imdg_get_eeam(imdg_ctx* c, double* P, double* dm) {
+  if (!c || !c->atoms_set) return IMDG_ERR_ARG;
+  if (!c->eeam) return fail(c, IMDG_ERR_STATE, "no EEAM table set");
+  hipSetDevice(c->dev);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  SlotView v;
+  int rc = read_slots(c, v, false);
+  if (rc) return rc;
+  size_t ns = c->nslots;
+  std::vector<double> hp(ns), hd(ns);
+  if (c->mixed) {
+    std::vector<float> t(ns);
+    HIPCHK(c, hipMemcpy(t.data(), c->d_eeP, ns * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < ns; k++) hp[k] = t[k];
+    HIPCHK(c, hipMemcpy(t.data(), c->d_eedm, ns * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < ns; k++) hd[k] = t[k];
+  } else {
+    HIPCHK(c, hipMemcpy(hp.data(), c->d_eeP, ns * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hd.data(), c->d_eedm, ns * 8, hipMemcpyDeviceToHost));
+  }
+  for (int r = 0; r < c->nrep; r++) {
+    const RepDev& R = c->rep[r];
+    for (int k = 0; k < R.natoms; k++) {
+      int s = R.atom_off + k;
+      long in = v.aid[s];
+      if (in < 0 || in >= c->natoms_tot)
+        return fail(c, IMDG_ERR_STATE, "slot %d has no atom", s);
+      if (P) P[in] = hp[s];
+      if (dm) dm[in] = hd[s];
+    }
+  }
   return IMDG_OK;
 }
 

@@ -212,6 +212,9 @@ struct StepArgs {
                           // (one species), -1 if there are none
   int cubic;              // tables hold cubic segments, two knots each
                           // (4-point or spline interpolation)
+  int eeam;               // EEAM: second density P = sum rho^2, energy E(P)
+  R* eeP;                 // EEAM P per slot (written with store)
+  R* eedm;                // EEAM 2E'(P) per slot (read by the force loop)
   int lpa_den, lpa_frc;   // lanes per slot of the density / force loops
   int thr_den, thr_frc;   // threads per block of the density / force loops
 };

@@ -604,13 +604,13 @@ struct ForceCols {
 // Four neighbours of the density loop: records and offsets, the prefetch,
 // r^2 and segments, all knot loads together, accumulation with mask
 // multiplies (no branches around the loads).
-template <typename R, bool NT1, bool CUB, typename R4, typename R2,
+template <typename R, bool NT1, bool CUB, bool EE, typename R4, typename R2,
           typename PF>
 __device__ __forceinline__ void density_group(
     uint4 e4, PF pf, uint4& slot, const R4* __restrict__ rec,
     const R4* off, const R4& ui, int it, int nt, const R2* __restrict__ K,
-    const ColPar<R>* cpr, const DensityCol<R>& c0, R& rho, int& is_short,
-    R& r2min) {
+    const ColPar<R>* cpr, const DensityCol<R>& c0, R& rho, R& pee,
+    int& is_short, R& r2min) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
   R4 uj[4], o[4];
 #pragma unroll
@@ -664,7 +664,10 @@ __device__ __forceinline__ void density_group(
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     bool in = r2[u] < ce[u];
-    rho = fma(in ? (R)1 : (R)0, tab_val<CUB, R>(p[u], q[u], chi[u]), rho);
+    R v = tab_val<CUB, R>(p[u], q[u], chi[u]);
+    rho = fma(in ? (R)1 : (R)0, v, rho);
+    // EEAM: P += rho^2 (src/imd_forces_nbl.c:591-593)
+    if (EE) pee = fma(in ? v : (R)0, v, pee);
     if (NT1)
       r2min = r2[u] < r2min ? r2[u] : r2min;  // r2 < begin implies in range
     else
@@ -719,21 +722,23 @@ __device__ __forceinline__ void density_group_poly(
 }
 
 // Four neighbours of the force loop (same phases as density_group).
-template <typename R, bool NT1, bool HP, bool HE, bool CUB, typename R4,
-          typename R2, typename PF>
+template <typename R, bool NT1, bool HP, bool HE, bool CUB, bool EE,
+          typename R4, typename R2, typename PF>
 __device__ __forceinline__ void force_group(
     uint4 e4, PF pf, uint4& slot, const R4* __restrict__ rec,
     const signed char* __restrict__ atype, const R4* off, const R4& ui, R dFi,
     int it, int nt, const R2* __restrict__ K, const ColPar<R>* cpp,
-    const ColPar<R>* cpr, const ForceCols<R>& c, R& fx, R& fy, R& fz, R& ep,
-    R& vir, int& is_short) {
+    const ColPar<R>* cpr, const ForceCols<R>& c, const R* __restrict__ eedm,
+    R dmi, R& fx, R& fy, R& fz, R& ep, R& vir, int& is_short) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
   R4 uj[4], o[4];
   int jt[4];
+  R dmj[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     uj[u] = rec[en[u] & IMDG_IDX_MASK];
     jt[u] = NT1 ? 0 : (int)atype[en[u] & IMDG_IDX_MASK];
+    dmj[u] = EE ? eedm[en[u] & IMDG_IDX_MASK] : (R)0;
     o[u] = off[en[u] >> IMDG_IDX_BITS];
   }
   __builtin_amdgcn_sched_barrier(0);
@@ -784,8 +789,13 @@ __device__ __forceinline__ void force_group(
       if (HE) {
         bool in = r2[u] < c.re;
         R m = in ? (R)0.5 : (R)0;
-        g1 = fma(m * (dFi + uj[u].w),
-                 tab_grad<CUB, R>(q0[u], q1[u], cr[u], c.r2is), g1);
+        R rg = tab_grad<CUB, R>(q0[u], q1[u], cr[u], c.r2is);
+        g1 = fma(m * (dFi + uj[u].w), rg, g1);
+        if (EE) {
+          // (DM_i + DM_j) rho rho' (one species: rho_i = rho_j)
+          R rv = tab_val<CUB, R>(q0[u], q1[u], cr[u]);
+          g1 = fma((in ? dmi + dmj[u] : (R)0) * rv, rg, g1);
+        }
         is_short |= (in && r2[u] < c.rb);
       }
       gg[u] = g1;
@@ -818,15 +828,22 @@ __device__ __forceinline__ void force_group(
           is_short |= (r2[u] < ci.begin);
           R rpi = tab_grad<CUB, R>(K[ci.off + k], K[ci.off + k + 1], chi,
                                    ci.twoistep);
-          R rpj = rpi;
+          R rvi = EE ? tab_val<CUB, R>(K[ci.off + k], K[ci.off + k + 1], chi)
+                     : (R)0;
+          R rpj = rpi, rvj = rvi;
           if (colT != col) {
             k = tab_knot<CUB>(
                 seg_locate(r2[u], cj.begin, cj.end, cj.invstep, cj.len, chi));
             is_short |= (r2[u] < cj.begin);
             rpj = tab_grad<CUB, R>(K[cj.off + k], K[cj.off + k + 1], chi,
                                    cj.twoistep);
+            if (EE)
+              rvj = tab_val<CUB, R>(K[cj.off + k], K[cj.off + k + 1], chi);
           }
           g1 += (R)0.5 * (dFi * rpj + uj[u].w * rpi);
+          // EEAM: DM_i rho_j rho'_j + DM_j rho_i rho'_i
+          // (src/imd_forces_nbl.c:1204-1207)
+          if (EE) g1 += dmi * rvj * rpj + dmj[u] * rvi * rpi;
         }
       }
       gg[u] = g1;
@@ -964,7 +981,7 @@ __device__ __forceinline__ void force_group_sgf(
 // list, then F and 2F' from the embedding column of type(i).  Writes the
 // (u, 2F') records the force loop gathers.  L lanes per slot; CUB: the
 // tables are cubic segments (4-point or spline interpolation).
-template <typename R, bool NT1, bool LT, int L, bool CUB>
+template <typename R, bool NT1, bool LT, int L, bool CUB, bool EE>
 __global__ void __launch_bounds__(IMDG_STEP_THREADS)
     k_density(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
   typedef typename V4<R>::T R4;
@@ -999,18 +1016,18 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
       wave_offsets(Rp, off);
       cur = r;
     }
-    R rho_keep = (R)0;
+    R rho_keep = (R)0, pee_keep = (R)0;
 #pragma unroll 1
     for (int p = 0; p < L; p++) {
       int i = c * IMDG_CHUNK + p * (IMDG_WAVE / L) + ga;
-      R rho = (R)0;
+      R rho = (R)0, pee = (R)0;
       if (i - Rp.atom_off < Rp.natoms) {
         R4 ui = rec[i];
         int it = NT1 ? 0 : (int)ui.w;
         DensityCol<R> col0 = {kb0, len0, b0, e0, is0};
         R r2min = (R)3.0e38;
         const uint32_t* nw = list_words<L>(a.nbr, i, a.kmax, gs);
-        if constexpr (sizeof(R) == 4 && NT1 && !CUB) {
+        if constexpr (sizeof(R) == 4 && NT1 && !CUB && !EE) {
           const float4* __restrict__ DQ =
               reinterpret_cast<const float4*>(K + (a.dpoly - klo));
           const float tc0 = -b0 * is0, tz = (float)len0 - 0.5f;
@@ -1022,17 +1039,22 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
         } else {
           list_walk<L>(nw, a.nnb[i],
                        [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                         density_group<R, NT1, CUB>(e4, pf, slot, rec, off,
-                                                    ui, it, nt, K, cpr, col0,
-                                                    rho, is_short, r2min);
+                         density_group<R, NT1, CUB, EE>(
+                             e4, pf, slot, rec, off, ui, it, nt, K, cpr, col0,
+                             rho, pee, is_short, r2min);
                        });
         }
         if (NT1) is_short |= (r2min < b0);
       }
       rho = lane_group_sum<L>(rho);
       if (gs == p) rho_keep = rho;
+      if (EE) {
+        pee = lane_group_sum<L>(pee);
+        if (gs == p) pee_keep = pee;
+      }
     }
     rho_keep = to_slot_lane<L>(rho_keep, lane);
+    if (EE) pee_keep = to_slot_lane<L>(pee_keep, lane);
     int i = c * IMDG_CHUNK + lane;
     double emb = 0.0;
     if (i - Rp.atom_off < Rp.natoms) {
@@ -1048,6 +1070,19 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
       R2 p = ke[k], q = ke[k + 1];
       R F = tab_val<CUB, R>(p, q, chi);
       R dF = tab_grad<CUB, R>(p, q, chi, te.twoistep[it]);
+      if (EE) {
+        // EEAM energy term E(P) and EAM_DM = 2E'(P)
+        // (src/imd_forces_nbl.c:1090-1095)
+        const TabDev<R>& tm = tabs[3];
+        const R2* __restrict__ km = a.knots + tm.off[it];
+        R chm;
+        int kq = tab_knot<CUB>(seg_locate(pee_keep, tm.begin[it], tm.end[it],
+                                          tm.invstep[it], tm.len[it], chm));
+        R2 pm = km[kq], qm = km[kq + 1];
+        F += tab_val<CUB, R>(pm, qm, chm);
+        a.eedm[i] = tab_grad<CUB, R>(pm, qm, chm, tm.twoistep[it]);
+        if (a.store) a.eeP[i] = pee_keep;
+      }
       R4 rb;
       rb.x = ui.x;
       rb.y = ui.y;
@@ -1084,7 +1119,7 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
 // (calc_forces, move_atoms and check_nblist of one main_loop iteration in
 // one launch).  L lanes per slot.
 template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT,
-          int L, bool CUB>
+          int L, bool CUB, bool EE>
 __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
     k_force(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
   constexpr bool NT1 = MODE != FM_NTN;
@@ -1141,7 +1176,7 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
         int it = NT1 ? 0 : (int)atype[i];
         R dFi = HE ? ui.w : (R)0;
         const uint32_t* nw = list_words<L>(a.nbr, i, a.kmax, gs);
-        if constexpr (MODE == FM_SG && !CUB) {
+        if constexpr (MODE == FM_SG && !CUB && !EE) {
           // one grid: sums in units of 1/(2 istep), scaled once per slot
           R r2min = (R)3.0e38;
           if constexpr (sizeof(R) == 4) {
@@ -1173,11 +1208,13 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
         } else {
           ForceCols<R> fc = {pk0, plen, pb, pe, pis, p2is,
                              rk0, rlen, rb, re, ris, r2is};
+          const R dmi = EE ? a.eedm[i] : (R)0;
           list_walk<L>(nw, a.nnb[i],
                        [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                         force_group<R, NT1, HP, HE, CUB>(
+                         force_group<R, NT1, HP, HE, CUB, EE>(
                              e4, pf, slot, rec, atype, off, ui, dFi, it, nt,
-                             K, cpp, cpr, fc, fx, fy, fz, ep, vir, is_short);
+                             K, cpp, cpr, fc, a.eedm, dmi, fx, fy, fz, ep,
+                             vir, is_short);
                        });
         }
       }
@@ -1194,7 +1231,7 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
         kvir = vir;
       }
     }
-    if (MODE == FM_SG && !CUB) {
+    if (MODE == FM_SG && !CUB && !EE) {
       kx *= p2is;
       ky *= p2is;
       kz *= p2is;
@@ -1316,6 +1353,11 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
     return cub ? cub_grad<R>(K[k0], K[k0 + 1], chi, twoistep)
                : knot_grad<R>(K[k0], K[k0 + 1], chi, twoistep);
   };
+  auto val_at = [&](int k0, R chi) {
+    return cub ? cub_val<R>(K[k0], K[k0 + 1], chi)
+               : knot_val<R>(K[k0], K[k0 + 1], chi);
+  };
+  const bool ee = a.eeam != 0;
   const int kmul = cub ? 2 : 1;
   const int i = b * IMDG_BLOCK + threadIdx.x;
   double s[6] = {0, 0, 0, 0, 0, 0};
@@ -1323,6 +1365,7 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
     const R4 ui = rec[i];
     const int it = a.atype[i];
     const R dFi = he ? ui.w : (R)0;
+    const R dmi = ee ? a.eedm[i] : (R)0;
     R t[6] = {0, 0, 0, 0, 0, 0};
     const uint4* lw = list_of(a.nbr, i, a.kmax);
     const int ngroup = (a.nnb[i] + IMDG_GROUP - 1) / IMDG_GROUP;
@@ -1355,13 +1398,16 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
           int k = seg_locate(r2, tr.begin[colT], tr.end[colT],
                              tr.invstep[colT], tr.len[colT], chi);
           R rpi = grad_at(tr.off[colT] + kmul * k, chi, tr.twoistep[colT]);
-          R rpj = rpi;
+          R rvi = ee ? val_at(tr.off[colT] + kmul * k, chi) : (R)0;
+          R rpj = rpi, rvj = rvi;
           if (colT != col) {
             k = seg_locate(r2, tr.begin[col], tr.end[col], tr.invstep[col],
                            tr.len[col], chi);
             rpj = grad_at(tr.off[col] + kmul * k, chi, tr.twoistep[col]);
+            if (ee) rvj = val_at(tr.off[col] + kmul * k, chi);
           }
           g += (R)0.5 * (dFi * rpj + uj.w * rpi);
+          if (ee) g += dmi * rvj * rpj + a.eedm[j] * rvi * rpi;
         }
         t[0] = fma(dx * dx, g, t[0]);
         t[1] = fma(dy * dy, g, t[1]);
@@ -1400,65 +1446,75 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
 // launch plans and launchers
 // ---------------------------------------------------------------------------
 
-template <typename R, bool NT1, bool LT, bool CUB>
+template <typename R, bool NT1, bool LT, bool CUB, bool EE>
 const void* density_fn(int lpa) {
   if (lpa == 4)
-    return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 4, CUB>);
+    return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 4, CUB, EE>);
   if (lpa == 2)
-    return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 2, CUB>);
-  return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 1, CUB>);
+    return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 2, CUB, EE>);
+  return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 1, CUB, EE>);
 }
 
 template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT,
-          bool CUB>
+          bool CUB, bool EE>
 const void* force_fn3(int lpa) {
   if (lpa == 4)
     return reinterpret_cast<const void*>(
-        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 4, CUB>);
+        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 4, CUB, EE>);
   if (lpa == 2)
     return reinterpret_cast<const void*>(
-        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 2, CUB>);
+        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 2, CUB, EE>);
   return reinterpret_cast<const void*>(
-      &k_force<R, MODE, INTEGRATE, HP, HE, LT, 1, CUB>);
+      &k_force<R, MODE, INTEGRATE, HP, HE, LT, 1, CUB, EE>);
 }
 
-template <typename R, int MODE, bool INTEGRATE, bool LT, bool CUB>
+template <typename R, int MODE, bool INTEGRATE, bool LT, bool CUB, bool EE>
 const void* force_fn2(bool hp, bool he, int lpa) {
   if (MODE == FM_SG)
-    return force_fn3<R, MODE, INTEGRATE, true, true, LT, CUB>(lpa);
-  if (hp && he) return force_fn3<R, MODE, INTEGRATE, true, true, LT, CUB>(lpa);
-  if (hp) return force_fn3<R, MODE, INTEGRATE, true, false, LT, CUB>(lpa);
-  return force_fn3<R, MODE, INTEGRATE, false, true, LT, CUB>(lpa);
+    return force_fn3<R, MODE, INTEGRATE, true, true, LT, CUB, EE>(lpa);
+  if (hp && he)
+    return force_fn3<R, MODE, INTEGRATE, true, true, LT, CUB, EE>(lpa);
+  if (EE) return nullptr;  // EEAM needs the EAM tables
+  if (hp) return force_fn3<R, MODE, INTEGRATE, true, false, LT, CUB, EE>(lpa);
+  return force_fn3<R, MODE, INTEGRATE, false, true, LT, CUB, EE>(lpa);
 }
 
 template <typename R, int MODE>
 const void* force_fn1(bool integrate, bool lt, bool hp, bool he, int lpa) {
   if (integrate)
-    return lt ? force_fn2<R, MODE, true, true, false>(hp, he, lpa)
-              : force_fn2<R, MODE, true, false, false>(hp, he, lpa);
-  return lt ? force_fn2<R, MODE, false, true, false>(hp, he, lpa)
-            : force_fn2<R, MODE, false, false, false>(hp, he, lpa);
+    return lt ? force_fn2<R, MODE, true, true, false, false>(hp, he, lpa)
+              : force_fn2<R, MODE, true, false, false, false>(hp, he, lpa);
+  return lt ? force_fn2<R, MODE, false, true, false, false>(hp, he, lpa)
+            : force_fn2<R, MODE, false, false, false, false>(hp, he, lpa);
 }
 
-// The cubic-segment tables (4-point, spline) take the generic loops with
-// the tables in HBM/L2 (two knots per segment: twice the 3-point size).
+// The table variants (cubic segments of the 4-point and spline
+// interpolations, EEAM) take the generic loops with the tables in HBM/L2.
+template <typename R, int MODE, bool CUB, bool EE>
+const void* force_fn1_var(bool integrate, bool hp, bool he, int lpa) {
+  return integrate ? force_fn2<R, MODE, true, false, CUB, EE>(hp, he, lpa)
+                   : force_fn2<R, MODE, false, false, CUB, EE>(hp, he, lpa);
+}
+
 template <typename R, int MODE>
-const void* force_fn1_cub(bool integrate, bool hp, bool he, int lpa) {
-  return integrate ? force_fn2<R, MODE, true, false, true>(hp, he, lpa)
-                   : force_fn2<R, MODE, false, false, true>(hp, he, lpa);
+const void* force_fn_var(bool integrate, bool hp, bool he, int lpa, bool cub,
+                         bool ee) {
+  if (cub && ee) return force_fn1_var<R, MODE, true, true>(integrate, hp, he, lpa);
+  if (cub) return force_fn1_var<R, MODE, true, false>(integrate, hp, he, lpa);
+  return force_fn1_var<R, MODE, false, true>(integrate, hp, he, lpa);
 }
 
 // mode: 0 one species, 1 one species on a shared grid (needs hp and he),
 // 2 several species; lpa: lanes per slot (1, 2 or 4); cub: cubic-segment
-// tables (modes 0 and 2 only, lt false)
+// tables, ee: EEAM (both: modes 0 and 2 only, lt false)
 template <typename R>
 const void* force_fn(int mode, bool integrate, bool lt, bool hp, bool he,
-                     int lpa, bool cub) {
-  if (cub) {
+                     int lpa, bool cub, bool ee) {
+  if (cub || ee) {
     if (lt || mode == FM_SG) return nullptr;
     if (mode == FM_NT1)
-      return force_fn1_cub<R, FM_NT1>(integrate, hp, he, lpa);
-    return force_fn1_cub<R, FM_NTN>(integrate, hp, he, lpa);
+      return force_fn_var<R, FM_NT1>(integrate, hp, he, lpa, cub, ee);
+    return force_fn_var<R, FM_NTN>(integrate, hp, he, lpa, cub, ee);
   }
   if (mode == FM_SG)
     return force_fn1<R, FM_SG>(integrate, lt, true, true, lpa);
@@ -1466,18 +1522,25 @@ const void* force_fn(int mode, bool integrate, bool lt, bool hp, bool he,
   return force_fn1<R, FM_NTN>(integrate, lt, hp, he, lpa);
 }
 
+template <typename R, bool CUB, bool EE>
+const void* density_fn_var(bool nt1, int lpa) {
+  return nt1 ? density_fn<R, true, false, CUB, EE>(lpa)
+             : density_fn<R, false, false, CUB, EE>(lpa);
+}
+
 template <typename R>
-const void* density_fn(bool nt1, bool lt, int lpa, bool cub) {
-  if (cub) {
+const void* density_fn(bool nt1, bool lt, int lpa, bool cub, bool ee) {
+  if (cub || ee) {
     if (lt) return nullptr;
-    return nt1 ? density_fn<R, true, false, true>(lpa)
-               : density_fn<R, false, false, true>(lpa);
+    if (cub && ee) return density_fn_var<R, true, true>(nt1, lpa);
+    if (cub) return density_fn_var<R, true, false>(nt1, lpa);
+    return density_fn_var<R, false, true>(nt1, lpa);
   }
   if (nt1)
-    return lt ? density_fn<R, true, true, false>(lpa)
-              : density_fn<R, true, false, false>(lpa);
-  return lt ? density_fn<R, false, true, false>(lpa)
-            : density_fn<R, false, false, false>(lpa);
+    return lt ? density_fn<R, true, true, false, false>(lpa)
+              : density_fn<R, true, false, false, false>(lpa);
+  return lt ? density_fn<R, false, true, false, false>(lpa)
+            : density_fn<R, false, false, false, false>(lpa);
 }
 
 // Persistent grid of a kernel: resident blocks per CU (from the occupancy
@@ -1519,8 +1582,9 @@ static int persistent_grid(const void* fn, int threads, size_t lds, int ncu) {
 
 template <typename R>
 int step_grid_density(bool nt1, bool lt, size_t lds, int ncu, int lpa,
-                      int threads, bool cub) {
-  return persistent_grid(density_fn<R>(nt1, lt, lpa, cub), threads, lds, ncu);
+                      int threads, bool cub, bool ee) {
+  return persistent_grid(density_fn<R>(nt1, lt, lpa, cub, ee), threads, lds,
+                         ncu);
 }
 
 template <typename R>
@@ -1532,10 +1596,12 @@ int force_threads(int mode) {
 
 template <typename R>
 int step_grid_force(int mode, bool integrate, bool lt, bool hp, bool he,
-                    size_t lds, int ncu, int lpa, int threads, bool cub) {
+                    size_t lds, int ncu, int lpa, int threads, bool cub,
+                    bool ee) {
   if (threads > force_threads<R>(mode)) return 0;
-  return persistent_grid(force_fn<R>(mode, integrate, lt, hp, he, lpa, cub),
-                         threads, lds, ncu);
+  return persistent_grid(
+      force_fn<R>(mode, integrate, lt, hp, he, lpa, cub, ee), threads, lds,
+      ncu);
 }
 
 template <typename R>
@@ -1554,7 +1620,8 @@ hipError_t launch_density(const StepArgs<R>& a, const TabDev<R>* tabs,
     const TabDev<R>* t;
   } pk = {a, tabs};
   void* kargs[] = {&pk.a, &pk.t};
-  const void* fn = density_fn<R>(nt1, lt, a.lpa_den, a.cubic != 0);
+  const void* fn =
+      density_fn<R>(nt1, lt, a.lpa_den, a.cubic != 0, a.eeam != 0);
   if (!fn) return hipErrorInvalidDeviceFunction;
   return hipLaunchKernel(fn, dim3(grid), dim3(a.thr_den), kargs, lds, st);
 }
@@ -1567,8 +1634,9 @@ hipError_t launch_force(const StepArgs<R>& a, const TabDev<R>* tabs,
   StepArgs<R> aa = a;
   const TabDev<R>* tt = tabs;
   void* kargs[] = {&aa, &tt};
-  const void* fn = force_fn<R>(mode, integrate, lt, a.has_pair != 0,
-                               a.has_eam != 0, a.lpa_frc, a.cubic != 0);
+  const void* fn =
+      force_fn<R>(mode, integrate, lt, a.has_pair != 0, a.has_eam != 0,
+                  a.lpa_frc, a.cubic != 0, a.eeam != 0);
   if (!fn) return hipErrorInvalidDeviceFunction;
   return hipLaunchKernel(fn, dim3(grid), dim3(a.thr_frc), kargs, lds, st);
 }
@@ -1611,9 +1679,9 @@ hipError_t launch_stress(const StepArgs<R>& a, const TabDev<R>* tabs,
 
 #define IMDG_INSTANTIATE(R)                                                   \
   template int step_grid_density<R>(bool, bool, size_t, int, int, int,     \
-                                    bool);                                    \
+                                    bool, bool);                              \
   template int step_grid_force<R>(int, bool, bool, bool, bool, size_t, int,  \
-                                  int, int, bool);                            \
+                                  int, int, bool, bool);                      \
   template int force_threads<R>(int);                                         \
   template int step_grid_move<R>(int);                                        \
   template hipError_t launch_density<R>(const StepArgs<R>&,                   \

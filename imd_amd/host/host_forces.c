@@ -76,6 +76,10 @@ int imd_gpu_init(void) {
   check(imdg_set_tables(ctx, have_potfile ? &tp : NULL, have_eam ? &tr : NULL,
                         have_eam ? &te : NULL),
         "imdg_set_tables");
+  if (have_eeam) {
+    imdg_table tm = as_imdg(&emod_pot);
+    check(imdg_set_eeam_table(ctx, &tm), "imdg_set_eeam_table");
+  }
   imdg_replica rep;
   memset(&rep, 0, sizeof rep);
   memcpy(rep.box_x, box_x, sizeof box_x);
@@ -158,6 +162,8 @@ void fetch_atoms(void) {
   check(imdg_get_atoms(ctx, atom_x, atom_p, NULL, atom_epot,
                        have_eam ? atom_rho : NULL, NULL),
         "imdg_get_atoms");
+  if (have_eeam)
+    check(imdg_get_eeam(ctx, atom_eeam_p, NULL), "imdg_get_eeam");
 }
 
 static double wall(void) {

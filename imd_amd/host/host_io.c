@@ -45,6 +45,7 @@ long natoms = 0;
 int nactive = 0;
 double *atom_x = NULL, *atom_p = NULL, *atom_mass = NULL;
 double *atom_epot = NULL, *atom_rho = NULL, *atom_press = NULL;
+double *atom_eeam_p = NULL;
 double tot_presstens[6];
 int *atom_type = NULL;
 long *atom_number = NULL;
@@ -64,8 +65,9 @@ static int grow_atoms(long cap) {
   atom_epot = realloc(atom_epot, sizeof(double) * cap);
   atom_rho = realloc(atom_rho, sizeof(double) * cap);
   atom_press = realloc(atom_press, sizeof(double) * 6 * cap);
+  atom_eeam_p = realloc(atom_eeam_p, sizeof(double) * cap);
   return (atom_x && atom_p && atom_mass && atom_type && atom_number &&
-          atom_epot && atom_rho && atom_press) ? 0 : 1;
+          atom_epot && atom_rho && atom_press && atom_eeam_p) ? 0 : 1;
 }
 
 /* the volume of make_box (src/imd_geom_3d.c:52-104) */
@@ -259,10 +261,10 @@ int write_config(int fzhlr, int steps) {
   out_name(fname, sizeof fname, fzhlr, "chkpt");
   FILE *f = fopen(fname, "w");
   if (!f) imd_error("Cannot open output file %s", fname);
-  int nd = have_eam ? 2 : 1;
+  int nd = 1 + (have_eam ? 1 : 0) + (have_eeam ? 1 : 0);
   fprintf(f, "#F A 1 1 1 3 3 %d\n", nd);
-  fprintf(f, "#C number type mass x y z vx vy vz Epot%s\n",
-          have_eam ? " eam_rho" : "");
+  fprintf(f, "#C number type mass x y z vx vy vz Epot%s%s\n",
+          have_eam ? " eam_rho" : "", have_eeam ? " eam_p" : "");
   write_box_header(f, steps);
   for (long i = 0; i < natoms; i++) {
     double m = atom_mass[i];
@@ -271,6 +273,7 @@ int write_config(int fzhlr, int steps) {
             atom_x[3 * i + 2], atom_p[3 * i] / m, atom_p[3 * i + 1] / m,
             atom_p[3 * i + 2] / m, atom_epot[i]);
     if (have_eam) fprintf(f, " %.17g", atom_rho[i]);
+    if (have_eeam) fprintf(f, " %.17g", atom_eeam_p[i]);
     fputc('\n', f);
   }
   if (fclose(f)) imd_error("Cannot write output file %s", fname);

@@ -50,11 +50,11 @@ char infilename[IMD_STRLEN], outfilename[IMD_STRLEN] = "imd";
 char itrfilename[IMD_STRLEN], paramfilename[IMD_STRLEN];
 char progname[IMD_STRLEN] = "imd_gpu";
 char potfilename[IMD_STRLEN], eam2_emb_E_filename[IMD_STRLEN];
-char eam2_at_rho_filename[IMD_STRLEN];
+char eam2_at_rho_filename[IMD_STRLEN], eeam_mod_E_filename[IMD_STRLEN];
 int gpu_precision = IMDG_MIXED, gpu_device = 0;
 int gpu_lanes = 0, gpu_autotune = 0;
 int interpolation = IMDG_INTERP_3POINT;
-int have_potfile = 0, have_eam = 0;
+int have_potfile = 0, have_eam = 0, have_eeam = 0;
 
 void imd_error(const char *fmt, ...) {
   va_list ap;
@@ -99,14 +99,14 @@ void init_parameters(void) {
   maxwalltime = 0.0;
   infilename[0] = potfilename[0] = eam2_emb_E_filename[0] = 0;
   itrfilename[0] = 0;
-  eam2_at_rho_filename[0] = 0;
+  eam2_at_rho_filename[0] = eeam_mod_E_filename[0] = 0;
   strcpy(outfilename, "imd");
   gpu_precision = IMDG_MIXED;
   gpu_device = 0;
   gpu_lanes = 0;
   gpu_autotune = 0;
   interpolation = IMDG_INTERP_3POINT;
-  have_potfile = have_eam = 0;
+  have_potfile = have_eam = have_eeam = 0;
 }
 
 static int curline;
@@ -207,6 +207,10 @@ static int getparamfile(const char *paramfile, int phase) {
     } else if (!strcasecmp(key, "atomic_e-density_file")) {
       get_str(key, eam2_at_rho_filename);
       have_eam = 1;
+    } else if (!strcasecmp(key, "eeam_energy_file")) {
+      /* EEAM energy term E(P) (src/imd_param.c:2615-2619) */
+      get_str(key, eeam_mod_E_filename);
+      have_eeam = 1;
     } else if (!strcasecmp(key, "ntypes")) {
       int nt;
       get_ints(key, &nt, 1);

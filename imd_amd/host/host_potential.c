@@ -19,7 +19,7 @@
 
 #define PSTEP 50 /* table growth step of format 1 (src/config.h:251) */
 
-pot_table_t pair_pot, embed_pot, rho_h_tab;
+pot_table_t pair_pot, embed_pot, rho_h_tab, emod_pot;
 double cellsz = 0.0;
 
 void free_pot_table(pot_table_t *pt) {
@@ -182,7 +182,8 @@ int read_pot_table(pot_table_t *pt, const char *filename, int ncols,
 }
 
 /* setup_potentials (src/imd_potential.c:43-131): pair table with ntypes^2
-   columns, EAM embedding (ntypes columns) and density (ntypes^2). */
+   columns, EAM embedding (ntypes columns) and density (ntypes^2), and with
+   EEAM the energy term E(P) (ntypes columns, src/imd_potential.c:82-85). */
 int setup_potentials(void) {
   int rc;
   cellsz = 0.0;
@@ -197,6 +198,12 @@ int setup_potentials(void) {
     rc = read_pot_table(&rho_h_tab, eam2_at_rho_filename, ntypes * ntypes, 1);
     if (rc)
       imd_error("cannot read density table %s (%d)", eam2_at_rho_filename, rc);
+  }
+  if (have_eeam) {
+    if (!have_eam) imd_error("eeam_energy_file needs the EAM tables");
+    rc = read_pot_table(&emod_pot, eeam_mod_E_filename, ntypes, 0);
+    if (rc)
+      imd_error("cannot read EEAM table %s (%d)", eeam_mod_E_filename, rc);
   }
   return 0;
 }

@@ -57,12 +57,12 @@ extern char infilename[IMD_STRLEN], outfilename[IMD_STRLEN];
 extern char itrfilename[IMD_STRLEN], paramfilename[IMD_STRLEN];
 extern char progname[IMD_STRLEN];
 extern char potfilename[IMD_STRLEN], eam2_emb_E_filename[IMD_STRLEN];
-extern char eam2_at_rho_filename[IMD_STRLEN];
+extern char eam2_at_rho_filename[IMD_STRLEN], eeam_mod_E_filename[IMD_STRLEN];
 extern int gpu_precision, gpu_device, gpu_lanes, gpu_autotune;
 extern int interpolation;  /* IMDG_INTERP_* */
-extern int have_potfile, have_eam;
+extern int have_potfile, have_eam, have_eeam;
 
-extern pot_table_t pair_pot, embed_pot, rho_h_tab;
+extern pot_table_t pair_pot, embed_pot, rho_h_tab, emod_pot;
 extern double cellsz;
 
 /* atoms, in input order */
@@ -70,6 +70,7 @@ extern long natoms;
 extern int nactive;
 extern double *atom_x, *atom_p, *atom_mass;
 extern double *atom_epot, *atom_rho; /* per-atom Epot and eam_rho */
+extern double *atom_eeam_p;          /* EEAM: eam_p (EAM_P) */
 extern double *atom_press;           /* 6 per atom: xx yy zz yz zx xy */
 extern double tot_presstens[6];      /* sum over the atoms */
 extern int *atom_type;

@@ -30,6 +30,7 @@ SYMBOLS = [
     "imdg_get_pairs", "imdg_get_grid", "imdg_get_info", "imdg_profile",
     "imdg_kernel_times", "imdg_struct_size", "imdg_get_tuning",
     "imdg_set_tuning", "imdg_autotune", "imdg_set_stress", "imdg_get_stress",
+    "imdg_set_eeam_table", "imdg_get_eeam",
 ]
 
 
@@ -118,6 +119,8 @@ def lib():
         L.imdg_get_pairs.argtypes = [vp, lp, C.c_long, lp]
         L.imdg_get_grid.argtypes = [vp, C.c_int, ip]
         L.imdg_set_stress.argtypes = [vp, C.c_int]
+        L.imdg_set_eeam_table.argtypes = [vp, C.POINTER(Table)]
+        L.imdg_get_eeam.argtypes = [vp, dp, dp]
         L.imdg_get_stress.argtypes = [vp, dp, dp]
         L.imdg_get_info.argtypes = [vp, C.POINTER(Info)]
         L.imdg_profile.argtypes = [vp, C.c_int]
@@ -239,6 +242,22 @@ class Context:
         ptrs = [C.byref(s) if s is not None else C.POINTER(Table)()
                 for s in structs]
         self._chk(lib().imdg_set_tables(self._h, *ptrs), "imdg_set_tables")
+
+    def set_eeam_table(self, emod: Optional[TableArrays]):
+        """EEAM energy term E(P) (ntypes columns); before set_systems."""
+        st = emod.cstruct() if emod is not None else None
+        self._keep_eeam = (emod, st)
+        self._chk(lib().imdg_set_eeam_table(
+            self._h, C.byref(st) if st is not None else C.POINTER(Table)()),
+            "imdg_set_eeam_table")
+
+    def eeam(self):
+        """EEAM P_i and 2E'(P_i) per atom (input order)."""
+        n = sum(self.natoms)
+        P, dm = np.zeros(n), np.zeros(n)
+        self._chk(lib().imdg_get_eeam(self._h, _dp(P), _dp(dm)),
+                  "imdg_get_eeam")
+        return P, dm
 
     def set_systems(self, systems: Sequence["W.System"], eta=0.0,
                     isq_tau_eta=0.0):
@@ -415,14 +434,17 @@ class Context:
 
 def eam_context(tables: "W.EamTables", systems, precision=MIXED,
                 ensemble=NVE, timestep=W.FS, nbl_margin=0.4, eta=0.0,
-                isq_tau_eta=0.0, **kw) -> Context:
-    """Context with IMD-read EAM tables and the given systems."""
+                isq_tau_eta=0.0, emod=None, **kw) -> Context:
+    """Context with IMD-read EAM tables and the given systems; emod: an
+    EEAM energy-term table (workloads.Table, ntypes columns)."""
     ntypes = len(tables.masses)
     ctx = Context(precision=precision, ensemble=ensemble, ntypes=ntypes,
                   timestep=timestep, nbl_margin=nbl_margin, **kw)
     ctx.set_tables(TableArrays.from_format2(tables.pair),
                    TableArrays.from_format2(tables.rho),
                    TableArrays.from_format2(tables.embed))
+    if emod is not None:
+        ctx.set_eeam_table(TableArrays.from_format2(emod))
     ctx.set_systems(systems, eta=eta, isq_tau_eta=isq_tau_eta)
     return ctx
 

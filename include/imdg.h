@@ -146,6 +146,13 @@ const char *imdg_last_error(const imdg_ctx *ctx);
    build (imd_nbl_nve). */
 int  imdg_set_tables(imdg_ctx *ctx, const imdg_table *pair,
                      const imdg_table *rho, const imdg_table *embed);
+/* EEAM (src/imd_forces_nbl.c:591-608, 1090-1095, 1180-1207): the energy
+   modification term E(P) of the second density P_i = sum_j rho_ij(r^2)^2,
+   ntypes columns, not radial (read_pot_table(&emod_pot, ...),
+   src/imd_potential.c:82-85; IMD key eeam_energy_file).  The force gains
+   2E'(P_i) rho_j rho'_j + 2E'(P_j) rho_i rho'_i per pair.  After
+   imdg_set_tables, before imdg_set_atoms; NULL removes it. */
+int  imdg_set_eeam_table(imdg_ctx *ctx, const imdg_table *emod);
 
 /* read_atoms (src/imd_io_3d.c:44-520) for nrep replicas.  Arrays hold the
    atoms of replica 0, then replica 1, ... (natoms[r] each): x and p 3 per
@@ -215,6 +222,11 @@ int  imdg_set_stress(imdg_ctx *ctx, int enable);
    order xx yy zz yz zx xy; either pointer may be NULL.  IMDG_ERR_STATE if
    no tensor of the current state was computed. */
 int  imdg_get_stress(imdg_ctx *ctx, double *per_atom, double *total);
+
+/* EEAM per-atom P (EAM_P, from the last step's force evaluation) and
+   2E'(P) (EAM_DM), input order; either pointer may be NULL.
+   IMDG_ERR_STATE without an EEAM table. */
+int  imdg_get_eeam(imdg_ctx *ctx, double *P, double *dm);
 int  imdg_get_info(imdg_ctx *ctx, imdg_info *info);
 
 /* Kernel timing with HIP events on the context's stream.  enable=1 starts

@@ -74,6 +74,8 @@ def lib():
         L.orc_get_grid.argtypes = [C.c_void_p, ip]
         L.orc_get_cells.argtypes = [C.c_void_p, ip]
         L.orc_set_press.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_eeam.argtypes = [C.c_void_p, C.POINTER(_Table)]
+        L.orc_get_eeam.argtypes = [C.c_void_p, dp, dp]
         L.orc_get_press.argtypes = [C.c_void_p, dp]
         L.orc_get_pairs.argtypes = [C.c_void_p, lp, C.c_long]
         L.orc_get_pairs.restype = C.c_long
@@ -225,6 +227,20 @@ class OracleSystem:
                             _dp(out["f"]), _dp(out["epot"]), _dp(out["rho"]),
                             _dp(out["df"]))
         return out
+
+    def set_eeam(self, emod):
+        """EEAM energy term E(P) (an OracleTable, ntypes columns)."""
+        self._emod = emod
+        rc = lib().orc_set_eeam(self._s, C.byref(emod._t) if emod else
+                                C.POINTER(_Table)())
+        if rc:
+            raise ValueError("orc_set_eeam failed %d" % rc)
+
+    def eeam(self):
+        """P_i = sum_j rho_ij^2 and 2E'(P_i), caller order."""
+        P, dm = np.zeros(self.n), np.zeros(self.n)
+        lib().orc_get_eeam(self._s, _dp(P), _dp(dm))
+        return P, dm
 
     def set_press(self, on=True):
         lib().orc_set_press(self._s, int(on))

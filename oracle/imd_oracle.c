@@ -392,6 +392,7 @@ struct orc_sys {
   int ensemble, ntypes;
   double timestep, temperature, eta, isq_tau_eta;
   const orc_table *pair, *rho_t, *embed;
+  const orc_table *emod; /* EEAM energy modification term E(P), or NULL */
   /* real atoms, caller order */
   int n;
   long *id;
@@ -404,6 +405,7 @@ struct orc_sys {
   int *stype, *ssrc, *sroot, *sshift; /* sshift: 3 per slot */
   double *svec; /* shift vector added when the slot was copied, 3 per slot */
   double *sf, *sepot, *srho, *sdf;
+  double *sP, *sdm; /* EEAM: P = sum rho^2 and 2E'(P) (EAM_P, EAM_DM) */
   /* per-slot stress tensor xx yy zz yz zx xy, when do_press is set */
   double *spress;
   int do_press;
@@ -528,6 +530,8 @@ void orc_free(orc_sys *s) {
   free(s->sf);
   free(s->sepot);
   free(s->spress);
+  free(s->sP);
+  free(s->sdm);
   free(s->srho);
   free(s->sdf);
   free(s->tl);
@@ -546,6 +550,12 @@ int orc_set_tables(orc_sys *s, const orc_table *pair, const orc_table *rho,
   s->pair = pair;
   s->rho_t = rho;
   s->embed = embed;
+  return ORC_OK;
+}
+
+int orc_set_eeam(orc_sys *s, const orc_table *emod) {
+  if (emod && (!s->rho_t || emod->ncols != s->ntypes)) return ORC_ERR_ARG;
+  s->emod = emod;
   return ORC_OK;
 }
 
@@ -653,6 +663,8 @@ static int slot_reserve(orc_sys *s, int need) {
   GROW(s->sepot, double, 1);
   GROW(s->srho, double, 1);
   GROW(s->sdf, double, 1);
+  GROW(s->sP, double, 1);
+  GROW(s->sdm, double, 1);
   GROW(s->spress, double, 6);
 #undef GROW
   s->slotcap = nc;
@@ -831,6 +843,13 @@ static void press_kin(double *pi, const double *p, double m) {
 
 void orc_set_press(orc_sys *s, int on) { s->do_press = on; }
 
+void orc_get_eeam(const orc_sys *s, double *P, double *dm) {
+  for (int i = 0; i < s->n; i++) {
+    if (P) P[i] = s->sP ? s->sP[i] : 0.0;
+    if (dm) dm[i] = s->sdm ? s->sdm[i] : 0.0;
+  }
+}
+
 void orc_get_press(const orc_sys *s, double *out) {
   for (int i = 0; i < s->n; i++)
     for (int k = 0; k < 6; k++)
@@ -854,6 +873,7 @@ int orc_calc_forces(orc_sys *s, int steps) {
     s->sf[3 * q] = s->sf[3 * q + 1] = s->sf[3 * q + 2] = 0.0;
     s->sepot[q] = 0.0;
     s->srho[q] = 0.0;
+    s->sP[q] = 0.0;
     if (s->do_press)
       for (int k = 0; k < 6; k++) s->spress[6 * q + k] = 0.0;
   }
@@ -863,7 +883,7 @@ int orc_calc_forces(orc_sys *s, int steps) {
   for (int n = 0; n < s->n; n++) {
     int ia = s->tl_atom[n];
     const double *d1 = s->sx + 3 * ia;
-    double ff[3] = {0, 0, 0}, ee = 0.0, eam_r = 0.0;
+    double ff[3] = {0, 0, 0}, ee = 0.0, eam_r = 0.0, eam_p = 0.0;
     double ps[6] = {0, 0, 0, 0, 0, 0};
     int it = s->stype[ia];
     for (int m = s->tl[n]; m < s->tl[n + 1]; m++) {
@@ -895,12 +915,17 @@ int orc_calc_forces(orc_sys *s, int steps) {
         if (r2 < rt->end[col]) {
           rho_h = orc_val_func2(rt, col, inc, r2, &is_short);
           eam_r += rho_h;
+          eam_p += rho_h * rho_h; /* EEAM (src/imd_forces_nbl.c:591-593) */
         }
         if (it == jt) {
-          if (r2 < rt->end[col]) s->srho[js] += rho_h;
+          if (r2 < rt->end[col]) {
+            s->srho[js] += rho_h;
+            s->sP[js] += rho_h * rho_h;
+          }
         } else if (r2 < rt->end[col2]) {
           rho_h = orc_val_func2(rt, col2, inc, r2, &is_short);
           s->srho[js] += rho_h;
+          s->sP[js] += rho_h * rho_h;
         }
       }
     }
@@ -909,12 +934,16 @@ int orc_calc_forces(orc_sys *s, int steps) {
     s->sf[3 * ia + 2] += ff[2];
     s->sepot[ia] += ee;
     s->srho[ia] += eam_r;
+    s->sP[ia] += eam_p;
     if (s->do_press)
       for (int k = 0; k < 6; k++) s->spress[6 * ia + k] += ps[k];
   }
   if (rt) {
     /* add_rho: buffer copies back to their sources (x, then y, then z) */
-    for (int q = s->nslot - 1; q >= s->n; q--) s->srho[s->ssrc[q]] += s->srho[q];
+    for (int q = s->nslot - 1; q >= s->n; q--) {
+      s->srho[s->ssrc[q]] += s->srho[q];
+      s->sP[s->ssrc[q]] += s->sP[q];
+    }
     /* embedding energy and 2F' (src/imd_forces_nbl.c:1079-1112) */
     int idummy = 0;
     for (int c = 0; c < s->ncells; c++) {
@@ -926,10 +955,21 @@ int orc_calc_forces(orc_sys *s, int steps) {
         s->sdf[ia] = df;
         s->sepot[ia] += pot;
         s->tot_pot_energy += pot;
+        if (s->emod) {
+          /* EEAM: E(P) and EAM_DM = 2E'(P) (src/imd_forces_nbl.c:1090-1095) */
+          orc_pair_int2(s->emod, s->stype[ia], nt, s->sP[ia], &pot, &df,
+                        &idummy);
+          s->sdm[ia] = df;
+          s->sepot[ia] += pot;
+          s->tot_pot_energy += pot;
+        }
       }
     }
     /* copy_dF into buffer cells (creation order) */
-    for (int q = s->n; q < s->nslot; q++) s->sdf[q] = s->sdf[s->ssrc[q]];
+    for (int q = s->n; q < s->nslot; q++) {
+      s->sdf[q] = s->sdf[s->ssrc[q]];
+      s->sdm[q] = s->sdm[s->ssrc[q]];
+    }
     /* loop B: EAM forces */
     for (int n = 0; n < s->n; n++) {
       int ia = s->tl_atom[n];
@@ -944,11 +984,25 @@ int orc_calc_forces(orc_sys *s, int steps) {
         double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         int jt = s->stype[js], col1 = jt * nt + it, col2 = it * nt + jt;
         if (r2 < rt->end[col1] || r2 < rt->end[col2]) {
-          double rho_i_s = orc_deriv_func2(rt, col1, inc, r2, &is_short);
-          double rho_j_s = (col1 == col2)
-                               ? rho_i_s
-                               : orc_deriv_func2(rt, col2, inc, r2, &is_short);
+          double rho_i = 0.0, rho_j = 0.0, rho_i_s, rho_j_s;
+          if (s->emod) {
+            /* EEAM: values too (src/imd_forces_nbl.c:1191-1199) */
+            orc_pair_int2(rt, col1, inc, r2, &rho_i, &rho_i_s, &is_short);
+            if (col1 == col2) {
+              rho_j = rho_i;
+              rho_j_s = rho_i_s;
+            } else {
+              orc_pair_int2(rt, col2, inc, r2, &rho_j, &rho_j_s, &is_short);
+            }
+          } else {
+            rho_i_s = orc_deriv_func2(rt, col1, inc, r2, &is_short);
+            rho_j_s = (col1 == col2)
+                          ? rho_i_s
+                          : orc_deriv_func2(rt, col2, inc, r2, &is_short);
+          }
           double grad = 0.5 * (s->sdf[ia] * rho_j_s + s->sdf[js] * rho_i_s);
+          if (s->emod)
+            grad += (s->sdm[ia] * rho_j * rho_j_s + s->sdm[js] * rho_i * rho_i_s);
           double fc[3] = {d[0] * grad, d[1] * grad, d[2] * grad};
           s->sf[3 * js] -= fc[0];
           s->sf[3 * js + 1] -= fc[1];

@@ -120,6 +120,12 @@ int  orc_calc_forces(orc_sys *s, int steps);
    src/imd_integrate.c:411-431, 1085-1094); orc_get_press gives 6 per atom
    (xx yy zz yz zx xy, PRESSTENS), caller order */
 void orc_set_press(orc_sys *s, int on);
+/* EEAM (src/imd_forces_nbl.c:591-608, 1090-1095, 1180-1207): a second
+   density P_i = sum_j rho_ij^2 with the energy term E(P) of the emod
+   table (ntypes columns, not radial; src/imd_potential.c:82-85); set after
+   orc_set_tables, NULL turns it off.  orc_get_eeam gives P and 2E'(P). */
+int  orc_set_eeam(orc_sys *s, const orc_table *emod);
+void orc_get_eeam(const orc_sys *s, double *P, double *dm);
 void orc_get_press(const orc_sys *s, double *out);
 void orc_move_atoms(orc_sys *s);
 void orc_check_nblist(orc_sys *s);

@@ -23,6 +23,15 @@ def lj():
     return W.lj_table()
 
 
+def emod_table(nt, pmax=40.0, npts=4000):
+    """A smooth EEAM energy term E(P) per type (not radial) on [0, pmax]."""
+    step, px = W._grid(0.0, pmax, npts)
+    cols = [(-0.4 - 0.1 * t) * np.sqrt(px + 0.5) + (0.05 + 0.02 * t) * px
+            for t in range(nt)]
+    return W.Table(np.zeros(nt), np.full(nt, pmax), np.full(nt, step), cols,
+                   False)
+
+
 def oracle_tables(eam: "W.EamTables", interp=0):
     import oracle as O
     pair = O.OracleTable.from_table(eam.pair, interp=interp)

@@ -285,3 +285,31 @@ def test_stress_outputs_match_oracle(tmp_path):
     ref = orc.press()
     assert rows.shape == (s.n, 12)
     assert np.abs(rows[:, 6:] - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def test_eeam_run_matches_oracle(tmp_path):
+    """eeam_energy_file (EEAM): the .eng rows follow the oracle with the
+    same E(P) table, and checkpoints carry the eam_p column."""
+    from helpers import emod_table
+    import oracle as O
+    eam, s, files, cfg = _setup(tmp_path)
+    et = emod_table(1)
+    epath = str(tmp_path / "emod.pt")
+    et.write(epath)
+    _run(tmp_path, _lines(s, files, cfg, "run", "nve", 20,
+                          ["eeam_energy_file %s" % epath, "checkpt_int 20"]))
+    eng = _read_eng(str(tmp_path / "run.eng"))
+    orc = oracle_eam(s, eam, nbl_margin=0.4, timestep=W.FS)
+    emod = O.OracleTable.from_table(et)
+    orc.set_eeam(emod)
+    th = orc.run(21)
+    for row in eng:
+        step = int(round(row[0] / W.FS))
+        ep = th[step, 0] / s.n
+        assert abs(row[1] - ep) <= 1e-9 * abs(ep), (step, row[1], ep)
+    lines = open(str(tmp_path / "run.00001.chkpt")).read().splitlines()
+    assert lines[0] == "#F A 1 1 1 3 3 3"
+    assert lines[1].endswith("Epot eam_rho eam_p")
+    rows = _rows(str(tmp_path / "run.00001.chkpt"))
+    P, _ = orc.eeam()
+    assert np.allclose(rows[:, 11], P, rtol=1e-9)

@@ -39,7 +39,9 @@ def poly_tables(nt, A, B, C, D, npts=400, rho_max=None):
     return pair, rho, emb
 
 
-def analytic(x, types, box, A, B, C, D):
+def analytic(x, types, box, A, B, C, D, G=None, H=None):
+    """Closed forms of the polynomial EAM; with G, H also EEAM's term
+    E(P) = G P^2 + H P of P_i = sum_j rho_ij^2."""
     n = x.shape[0]
     rc2 = RC * RC
     L = np.diag(box)
@@ -61,8 +63,17 @@ def analytic(x, types, box, A, B, C, D):
     F = Ct * rho ** 2 + Dt * rho
     Fp = 2 * Ct * rho + Dt
     g = 2 * dphi + 2 * (Fp[:, None] * df_j + Fp[None, :] * df_i)
+    P = (f_j ** 2).sum(1)
+    Ep = np.zeros_like(P)
+    E = np.zeros_like(P)
+    if G is not None:
+        f_i = np.where(m, Bi * (rc2 - r2) ** 2, 0.0)       # density at j from i
+        Gt, Ht = np.asarray(G)[types], np.asarray(H)[types]
+        E = Gt * P ** 2 + Ht * P
+        Ep = 2 * Gt * P + Ht
+        g = g + 4 * (Ep[:, None] * f_j * df_j + Ep[None, :] * f_i * df_i)
     f = np.einsum("ij,ijk->ik", g, d)
-    epot_i = 0.5 * phi.sum(1) + F
+    epot_i = 0.5 * phi.sum(1) + F + E
     epot = epot_i.sum()
     virial = -0.5 * np.sum(r2 * g)
     # per-atom stress tensor: -1/2 sum_j g d_a d_b, xx yy zz yz zx xy
@@ -70,7 +81,7 @@ def analytic(x, types, box, A, B, C, D):
     press = np.stack([-0.5 * np.einsum("ij,ij,ij->i", g, d[:, :, a], d[:, :, b])
                       for a, b in ab], axis=1)
     return dict(f=f, rho=rho, df=2 * Fp, epot_i=epot_i, epot=epot,
-                virial=virial, press=press)
+                virial=virial, press=press, P=P, dm=2 * Ep)
 
 
 def run_oracle(s, tabs, nt, press=False):
@@ -258,3 +269,88 @@ def test_stress_kinetic_part(ensemble):
     assert np.allclose(orc.press() - pot, kin, rtol=1e-12, atol=1e-15)
     # the diagonal carries 2 Ekin of the new momenta
     assert abs(kin[:, :3].sum() - (p * p / m).sum()) < 1e-12 * kin[:, :3].sum()
+
+
+# ---------------------------------------------------------------------------
+# EEAM (second density P = sum rho^2 and its energy term E(P))
+# ---------------------------------------------------------------------------
+
+def emod_table(nt, G, H, p_max, npts=400):
+    fstep, px = W._grid(0.0, p_max, npts)
+    return W.Table(np.zeros(nt), np.full(nt, p_max), np.full(nt, fstep),
+                   [G[t] * px * px + H[t] * px for t in range(nt)], False)
+
+
+@pytest.mark.parametrize("nt", [1, 2])
+def test_eeam_closed_form(nt):
+    """EEAM on quadratic tables: P_i, 2E'(P_i), energies, forces (with the
+    DM_i rho_j rho'_j + DM_j rho_i rho'_i term of src/imd_forces_nbl.c:
+    1204-1207), virial and stress against the brute-force closed form."""
+    s = W.fcc_system((4, 4, 4), 3.25, [1.0] * nt, 0.0, 9, displace=0.15,
+                     type_fractions=None if nt == 1 else (0.6, 0.4))
+    A = [0.05, 0.08][:nt]
+    B = [0.02, 0.035][:nt]
+    C = [0.3, 0.2][:nt]
+    D = [-1.0, -0.7][:nt]
+    Gq = [0.01, -0.02][:nt]
+    Hq = [-0.05, 0.03][:nt]
+    ref = analytic(s.x, s.type, s.box, A, B, C, D, Gq, Hq)
+    tabs = poly_tables(nt, A, B, C, D, rho_max=1.3 * ref["rho"].max())
+    pair, rho, emb = [O.OracleTable.from_table(t) for t in tabs]
+    emod = O.OracleTable.from_table(emod_table(nt, Gq, Hq, 1.3 * ref["P"].max()))
+    orc = O.OracleSystem(s, pair, rho, emb, nbl_margin=0.3, timestep=0.001,
+                         ntypes=nt)
+    orc.set_eeam(emod)
+    orc.set_press(True)
+    orc.calc_forces()
+    a = orc.atoms()
+    P, dm = orc.eeam()
+    assert np.allclose(P, ref["P"], rtol=1e-11, atol=0)
+    assert np.allclose(dm, ref["dm"], rtol=1e-10, atol=1e-13)
+    scale = np.abs(ref["f"]).max()
+    assert np.max(np.abs(a["f"] - ref["f"])) < 1e-10 * scale
+    assert np.allclose(a["epot"], ref["epot_i"], rtol=1e-10, atol=1e-12)
+    assert abs(orc.scalar("epot") - ref["epot"]) < 1e-10 * abs(ref["epot"])
+    assert abs(orc.scalar("virial") - ref["virial"]) < 1e-9 * abs(ref["virial"])
+    ps = np.abs(ref["press"]).max()
+    assert np.max(np.abs(orc.press() - ref["press"])) < 1e-10 * ps
+    # the EEAM term is not negligible in this case
+    plain = analytic(s.x, s.type, s.box, A, B, C, D)
+    assert np.abs(plain["f"] - ref["f"]).max() > 1e-3 * scale
+
+
+def test_eeam_forces_are_energy_gradients():
+    """Johnson EAM plus a smooth E(P) table: forces are minus the finite-
+    difference gradient of the energy."""
+    eam = W.johnson_eam()
+    s = W.eam_crystal(n_cells=4, temp_k=300.0, seed=4, displace=0.08)
+    pmax = 40.0
+    fstep, px = W._grid(0.0, pmax, 4000)
+    et = W.Table(np.zeros(1), np.full(1, pmax), np.full(1, fstep),
+                 [-0.4 * np.sqrt(px + 0.5) + 0.05 * px], False)
+
+    def energy(sys_):
+        pair, rho, emb = [O.OracleTable.from_table(t)
+                          for t in (eam.pair, eam.rho, eam.embed)]
+        emod = O.OracleTable.from_table(et)
+        orc = O.OracleSystem(sys_, pair, rho, emb, nbl_margin=0.3,
+                             timestep=0.001, ntypes=1)
+        orc.set_eeam(emod)
+        orc.calc_forces()
+        return orc.scalar("epot"), orc
+
+    e0, orc = energy(s)
+    assert orc.eeam()[0].max() < pmax
+    f = orc.atoms()["f"]
+    h = 1e-5
+    rng = np.random.default_rng(5)
+    for atom in rng.choice(s.n, 5, replace=False):
+        for k in range(3):
+            sp = W.System(**{**s.__dict__})
+            sp.x = s.x.copy()
+            sp.x[atom, k] += h
+            sm = W.System(**{**s.__dict__})
+            sm.x = s.x.copy()
+            sm.x[atom, k] -= h
+            fd = -(energy(sp)[0] - energy(sm)[0]) / (2 * h)
+            assert abs(fd - f[atom, k]) < 2e-5 + 1e-4 * abs(f[atom, k])
