@@ -209,7 +209,9 @@ __device__ bool replica_step_end(StepArgs<R>& a, int r, const double* tot,
   } else {
     ekin = (tot[PB_E1] + tot[PB_E2]) / 4.0;
     double ttt = Rp.nactive * Rp.temperature;
-    st.eta += a.dt * (tot[PB_E2] / ttt - 1.0) * Rp.isq_tau_eta;
+    // a replica without atoms has no temperature to steer
+    if (Rp.nactive > 0)
+      st.eta += a.dt * (tot[PB_E2] / ttt - 1.0) * Rp.isq_tau_eta;
   }
   int need = tot[PB_MAXD] > Rp.skin2;
   st.need_rebuild = need;
