@@ -549,10 +549,14 @@ struct ListLane {
   int cx, cy, cz;  // the atom's cell (image shifts of the FP64 decision)
 };
 
-// Entry m of the lane's list: group m/4 of the lane, slot m%4.
+// Entry m of the lane's list: group m/4 of the lane, slot m%4 (a 32-bit
+// byte offset from the lane's first entry: groups of 64 lanes are 1 KiB
+// apart).  Past kmax the last slot is overwritten: such a list has
+// overflowed, and the build is repeated with a longer one.
 __device__ __forceinline__ void list_append(ListLane& L, uint32_t e) {
-  if (L.cnt < L.kmax)
-    L.lst[(size_t)(L.cnt >> 2) * (IMDG_WAVE * IMDG_GROUP) + (L.cnt & 3)] = e;
+  const unsigned m = (unsigned)min(L.cnt, L.kmax - 1);
+  const unsigned off = ((m >> 2) << 10) | ((m & 3u) << 2);
+  *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(L.lst) + off) = e;
   L.cnt++;
 }
 
@@ -569,14 +573,6 @@ typedef __attribute__((address_space(3))) const float3 lds_float3;
 typedef __attribute__((address_space(3))) const char lds_char;
 typedef __attribute__((address_space(3))) const float lds_float;
 
-// Record k of the tile (k < IMDG_TILE_CAP, so the byte offset is a 24-bit
-// product).
-__device__ __forceinline__ float3 tile_rec(lds_float3* t, int k) {
-  unsigned off = __umul24((unsigned)k, (unsigned)sizeof(float3));
-  lds_float* p = (lds_float*)((lds_char*)t + off);
-  return make_float3(p[0], p[1], p[2]);
-}
-
 // Candidates of one neighbour cell, four at a time (their records in
 // flight together): FP32 screen, FP64 decision inside the band (one branch
 // per four candidates; the band is rarely hit), append.  The separation is
@@ -588,15 +584,19 @@ __device__ __forceinline__ void list_cell(
     lds_float3* trec, ListLane& L, int jb, int jn, int tb, float o0,
     float o1, float o2, bool self, bool owner, uint32_t code) {
   const float w0 = o0 - L.ui.x, w1 = o1 - L.ui.y, w2 = o2 - L.ui.z;
+  // the cell's run in the tile: four records per pass at immediate offsets
+  // from one address (up to three records past the run: inside the tile
+  // array of IMDG_TILE_CAP + 4 records, and masked out below)
+  lds_char* run = (lds_char*)trec + __umul24((unsigned)tb, 12u);
   for (int t0 = 0; t0 < jn; t0 += 4) {
     float3 uj[4];
+    lds_float* q = (lds_float*)(run + __umul24((unsigned)t0, 12u));
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      int t = min(t0 + u, jn - 1);
       if (TOK) {
-        uj[u] = tile_rec(trec, tb + t);
+        uj[u] = make_float3(q[3 * u], q[3 * u + 1], q[3 * u + 2]);
       } else {
-        uj[u] = rec_xyz<MIXED>(a, jb + t);
+        uj[u] = rec_xyz<MIXED>(a, jb + min(t0 + u, jn - 1));
       }
     }
     bool in[4], band[4];
@@ -640,7 +640,7 @@ __device__ __forceinline__ void list_cell(
 // staged in LDS (12 B each); entries are written four at a time.
 template <bool MIXED>
 __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
-  __shared__ float3 trec[IMDG_TILE_CAP];
+  __shared__ float3 trec[IMDG_TILE_CAP + 4];
   __shared__ int4 truns[IMDG_MAXRUN];
   __shared__ float off[3 * IMDG_NCODES];
   __shared__ int tinfo[2];
