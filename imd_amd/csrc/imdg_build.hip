@@ -725,9 +725,23 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
         jn[k] = valid ? cn : 0;
       }
       if (tok) {
+        // tile positions: a search for the centre cell of each of the 9
+        // columns; the z neighbours follow it in slot order (cells are
+        // sorted z fastest) and so in the tile, whose runs are laid out in
+        // slot order without gaps over the block's neighbour cells, unless
+        // the column wraps through a periodic z face
 #pragma unroll
-        for (int k = 0; k < 27; k++)
-          tb[k] = jn[k] > 0 ? tile_index(truns, nrun, jb[k]) : 0;
+        for (int k = 1; k < 27; k += 3) tb[k] = tile_index(truns, nrun, jb[k]);
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+          const int oz = k % 3 - 1;
+          if (oz == 0) continue;
+          const int n2 = cz + oz;
+          if (n2 < 0 || n2 >= g2)
+            tb[k] = jn[k] > 0 ? tile_index(truns, nrun, jb[k]) : 0;
+          else
+            tb[k] = tb[k - oz] + (jb[k] - jb[k - oz]);
+        }
 #pragma unroll
         for (int k = 0; k < 27; k++)
           list_cell<MIXED, true>(a, R, pos, (lds_float3*)trec, L, jb[k], jn[k], tb[k],
