@@ -156,6 +156,8 @@ struct imdg_ctx {
   // profiling
   bool profiling = false;
   unsigned prof_mask = 0;  // kernel classes timed (bit k: class k)
+  long prof_stride = 1;    // time every prof_stride-th launch of a class
+  long kc_seen[KC_N] = {};  // launches of each class while profiling
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> event_pool;
   double kc_ms[KC_N] = {};
@@ -680,7 +682,9 @@ struct ProfScope {
   hipEvent_t a = nullptr, b = nullptr;
   bool on;
   ProfScope(imdg_ctx* c_, int kc_)
-      : c(c_), kc(kc_), on(c_->profiling && ((c_->prof_mask >> kc_) & 1)) {
+      : c(c_), kc(kc_),
+        on(c_->profiling && ((c_->prof_mask >> kc_) & 1) &&
+           (c_->kc_seen[kc_]++ % c_->prof_stride) == 0) {
     if (on) {
       a = get_event(c);
       b = get_event(c);
@@ -1868,9 +1872,15 @@ int imdg_profile(imdg_ctx* c, int enable) {
   collect_profile(c);
   c->profiling = enable != 0;
   c->prof_mask = enable == 1 ? ~0u : (unsigned)enable;
+  // IMDG_PROFILE_STRIDE=k: time every k-th launch of each class (an event
+  // pair makes the stream wait at both ends of the timed launch)
+  c->prof_stride = 1;
+  if (const char* e = getenv("IMDG_PROFILE_STRIDE"))
+    c->prof_stride = std::max(1, atoi(e));
   for (int k = 0; k < KC_N; k++) {
     c->kc_ms[k] = 0;
     c->kc_n[k] = 0;
+    c->kc_seen[k] = 0;
   }
   return IMDG_OK;
 }
