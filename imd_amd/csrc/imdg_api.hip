@@ -126,7 +126,10 @@ struct imdg_ctx {
   int knots_embed = 0;
   int knots_joint = -1;                // interleaved (pair, rho), one species
   int knots_dpoly = -1;                // FP32 rho segment polynomials
-  int force_mode = 0;                  // 0 one species, 1 shared grid, 2 several
+  int knots_jointn = -1;               // FP32, several species, one grid:
+  int knots_dpolyn = -1;               // force and rho segment records
+  int force_mode = 0;                  // 0 one species, 1 shared grid,
+                                       // 2 several, 3 several on one grid
   int lpa_den = 0, lpa_frc = 0;        // lanes per slot of the two loops
   int thr_den = 0, thr_frc = 0;        // threads per block (0: default)
   int units_cap = 0;                   // wavefront partial slots allocated
@@ -475,6 +478,67 @@ int upload_tables(imdg_ctx* c) {
       kn.push_back(b);
     }
   }
+  // FP32, several species, every pair and rho column on one grid: per type
+  // pair p = it*nt + jt and segment k a force record of 32 B, (phi p0, phi
+  // A, phi B, phi D) of pair column p and (A, D) of rho columns p (density
+  // at an it-atom due to a jt-atom) and jt*nt + it, and a density record
+  // (rho p0, A, B, 0) of rho column p; segment len-1 zero, as for one
+  // species (the table argument is clamped instead of the cutoff tests)
+  c->knots_jointn = c->knots_dpolyn = -1;
+  const int nt = c->cfg.ntypes;
+  bool grid_n = sizeof(R) == 4 && !c->cubic && !c->eeam && nt > 1 &&
+                hp.present && hr.present;
+  for (int col = 0; grid_n && col < nt * nt; col++)
+    grid_n = hp.begin[col] == hp.begin[0] && hp.end[col] == hp.end[0] &&
+             hp.invstep[col] == hp.invstep[0] && hp.len[col] == hp.len[0] &&
+             hr.begin[col] == hp.begin[0] && hr.end[col] == hp.end[0] &&
+             hr.invstep[col] == hp.invstep[0] && hr.len[col] == hp.len[0];
+  if (const char* e = getenv("IMDG_JOINT_TABLE"))
+    if (atoi(e) == 0) grid_n = false;
+  if (grid_n) {
+    const int len = hp.len[0];
+    // (p0, A, B, D) of segment k of column col, zero for k = len-1
+    auto seg = [&](const HostTable& h, int col, int k, double* q) {
+      q[0] = q[1] = q[2] = q[3] = 0.0;
+      if (k >= len - 1) return;
+      const size_t nc = h.ncols;
+      double p0 = h.table[k * nc + col], p1 = h.table[(k + 1) * nc + col];
+      double p2 = h.table[(k + 2) * nc + col];
+      double dv = p1 - p0, d2v = p2 - 2 * p1 + p0;
+      q[0] = p0;
+      q[1] = dv - 0.5 * d2v;
+      q[2] = 0.5 * d2v;
+      q[3] = d2v;
+    };
+    auto push4 = [&](double a, double b, double cc, double d) {
+      R2 x, y;
+      x.x = (R)a;
+      x.y = (R)b;
+      y.x = (R)cc;
+      y.y = (R)d;
+      kn.push_back(x);
+      kn.push_back(y);
+    };
+    if (kn.size() & 1) kn.push_back(kn.back());
+    c->knots_jointn = (int)kn.size();
+    for (int it = 0; it < nt; it++)
+      for (int jt = 0; jt < nt; jt++)
+        for (int k = 0; k < len; k++) {
+          double ph[4], rc[4], rt[4];
+          seg(hp, it * nt + jt, k, ph);
+          seg(hr, it * nt + jt, k, rc);
+          seg(hr, jt * nt + it, k, rt);
+          push4(ph[0], ph[1], ph[2], ph[3]);
+          push4(rc[1], rc[3], rt[1], rt[3]);
+        }
+    c->knots_dpolyn = (int)kn.size();
+    for (int p = 0; p < nt * nt; p++)
+      for (int k = 0; k < len; k++) {
+        double rc[4];
+        seg(hr, p, k, rc);
+        push4(rc[0], rc[1], rc[2], 0.0);
+      }
+  }
   int rc = dalloc(c, &c->d_knots, kn.size() * sizeof(R2));
   if (rc) return rc;
   HIPCHK(c, hipMemcpy(c->d_knots, kn.data(), kn.size() * sizeof(R2),
@@ -499,7 +563,8 @@ int plan_steps(imdg_ctx* c) {
   bool nt1 = c->cfg.ntypes == 1;
   bool hp = c->tab[0].present, he = c->tab[1].present;
   const bool cub = c->cubic;
-  c->force_mode = !nt1 ? 2 : (c->knots_joint >= 0 ? 1 : 0);
+  c->force_mode = !nt1 ? (c->knots_jointn >= 0 ? 3 : 2)
+                       : (c->knots_joint >= 0 ? 1 : 0);
   int mode = c->force_mode;
   // default shapes (what imdg_autotune picks on the 2M-atom EAM crystal):
   // four lanes per atom and full blocks, except the FP64 force loop, whose
@@ -514,7 +579,8 @@ int plan_steps(imdg_ctx* c) {
   if (const char* e = getenv("IMDG_TABLES_IN_LDS")) no_lds = (atoi(e) == 0);
   // cubic segments, EEAM: the generic loops with the tables in HBM/L2
   if (cub || c->eeam) no_lds = 1;
-  // density: the rho knots
+  // density: the rho knots (the several-species segment records stay in
+  // HBM/L2: nt^2 columns)
   c->lds_den_lo = c->knots_rho;
   c->lds_den_n = he ? c->knots_embed - c->knots_rho : 0;
   if (c->knots_dpoly >= 0) {
@@ -522,7 +588,7 @@ int plan_steps(imdg_ctx* c) {
     c->lds_den_n = 2 * c->tab[1].len[0];
   }
   c->grid_den = 0;
-  if (he && !no_lds)
+  if (he && !no_lds && c->knots_dpolyn < 0)
     c->grid_den = step_grid_density<R>(nt1, true, c->lds_den_n * sizeof(R2),
                                        c->ncu, c->lpa_den, td, cub, c->eeam);
   if (he && c->grid_den == 0) {
@@ -542,7 +608,7 @@ int plan_steps(imdg_ctx* c) {
   }
   for (int integ = 0; integ < 2; integ++) {
     int g = 0;
-    if (!no_lds)
+    if (!no_lds && mode != 3)
       g = step_grid_force<R>(mode, integ, true, hp, he,
                              c->lds_frc_n * sizeof(R2), c->ncu, c->lpa_frc,
                              tf, cub, c->eeam);
@@ -595,6 +661,8 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.knots = (const typename V2<R>::T*)c->d_knots;
   a.joint = c->knots_joint;
   a.dpoly = c->knots_dpoly;
+  a.jointn = c->knots_jointn;
+  a.dpolyn = c->knots_dpolyn;
   a.cubic = c->cubic ? 1 : 0;
   a.eeam = c->eeam ? 1 : 0;
   a.eeP = (R*)c->d_eeP;

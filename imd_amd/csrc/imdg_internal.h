@@ -210,6 +210,8 @@ struct StepArgs {
                           // (pair, rho) table, -1 if there is none
   int dpoly;              // first knot of the FP32 rho segment polynomials
                           // (one species), -1 if there are none
+  int jointn, dpolyn;     // first knot of the FP32 several-species
+                          // segment records (one grid), -1 if none
   int cubic;              // tables hold cubic segments, two knots each
                           // (4-point or spline interpolation)
   int eeam;               // EEAM: second density P = sum rho^2, energy E(P)

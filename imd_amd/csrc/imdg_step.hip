@@ -39,7 +39,9 @@ enum {
   FM_NT1 = 0,  // one species
   FM_SG = 1,   // one species, pair and rho tables on the same grid: one
                // segment lookup and one interleaved knot table
-  FM_NTN = 2   // several species
+  FM_NTN = 2,  // several species
+  FM_SGN = 3   // several species, every pair and rho column on one grid
+               // (FP32): one segment lookup, one 32 B record per type pair
 };
 
 // ---------------------------------------------------------------------------
@@ -974,6 +976,102 @@ __device__ __forceinline__ void force_group_sgf(
   }
 }
 
+// Several species on one grid (FP32): the record pair S[2 (p len + k)] of
+// type pair p = it*nt + jt is (phi p0, phi A, phi B, phi D) of pair column p
+// and (A, D) of rho columns p and jt*nt + it; gradients in units of
+// 1/(2 istep) as in force_group_sgf.  rho'_j (column p) goes with 2F'_i,
+// rho'_i (column jt*nt + it) with 2F'_j (src/imd_forces_nbl.c:1156-1203).
+template <typename PF>
+__device__ __forceinline__ void force_group_sgn(
+    uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
+    const signed char* __restrict__ atype, const float4* off,
+    const float4& ui, float hFi, const float4* __restrict__ S, int pbase,
+    int len, float tc0, float istep, float tz, float& sx, float& sy,
+    float& sz, float& ep, float& svir, float& r2min) {
+  uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
+  float4 uj[4], o[4];
+  int jt[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    uj[u] = rec[en[u] & IMDG_IDX_MASK];
+    jt[u] = atype[en[u] & IMDG_IDX_MASK];
+    o[u] = off[en[u] >> IMDG_IDX_BITS];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  slot = pf.load();
+  float dx[4], dy[4], dz[4], r2[4], chi[4];
+  int kk[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    dx[u] = (uj[u].x - ui.x) + o[u].x;
+    dy[u] = (uj[u].y - ui.y) + o[u].y;
+    dz[u] = (uj[u].z - ui.z) + o[u].z;
+    r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+    float t = __builtin_amdgcn_fmed3f(fmaf(r2[u], istep, tc0), 0.0f, tz);
+    kk[u] = 2 * ((pbase + jt[u]) * len + (int)t);
+    chi[u] = __builtin_amdgcn_fractf(t);
+  }
+  float4 q[4], rr[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    q[u] = S[kk[u]];
+    rr[u] = S[kk[u] + 1];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    float gp = fmaf(chi[u], q[u].w, q[u].y);
+    float grj = fmaf(chi[u], rr[u].y, rr[u].x);
+    float gri = fmaf(chi[u], rr[u].w, rr[u].z);
+    float vp = fmaf(chi[u], fmaf(chi[u], q[u].z, q[u].y), q[u].x);
+    float g = fmaf(hFi, grj, fmaf(0.5f * uj[u].w, gri, gp));
+    ep += vp;
+    sx = fmaf(dx[u], g, sx);
+    sy = fmaf(dy[u], g, sy);
+    sz = fmaf(dz[u], g, sz);
+    svir = fmaf(r2[u], g, svir);
+    r2min = fminf(r2min, r2[u]);
+  }
+}
+
+// Several species on one grid (FP32): density records DQ[p len + k] = (rho
+// p0, A, B, 0) of column p = it*nt + jt, jt from the gathered record.
+template <typename PF>
+__device__ __forceinline__ void density_group_polyn(
+    uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
+    const float4* off, const float4& ui, const float4* __restrict__ DQ,
+    int pbase, int len, float tc0, float istep, float tz, float& rho,
+    float& r2min) {
+  uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
+  float4 uj[4], o[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    uj[u] = rec[en[u] & IMDG_IDX_MASK];
+    o[u] = off[en[u] >> IMDG_IDX_BITS];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  slot = pf.load();
+  float r2[4], chi[4];
+  int kk[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    float dx = (uj[u].x - ui.x) + o[u].x;
+    float dy = (uj[u].y - ui.y) + o[u].y;
+    float dz = (uj[u].z - ui.z) + o[u].z;
+    r2[u] = dx * dx + dy * dy + dz * dz;
+    float t = __builtin_amdgcn_fmed3f(fmaf(r2[u], istep, tc0), 0.0f, tz);
+    kk[u] = (pbase + (int)uj[u].w) * len + (int)t;
+    chi[u] = __builtin_amdgcn_fractf(t);
+  }
+  float4 q[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) q[u] = DQ[kk[u]];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    rho += fmaf(chi[u], fmaf(chi[u], q[u].z, q[u].y), q[u].x);
+    r2min = fminf(r2min, r2[u]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // density loop
 // ---------------------------------------------------------------------------
@@ -1039,12 +1137,33 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
                                             tc0, is0, tz, rho, r2min);
                        });
         } else {
-          list_walk<L>(nw, a.nnb[i],
-                       [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                         density_group<R, NT1, CUB, EE>(
-                             e4, pf, slot, rec, off, ui, it, nt, K, cpr, col0,
-                             rho, pee, is_short, r2min);
-                       });
+          auto generic = [&]() {
+            list_walk<L>(nw, a.nnb[i],
+                         [&](uint4 e4, ListRef<L> pf, uint4& slot) {
+                           density_group<R, NT1, CUB, EE>(
+                               e4, pf, slot, rec, off, ui, it, nt, K, cpr,
+                               col0, rho, pee, is_short, r2min);
+                         });
+          };
+          if constexpr (sizeof(R) == 4 && !NT1 && !CUB && !EE && !LT) {
+            if (a.dpolyn >= 0) {
+              // several species on one grid: segment records in HBM/L2
+              const float4* __restrict__ DQ =
+                  reinterpret_cast<const float4*>(a.knots + a.dpolyn);
+              const float tc0 = -b0 * is0, tz = (float)len0 - 0.5f;
+              list_walk<L>(nw, a.nnb[i],
+                           [&](uint4 e4, ListRef<L> pf, uint4& slot) {
+                             density_group_polyn(e4, pf, slot, rec, off, ui,
+                                                 DQ, it * nt, len0, tc0, is0,
+                                                 tz, rho, r2min);
+                           });
+              is_short |= (r2min < b0);
+            } else {
+              generic();
+            }
+          } else {
+            generic();
+          }
         }
         if (NT1) is_short |= (r2min < b0);
       }
@@ -1124,7 +1243,7 @@ template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT,
           int L, bool CUB, bool EE>
 __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
     k_force(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
-  constexpr bool NT1 = MODE != FM_NTN;
+  constexpr bool NT1 = MODE != FM_NTN && MODE != FM_SGN;
   typedef typename V4<R>::T R4;
   typedef typename V2<R>::T R2;
   extern __shared__ __align__(16) unsigned char dyn_lds[];
@@ -1178,7 +1297,20 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
         int it = NT1 ? 0 : (int)atype[i];
         R dFi = HE ? ui.w : (R)0;
         const uint32_t* nw = list_words<L>(a.nbr, i, a.kmax, gs);
-        if constexpr (MODE == FM_SG && !CUB && !EE) {
+        if constexpr (MODE == FM_SGN && sizeof(R) == 4 && !CUB && !EE) {
+          // several species on one grid: sums in units of 1/(2 istep)
+          R r2min = (R)3.0e38;
+          const float4* __restrict__ S =
+              reinterpret_cast<const float4*>(a.knots + a.jointn);
+          const float tc0 = -pb * pis, tz = (float)plen - 0.5f;
+          list_walk<L>(nw, a.nnb[i],
+                       [&](uint4 e4, ListRef<L> pf, uint4& slot) {
+                         force_group_sgn(e4, pf, slot, rec, atype, off, ui,
+                                         0.5f * dFi, S, it * nt, plen, tc0,
+                                         pis, tz, fx, fy, fz, ep, vir, r2min);
+                       });
+          is_short |= (r2min < pb);
+        } else if constexpr (MODE == FM_SG && !CUB && !EE) {
           // one grid: sums in units of 1/(2 istep), scaled once per slot
           R r2min = (R)3.0e38;
           if constexpr (sizeof(R) == 4) {
@@ -1233,7 +1365,7 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
         kvir = vir;
       }
     }
-    if (MODE == FM_SG && !CUB && !EE) {
+    if ((MODE == FM_SG || (MODE == FM_SGN && sizeof(R) == 4)) && !CUB && !EE) {
       kx *= p2is;
       ky *= p2is;
       kz *= p2is;
@@ -1472,7 +1604,7 @@ const void* force_fn3(int lpa) {
 
 template <typename R, int MODE, bool INTEGRATE, bool LT, bool CUB, bool EE>
 const void* force_fn2(bool hp, bool he, int lpa) {
-  if (MODE == FM_SG)
+  if (MODE == FM_SG || MODE == FM_SGN)
     return force_fn3<R, MODE, INTEGRATE, true, true, LT, CUB, EE>(lpa);
   if (hp && he)
     return force_fn3<R, MODE, INTEGRATE, true, true, LT, CUB, EE>(lpa);
@@ -1520,6 +1652,19 @@ const void* force_fn(int mode, bool integrate, bool lt, bool hp, bool he,
   }
   if (mode == FM_SG)
     return force_fn1<R, FM_SG>(integrate, lt, true, true, lpa);
+  if (mode == FM_SGN) {
+    // FP32 only, tables in HBM/L2
+    if constexpr (sizeof(R) != 4) {
+      return nullptr;
+    } else {
+      if (lt) return nullptr;
+      return integrate
+                 ? force_fn2<R, FM_SGN, true, false, false, false>(true, true,
+                                                                   lpa)
+                 : force_fn2<R, FM_SGN, false, false, false, false>(true, true,
+                                                                    lpa);
+    }
+  }
   if (mode == FM_NT1) return force_fn1<R, FM_NT1>(integrate, lt, hp, he, lpa);
   return force_fn1<R, FM_NTN>(integrate, lt, hp, he, lpa);
 }
@@ -1592,6 +1737,7 @@ int step_grid_density(bool nt1, bool lt, size_t lds, int ncu, int lpa,
 template <typename R>
 int force_threads(int mode) {
   return mode == FM_SG    ? ForceThreads<R, FM_SG>::value
+         : mode == FM_SGN ? ForceThreads<R, FM_SGN>::value
          : mode == FM_NT1 ? ForceThreads<R, FM_NT1>::value
                           : ForceThreads<R, FM_NTN>::value;
 }
