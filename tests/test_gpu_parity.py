@@ -205,3 +205,57 @@ def test_list_overflow_grows_and_matches():
         big.run(10)
         a_big = big.atoms()
     assert max_abs(a_small["x"], a_big["x"]) == 0.0
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+def test_alloy_mixed_shared_grid(lanes):
+    """Two species whose pair and density columns share one grid, mixed
+    precision: the force loop reads one record per type pair and segment
+    and the density loop one (the several-species shared-grid path).  Same
+    bars as the one-species mixed path; the generic several-species loop
+    (IMDG_JOINT_TABLE=0) agrees with it to the FP32 rounding."""
+    import os
+    eam = eam_tables(nspecies=2, rc=5.5)
+    s = W.fcc_system((6, 6, 4), 3.615, eam.masses, W.kT(500.0), 37,
+                     type_fractions=(0.6, 0.4), pbc=(1, 1, 0), vacuum_z=20.0,
+                     displace=0.04)
+    orc = oracle_eam(s, eam, nbl_margin=0.4, timestep=W.FS)
+    orc.calc_forces()
+    res = {}
+    for joint in ("1", "0"):
+        old = os.environ.get("IMDG_JOINT_TABLE")
+        os.environ["IMDG_JOINT_TABLE"] = joint
+        try:
+            gpu = G.eam_context(eam, [s], precision=G.MIXED)
+        finally:
+            if old is None:
+                del os.environ["IMDG_JOINT_TABLE"]
+            else:
+                os.environ["IMDG_JOINT_TABLE"] = old
+        with gpu:
+            gpu.set_tuning(lanes, 512, lanes, 512)
+            gpu.calc_forces()
+            _compare_static(gpu, orc, s.n, 1e-5, 1e-6)
+            res[joint] = gpu.atoms()
+    assert rel_rms(res["1"]["f"], res["0"]["f"]) <= 2e-6
+    assert rel_rms(res["1"]["rho"], res["0"]["rho"]) <= 1e-6
+
+
+def test_alloy_mixed_trajectory():
+    """20 fused NVT steps of the two-species slab in mixed precision follow
+    the FP64 oracle (energies to 1e-6, momenta to the mixed force error)."""
+    eam = eam_tables(nspecies=2, rc=5.5)
+    s = W.fcc_system((6, 6, 4), 3.615, eam.masses, W.kT(500.0), 38,
+                     type_fractions=(0.7, 0.3), pbc=(1, 1, 0), vacuum_z=20.0,
+                     displace=0.02)
+    T, isq = W.kT(500.0), 1.0 / (50 * W.FS) ** 2
+    orc = oracle_eam(s, eam, nbl_margin=0.4, timestep=W.FS, ensemble=1,
+                     temperature=T, eta=0.0, isq_tau_eta=isq)
+    th = orc.run(20)
+    with G.eam_context(eam, [s], precision=G.MIXED, ensemble=G.NVT) as gpu:
+        gpu.set_nvt(0, T, 0.0, isq)
+        gpu.run(20)
+        hist = gpu.thermo_history(0, 20)[:, 0, :]
+        p = gpu.atoms()["p"]
+    assert np.allclose(hist[:, 1], th[:, 0], rtol=1e-6, atol=0)
+    assert rel_rms(p, orc.atoms()["p"]) <= 1e-5
