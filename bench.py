@@ -274,7 +274,7 @@ def config_run(args):
         raise SystemExit("unknown config %s" % name)
     natoms = sum(x.n for x in systems)
     # the batched replicas replay one captured step per step (hipGraph)
-    graph = name == "c3"
+    graph = name == "c3" if args.graph == "auto" else args.graph == "on"
     res = {}
     for label, prec in runs:
         old = os.environ.get("IMDG_GRAPH")
@@ -349,6 +349,9 @@ def main(argv=None):
     ap.add_argument("--config", choices=sorted(CONFIGS), default=None,
                     help="measure one of the other configurations instead "
                          "of the 2M-atom metric workload")
+    ap.add_argument("--graph", choices=["auto", "on", "off"], default="auto",
+                    help="--config runs: replay one captured step per step "
+                         "(auto: for the batched replicas only)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of timing "
                          "every lanes-per-atom x block-size shape first")
