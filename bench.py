@@ -273,10 +273,14 @@ def config_run(args):
     else:
         raise SystemExit("unknown config %s" % name)
     natoms = sum(x.n for x in systems)
-    # the batched replicas replay one captured step per step (hipGraph)
-    graph = name == "c3" if args.graph == "auto" else args.graph == "on"
+    # the batched replicas replay one captured step per step (hipGraph), and
+    # are also measured with plain stream launches (faster on MI355X)
+    graph_default = name == "c3" if args.graph == "auto" else args.graph == "on"
+    runs = [(label, prec, graph_default) for label, prec in runs]
+    if name == "c3" and args.graph == "auto":
+        runs += [(label + "_stream", prec, False) for label, prec, _ in runs]
     res = {}
-    for label, prec in runs:
+    for label, prec, graph in runs:
         old = os.environ.get("IMDG_GRAPH")
         os.environ["IMDG_GRAPH"] = "1" if graph else "0"
         try:
