@@ -92,7 +92,8 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libimdg.so is not built (%s); run "
-                               "__graft_entry__.build()" % LIB_PATH)
+                               "python -c 'from imd_amd import build; "
+                               "build.build()'" % LIB_PATH)
         L = C.CDLL(LIB_PATH)
         vp, dp, ip, lp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
         L.imdg_ctx_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]

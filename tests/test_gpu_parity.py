@@ -132,7 +132,7 @@ def test_eam_trajectory_fp64_nvt_unfused():
 
 
 def test_mixed_force_error_vs_fp64():
-    """Mixed vs FP64 GPU on the same configuration: the north-star bar of
+    """Mixed vs FP64 GPU on the same configuration: the accuracy target of
     <= 1e-5 relative RMS force error."""
     eam = eam_tables()
     s = _eam_system(n_cells=8, seed=21, temp_k=800.0, displace=0.08)
