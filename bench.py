@@ -57,10 +57,25 @@ def make_workload(cells: int, seed: int, temp_k: float):
     return eam, s
 
 
-def cpu_baseline(eam, s, steps: int, tau_fs: float, temp_k: float):
-    """The oracle (single-threaded C restatement of IMD's NBL/EAM path) on
+def cpu_threads() -> int:
+    """Host cores for the CPU baseline: the process's CPU share, at most 16
+    (one GPU's share of a host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(eam, s, steps: int, tau_fs: float, temp_k: float,
+                 threads: int):
+    """The oracle (C restatement of IMD's NBL/EAM path, FP64, half list) on
     the same workload, timed over a bounded number of steps after its first
-    (list-building) step."""
+    (list-building) step, on `threads` host cores (OpenMP by atom, per-thread
+    scatter accumulators; 1 = IMD's serial loop)."""
     import oracle as O
     pair = O.OracleTable.from_table(eam.pair)
     rho = O.OracleTable.from_table(eam.rho)
@@ -68,15 +83,16 @@ def cpu_baseline(eam, s, steps: int, tau_fs: float, temp_k: float):
     orc = O.OracleSystem(s, pair, rho, emb, nbl_margin=0.4, timestep=W.FS,
                          ensemble=O.ORC_NVT, temperature=W.kT(temp_k),
                          eta=0.0, isq_tau_eta=1.0 / (tau_fs * W.FS) ** 2)
+    orc.set_threads(threads)
     orc.run(1)
     t0 = time.perf_counter()
     orc.run(steps)
     dt = time.perf_counter() - t0
-    return dict(value=s.n * steps / dt, unit="atom-steps/s", cores=1,
+    return dict(value=s.n * steps / dt, unit="atom-steps/s", cores=threads,
                 kind="port",
                 sample="%d NVT steps of the same %d-atom workload after one "
-                       "warm-up step, oracle (FP64 C, half list), 1 thread"
-                       % (steps, s.n))
+                       "warm-up step, oracle (FP64 C, half list), %d "
+                       "thread(s)" % (steps, s.n, threads))
 
 
 def copy_probe(mib: int) -> int:
@@ -339,7 +355,10 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1002)
     ap.add_argument("--temp", type=float, default=800.0)
     ap.add_argument("--tau-fs", type=float, default=100.0)
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline (0: the "
+                         "process's CPU share, at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
@@ -463,7 +482,8 @@ def main(argv=None):
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(eam, s, args.cpu_steps, args.tau_fs, args.temp)
+        cpu = cpu_baseline(eam, s, args.cpu_steps, args.tau_fs, args.temp,
+                           args.cpu_threads or cpu_threads())
 
     compare = None
     if not args.no_compare:

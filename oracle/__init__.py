@@ -74,6 +74,7 @@ def lib():
         L.orc_get_grid.argtypes = [C.c_void_p, ip]
         L.orc_get_cells.argtypes = [C.c_void_p, ip]
         L.orc_set_press.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_threads.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_eeam.argtypes = [C.c_void_p, C.POINTER(_Table)]
         L.orc_get_eeam.argtypes = [C.c_void_p, dp, dp]
         L.orc_get_press.argtypes = [C.c_void_p, dp]
@@ -244,6 +245,10 @@ class OracleSystem:
 
     def set_press(self, on=True):
         lib().orc_set_press(self._s, int(on))
+
+    def set_threads(self, n):
+        """Threads for timing runs (1 = IMD's serial order, the checker)."""
+        lib().orc_set_threads(self._s, int(n))
 
     def press(self):
         """Per-atom stress tensor (n, 6): xx yy zz yz zx xy."""

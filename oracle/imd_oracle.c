@@ -9,6 +9,7 @@
 #include "imd_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -421,6 +422,11 @@ struct orc_sys {
   /* sum of |terms| of the virial: the scale of its rounding error (a
      tolerance for reduced-precision results, not part of IMD) */
   double virial_abs;
+  /* threads of the timing variant (orc_set_threads); 1 = IMD's serial
+     order.  tacc: per-thread copies of the scattered accumulators. */
+  int nthreads, tused;
+  double *tacc;
+  long tacc_cap;
 };
 
 static void vec_prod(const double *u, const double *v, double *w) {
@@ -439,6 +445,7 @@ orc_sys *orc_new(int ntypes, const double *box9, const int *pbc3,
   orc_sys *s = calloc(1, sizeof(orc_sys));
   if (!s) return NULL;
   s->ntypes = ntypes;
+  s->nthreads = 1;
   for (int d = 0; d < 3; d++) {
     for (int e = 0; e < 3; e++) s->box[d][e] = box9[3 * d + e];
     s->pbc[d] = pbc3[d];
@@ -537,6 +544,7 @@ void orc_free(orc_sys *s) {
   free(s->tl);
   free(s->tb);
   free(s->tl_atom);
+  free(s->tacc);
   free(s);
 }
 
@@ -767,6 +775,78 @@ static void refresh_buffers(orc_sys *s) {
 }
 
 /* make_nblist (src/imd_forces_nbl.c:136-273) */
+/* make_nblist on several threads (orc_set_threads): a counting pass and a
+   filling pass over the cells, each split over threads, with the same
+   per-atom entry order as the serial build, so the list is identical. */
+static int nblist_cell(const orc_sys *s, int c, int i, int *out) {
+  int c1 = s->inner[c], k = 0;
+  const ivec *p = &s->cells[c1];
+  int ia = p->a[i];
+  double d1[3] = {s->sx[3 * ia], s->sx[3 * ia + 1], s->sx[3 * ia + 2]};
+  for (int m = 0; m < 14; m++) {
+    int c2 = s->nq[c][m];
+    if (c2 < 0) continue;
+    int jstart = (c2 == c1) ? i + 1 : 0;
+    const ivec *q = &s->cells[c2];
+    for (int j = jstart; j < q->n; j++) {
+      int js = q->a[j];
+      double d[3] = {s->sx[3 * js] - d1[0], s->sx[3 * js + 1] - d1[1],
+                     s->sx[3 * js + 2] - d1[2]};
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (r2 < s->cellsz) {
+        if (out) out[k] = js;
+        k++;
+      }
+    }
+  }
+  return k;
+}
+
+static int make_nblist_threaded(orc_sys *s) {
+  int T = s->nthreads, nc = s->ncells;
+  int *first = malloc(sizeof(int) * (nc + 1));
+  if (!first) return ORC_ERR_NOMEM;
+  first[0] = 0;
+  for (int c = 0; c < nc; c++) first[c + 1] = first[c] + s->cells[s->inner[c]].n;
+#pragma omp parallel for num_threads(T) schedule(dynamic, 16)
+  for (int c = 0; c < nc; c++) {
+    const ivec *p = &s->cells[s->inner[c]];
+    for (int i = 0; i < p->n; i++) {
+      s->tl_atom[first[c] + i] = p->a[i];
+      s->tl[first[c] + i + 1] = nblist_cell(s, c, i, NULL);
+    }
+  }
+  long tn = 0;
+  for (int n = 0; n < s->n; n++) {
+    tn += s->tl[n + 1];
+    if (tn > 2147483647L) {
+      free(first);
+      return ORC_ERR_NOMEM;
+    }
+    s->tl[n + 1] = (int)tn;
+  }
+  if (tn > s->tbcap) {
+    int *t = realloc(s->tb, sizeof(int) * (tn > 0 ? tn : 1));
+    if (!t) {
+      free(first);
+      return ORC_ERR_NOMEM;
+    }
+    s->tb = t;
+    s->tbcap = tn;
+  }
+#pragma omp parallel for num_threads(T) schedule(dynamic, 16)
+  for (int c = 0; c < nc; c++) {
+    const ivec *p = &s->cells[s->inner[c]];
+    for (int i = 0; i < p->n; i++)
+      nblist_cell(s, c, i, s->tb + s->tl[first[c] + i]);
+  }
+  free(first);
+  s->tbn = tn;
+  s->have_valid_nbl = 1;
+  s->nbl_count++;
+  return ORC_OK;
+}
+
 static int make_nblist(orc_sys *s) {
   for (int i = 0; i < 3 * s->n; i++) s->x_nbl[i] = s->x[i];
   free(s->tl);
@@ -774,6 +854,8 @@ static int make_nblist(orc_sys *s) {
   s->tl = malloc(sizeof(int) * (s->n + 1));
   s->tl_atom = malloc(sizeof(int) * (s->n + 1));
   if (!s->tl || !s->tl_atom) return ORC_ERR_NOMEM;
+  s->tl[0] = 0;
+  if (s->nthreads > 1) return make_nblist_threaded(s);
   long tn = 0;
   int n = 0;
   s->tl[0] = 0;
@@ -856,6 +938,204 @@ void orc_get_press(const orc_sys *s, double *out) {
       out[6 * i + k] = s->spress ? s->spress[6 * i + k] : 0.0;
 }
 
+/* Timing variant of the force loops on several cores (orc_set_threads):
+   the same half list and the same pair, density, embedding and EAM terms
+   as orc_calc_forces, with loops A and B split over threads by atom.  The
+   scatter to j goes into per-thread copies of the accumulators, which are
+   summed after each loop, so the sums are taken in another order than
+   IMD's serial loop and differ from it in the last bits.  Used for pair
+   and EAM runs without stress or EEAM. */
+void orc_set_threads(orc_sys *s, int n) { s->nthreads = n > 1 ? n : 1; }
+
+static int tacc_reserve(orc_sys *s, long per_thread) {
+  long need = (long)s->nthreads * per_thread;
+  if (need > s->tacc_cap) {
+    double *t = realloc(s->tacc, (size_t)need * sizeof(double));
+    if (!t) return ORC_ERR_NOMEM;
+    s->tacc = t;
+    s->tacc_cap = need;
+  }
+  return ORC_OK;
+}
+
+static int forces_threaded(orc_sys *s) {
+  int nt = s->ntypes, inc = nt * nt, ns = s->nslot, T = s->nthreads;
+  const orc_table *pp = s->pair, *rt = s->rho_t, *et = s->embed;
+  long blk = 5L * ns;
+  int rc;
+  if ((rc = tacc_reserve(s, blk))) return rc;
+  double epot = 0.0, vir = 0.0, vabs = 0.0;
+  int shrt = 0;
+  /* loop A: pair forces and host electron density */
+#pragma omp parallel num_threads(T) reduction(+ : epot, vir, vabs) \
+    reduction(| : shrt)
+  {
+    int t = omp_get_thread_num();
+    if (t == 0) s->tused = omp_get_num_threads();
+    double *af = s->tacc + t * blk, *ae = af + 3L * ns, *ar = ae + ns;
+    memset(af, 0, (size_t)blk * sizeof(double));
+#pragma omp for schedule(static)
+    for (int n = 0; n < s->n; n++) {
+      int ia = s->tl_atom[n];
+      const double *d1 = s->sx + 3 * ia;
+      double ff[3] = {0, 0, 0}, ee = 0.0, eam_r = 0.0;
+      int it = s->stype[ia];
+      for (int m = s->tl[n]; m < s->tl[n + 1]; m++) {
+        int js = s->tb[m];
+        double d[3] = {s->sx[3 * js] - d1[0], s->sx[3 * js + 1] - d1[1],
+                       s->sx[3 * js + 2] - d1[2]};
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        int jt = s->stype[js], col = it * nt + jt, col2 = jt * nt + it;
+        if (pp && r2 <= pp->end[col]) {
+          double pot, grad;
+          orc_pair_int2(pp, col, inc, r2, &pot, &grad, &shrt);
+          epot += pot;
+          double fc[3] = {d[0] * grad, d[1] * grad, d[2] * grad};
+          af[3 * js] -= fc[0];
+          af[3 * js + 1] -= fc[1];
+          af[3 * js + 2] -= fc[2];
+          ff[0] += fc[0];
+          ff[1] += fc[1];
+          ff[2] += fc[2];
+          pot *= 0.5;
+          ee += pot;
+          ae[js] += pot;
+          vir -= r2 * grad;
+          vabs += fabs(r2 * grad);
+        }
+        if (rt) {
+          double rho_h = 0.0;
+          if (r2 < rt->end[col]) {
+            rho_h = orc_val_func2(rt, col, inc, r2, &shrt);
+            eam_r += rho_h;
+          }
+          if (it == jt) {
+            if (r2 < rt->end[col]) ar[js] += rho_h;
+          } else if (r2 < rt->end[col2]) {
+            ar[js] += orc_val_func2(rt, col2, inc, r2, &shrt);
+          }
+        }
+      }
+      af[3 * ia] += ff[0];
+      af[3 * ia + 1] += ff[1];
+      af[3 * ia + 2] += ff[2];
+      ae[ia] += ee;
+      ar[ia] += eam_r;
+    }
+  }
+  int used = s->tused;
+#pragma omp parallel for num_threads(T) schedule(static)
+  for (int q = 0; q < ns; q++) {
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0, e = 0.0, r = 0.0;
+    for (int t = 0; t < used; t++) {
+      const double *b = s->tacc + t * blk;
+      f0 += b[3L * q];
+      f1 += b[3L * q + 1];
+      f2 += b[3L * q + 2];
+      e += b[3L * ns + q];
+      r += b[4L * ns + q];
+    }
+    s->sf[3 * q] = f0;
+    s->sf[3 * q + 1] = f1;
+    s->sf[3 * q + 2] = f2;
+    s->sepot[q] = e;
+    s->srho[q] = r;
+  }
+  if (rt) {
+    /* add_rho: buffer copies back to their sources (x, then y, then z) */
+    for (int q = ns - 1; q >= s->n; q--) s->srho[s->ssrc[q]] += s->srho[q];
+    /* embedding energy and 2F' */
+    double eemb = 0.0;
+#pragma omp parallel for num_threads(T) schedule(dynamic, 64) \
+    reduction(+ : eemb)
+    for (int c = 0; c < s->ncells; c++) {
+      ivec *p = &s->cells[s->inner[c]];
+      int idummy = 0;
+      for (int i = 0; i < p->n; i++) {
+        int ia = p->a[i];
+        double pot, df;
+        orc_pair_int2(et, s->stype[ia], nt, s->srho[ia], &pot, &df, &idummy);
+        s->sdf[ia] = df;
+        s->sepot[ia] += pot;
+        eemb += pot;
+      }
+    }
+    epot += eemb;
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (int q = s->n; q < ns; q++) s->sdf[q] = s->sdf[s->ssrc[q]];
+    /* loop B: EAM forces */
+    long fblk = 3L * ns;
+    double vir2 = 0.0, vabs2 = 0.0;
+#pragma omp parallel num_threads(T) reduction(+ : vir2, vabs2) \
+    reduction(| : shrt)
+    {
+      int t = omp_get_thread_num();
+      if (t == 0) s->tused = omp_get_num_threads();
+      double *af = s->tacc + t * fblk;
+      memset(af, 0, (size_t)fblk * sizeof(double));
+#pragma omp for schedule(static)
+      for (int n = 0; n < s->n; n++) {
+        int ia = s->tl_atom[n];
+        const double *d1 = s->sx + 3 * ia;
+        double ff[3] = {0, 0, 0};
+        int it = s->stype[ia];
+        for (int m = s->tl[n]; m < s->tl[n + 1]; m++) {
+          int js = s->tb[m];
+          double d[3] = {s->sx[3 * js] - d1[0], s->sx[3 * js + 1] - d1[1],
+                         s->sx[3 * js + 2] - d1[2]};
+          double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          int jt = s->stype[js], col1 = jt * nt + it, col2 = it * nt + jt;
+          if (r2 < rt->end[col1] || r2 < rt->end[col2]) {
+            double rho_i_s = orc_deriv_func2(rt, col1, inc, r2, &shrt);
+            double rho_j_s = (col1 == col2)
+                                 ? rho_i_s
+                                 : orc_deriv_func2(rt, col2, inc, r2, &shrt);
+            double grad = 0.5 * (s->sdf[ia] * rho_j_s + s->sdf[js] * rho_i_s);
+            double fc[3] = {d[0] * grad, d[1] * grad, d[2] * grad};
+            af[3 * js] -= fc[0];
+            af[3 * js + 1] -= fc[1];
+            af[3 * js + 2] -= fc[2];
+            ff[0] += fc[0];
+            ff[1] += fc[1];
+            ff[2] += fc[2];
+            double w = d[0] * fc[0] + d[1] * fc[1] + d[2] * fc[2];
+            vir2 -= w;
+            vabs2 += fabs(w);
+          }
+        }
+        af[3 * ia] += ff[0];
+        af[3 * ia + 1] += ff[1];
+        af[3 * ia + 2] += ff[2];
+      }
+    }
+    used = s->tused;
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (int q = 0; q < ns; q++) {
+      for (int t = 0; t < used; t++) {
+        const double *b = s->tacc + t * fblk;
+        s->sf[3 * q] += b[3L * q];
+        s->sf[3 * q + 1] += b[3L * q + 1];
+        s->sf[3 * q + 2] += b[3L * q + 2];
+      }
+    }
+    vir += vir2;
+    vabs += vabs2;
+  }
+  /* add_forces: buffer copies back to their sources (x, then y, then z) */
+  for (int q = ns - 1; q >= s->n; q--) {
+    int src = s->ssrc[q];
+    s->sf[3 * src] += s->sf[3 * q];
+    s->sf[3 * src + 1] += s->sf[3 * q + 1];
+    s->sf[3 * src + 2] += s->sf[3 * q + 2];
+    s->sepot[src] += s->sepot[q];
+  }
+  s->tot_pot_energy = epot;
+  s->virial = vir;
+  s->virial_abs = vabs;
+  s->is_short |= shrt;
+  return ORC_OK;
+}
+
 int orc_calc_forces(orc_sys *s, int steps) {
   (void)steps;
   int rc, nt = s->ntypes, inc = nt * nt;
@@ -877,6 +1157,7 @@ int orc_calc_forces(orc_sys *s, int steps) {
     if (s->do_press)
       for (int k = 0; k < 6; k++) s->spress[6 * q + k] = 0.0;
   }
+  if (s->nthreads > 1 && !s->do_press && !s->emod) return forces_threaded(s);
   const orc_table *pp = s->pair, *rt = s->rho_t, *et = s->embed;
   int is_short = 0;
   /* loop A: pair forces and host electron density */
@@ -1042,7 +1323,11 @@ int orc_calc_forces(orc_sys *s, int steps) {
 void orc_move_atoms(orc_sys *s) {
   double dt = s->timestep;
   if (s->ensemble == ORC_NVE) {
-    s->tot_kin_energy = 0.0;
+    double ek = 0.0;
+    /* one thread unless orc_set_threads asked for more; one thread keeps
+       IMD's serial order (the reduction starts from 0 and adds once) */
+#pragma omp parallel for if (s->nthreads > 1) num_threads(s->nthreads) \
+    schedule(static) reduction(+ : ek)
     for (int c = 0; c < s->ncells; c++) {
       ivec *cl = &s->cells[s->inner[c]];
       for (int a = 0; a < cl->n; a++) {
@@ -1053,7 +1338,7 @@ void orc_move_atoms(orc_sys *s) {
         p[1] += dt * f[1];
         p[2] += dt * f[2];
         double k2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
-        s->tot_kin_energy += (k1 + k2) / (4 * s->mass[i]);
+        ek += (k1 + k2) / (4 * s->mass[i]);
         double tmp = dt / s->mass[i];
         x[0] += tmp * p[0];
         x[1] += tmp * p[1];
@@ -1061,10 +1346,13 @@ void orc_move_atoms(orc_sys *s) {
         if (s->do_press) press_kin(s->spress + 6 * i, p, s->mass[i]);
       }
     }
+    s->tot_kin_energy = ek;
   } else {
     double e1 = 0.0, e2 = 0.0;
     double reibung = 1.0 - s->eta * dt / 2.0;
     double eins_d_reib = 1.0 / (1.0 + s->eta * dt / 2.0);
+#pragma omp parallel for if (s->nthreads > 1) num_threads(s->nthreads) \
+    schedule(static) reduction(+ : e1, e2)
     for (int c = 0; c < s->ncells; c++) {
       ivec *cl = &s->cells[s->inner[c]];
       for (int a = 0; a < cl->n; a++) {
@@ -1091,6 +1379,8 @@ void orc_move_atoms(orc_sys *s) {
 /* check_nblist (src/imd_forces_nbl.c:2007-2037) */
 void orc_check_nblist(orc_sys *s) {
   double max1 = 0.0;
+#pragma omp parallel for if (s->nthreads > 1) num_threads(s->nthreads) \
+    schedule(static) reduction(max : max1)
   for (int i = 0; i < s->n; i++) {
     double d[3] = {s->x[3 * i] - s->x_nbl[3 * i],
                    s->x[3 * i + 1] - s->x_nbl[3 * i + 1],

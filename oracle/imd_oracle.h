@@ -120,6 +120,12 @@ int  orc_calc_forces(orc_sys *s, int steps);
    src/imd_integrate.c:411-431, 1085-1094); orc_get_press gives 6 per atom
    (xx yy zz yz zx xy, PRESSTENS), caller order */
 void orc_set_press(orc_sys *s, int on);
+/* Threads for a timing run on several cores (default 1: IMD's serial
+   order).  With more than one, pair/EAM force loops (without stress or
+   EEAM), the integrator and the skin test run split over threads and sum
+   in another order, so results differ from the serial run in the last
+   bits; the serial run is the checker. */
+void orc_set_threads(orc_sys *s, int n);
 /* EEAM (src/imd_forces_nbl.c:591-608, 1090-1095, 1180-1207): a second
    density P_i = sum_j rho_ij^2 with the energy term E(P) of the emod
    table (ntypes columns, not radial; src/imd_potential.c:82-85); set after
