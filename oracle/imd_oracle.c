@@ -1061,8 +1061,11 @@ static int forces_threaded(orc_sys *s) {
       }
     }
     epot += eemb;
+    /* copy_dF: a buffer slot may be a copy of an earlier buffer slot, so
+       each takes the value of its real root atom (the same value the
+       chain of copies carries), which keeps the copies independent */
 #pragma omp parallel for num_threads(T) schedule(static)
-    for (int q = s->n; q < ns; q++) s->sdf[q] = s->sdf[s->ssrc[q]];
+    for (int q = s->n; q < ns; q++) s->sdf[q] = s->sdf[s->sroot[q]];
     /* loop B: EAM forces */
     long fblk = 3L * ns;
     double vir2 = 0.0, vabs2 = 0.0;
