@@ -114,7 +114,8 @@ void imd_gpu_reconfigure(void) {
   /* a new phase may change the ensemble, time step, thermostat, list
      parameters or box: carry the atoms over into a new context */
   fetch_atoms();
-  if (do_maxwell) maxwell(temperature);
+  /* main_loop's Maxwell step (src/imd_main_3d.c:87-91): not on a restart */
+  if (!imdrestart && do_maxwell) maxwell(temperature);
   do_maxwell = 0;
   make_box();
   imd_gpu_finish();

@@ -12,7 +12,12 @@
  *   in box_param unit cells of edge box_unit (src/imd_generate.c:306-486).
  * maxwell: momenta N(0, sqrt(kT m)) per component from drand48 with the
  *   polar method, then the mean momentum removed (src/imd_maxwell.c:44-323),
- *   in input order.
+ *   in IMD's cell traversal order: inner cells x-major, then y, then z
+ *   (make_cell_lists), atoms of a cell in input order, cells from
+ *   init_cells' grid and cell_coord after back_into_box, as read_atoms
+ *   fills them (src/imd_io_3d.c:318-333, 425-442).  drand48 is seeded
+ *   once, when the atoms are read or generated (src/imd_io_3d.c:87,
+ *   src/imd_generate.c:80), so a later phase continues the stream.
  * write_eng_file: one line per eng_int steps with time, Epot per atom,
  *   temperature 2 Ekin / nactive, pressure T/(V/N) + virial/(3V), volume per
  *   atom, for NVT eta * tau_eta, and with the stress tensor on the six
@@ -165,6 +170,7 @@ int read_atoms(const char *filename) {
   if (missing_vel) do_maxwell = 1;
   nactive = (int)(3 * natoms);
   make_box();
+  srand48(seed);
   return 0;
 }
 
@@ -196,6 +202,7 @@ int generate_fcc(void) {
       }
   nactive = (int)(3 * natoms);
   make_box();
+  srand48(seed);
   do_maxwell = 1;
   return 0;
 }
@@ -211,16 +218,78 @@ static double gaussian(double sigma) {
   return sigma * y * sqrt(-2.0 * log(r2) / r2);
 }
 
+/* Atom indices in IMD's cell traversal order: the grid of init_cells
+   (src/imd_geom_3d.c:113-170: the skin added to the tables' list radius,
+   gdim = (int)(1 / sqrt(cellsz / height)) per direction), positions put
+   back into the box along periodic directions (back_into_box), cell_coord
+   (src/imd_geom_3d.c:1054-1074: (int)(gdim (x . tbox)), clamped), inner
+   cells x-major, then y, then z, atoms of a cell in input order (a stable
+   counting sort).  NULL when the box or the radius leaves no grid; the
+   caller then keeps input order. */
+static long *cell_order(void) {
+  const double *b[3] = {box_x, box_y, box_z};
+  double t[3][3], vol;
+  for (int d = 0; d < 3; d++) {
+    const double *u = b[(d + 1) % 3], *v = b[(d + 2) % 3];
+    t[d][0] = u[1] * v[2] - u[2] * v[1];
+    t[d][1] = u[2] * v[0] - u[0] * v[2];
+    t[d][2] = u[0] * v[1] - u[1] * v[0];
+  }
+  vol = box_x[0] * t[0][0] + box_x[1] * t[0][1] + box_x[2] * t[0][2];
+  if (vol == 0 || natoms == 0 || !(cellsz > 0)) return NULL;
+  int g[3];
+  double rl = sqrt(cellsz) + nbl_margin;
+  for (int d = 0; d < 3; d++) {
+    for (int e = 0; e < 3; e++) t[d][e] /= vol;
+    double h = 1.0 / (t[d][0] * t[d][0] + t[d][1] * t[d][1] + t[d][2] * t[d][2]);
+    g[d] = (int)(1.0 / sqrt(rl * rl / h));
+    if (g[d] < 1) return NULL;
+  }
+  long ncell = (long)g[0] * g[1] * g[2];
+  long *cell = malloc(sizeof(long) * natoms);
+  long *first = calloc(ncell + 1, sizeof(long));
+  long *order = malloc(sizeof(long) * natoms);
+  if (!cell || !first || !order) {
+    free(cell);
+    free(first);
+    free(order);
+    return NULL;
+  }
+  for (long i = 0; i < natoms; i++) {
+    double x[3] = {atom_x[3 * i], atom_x[3 * i + 1], atom_x[3 * i + 2]};
+    for (int d = 0; d < 3; d++) {
+      if (pbc_dirs[d] != 1) continue;
+      double f = -floor(x[0] * t[d][0] + x[1] * t[d][1] + x[2] * t[d][2]);
+      for (int e = 0; e < 3; e++) x[e] += f * b[d][e];
+    }
+    int c[3];
+    for (int d = 0; d < 3; d++) {
+      c[d] = (int)(g[d] * (x[0] * t[d][0] + x[1] * t[d][1] + x[2] * t[d][2]));
+      if (c[d] < 0) c[d] = 0;
+      if (c[d] > g[d] - 1) c[d] = g[d] - 1;
+    }
+    cell[i] = ((long)c[0] * g[1] + c[1]) * g[2] + c[2];
+    first[cell[i] + 1]++;
+  }
+  for (long k = 0; k < ncell; k++) first[k + 1] += first[k];
+  for (long i = 0; i < natoms; i++) order[first[cell[i]]++] = i;
+  free(cell);
+  free(first);
+  return order;
+}
+
 void maxwell(double temp) {
   double tot[3] = {0, 0, 0};
-  srand48(seed);
-  for (long i = 0; i < natoms; i++) {
+  long *order = cell_order();
+  for (long n = 0; n < natoms; n++) {
+    long i = order ? order[n] : n;
     double s = sqrt(temp * atom_mass[i]);
     for (int c = 0; c < 3; c++) {
       atom_p[3 * i + c] = gaussian(s);
       tot[c] += atom_p[3 * i + c];
     }
   }
+  free(order);
   for (int c = 0; c < 3; c++) tot[c] = natoms ? tot[c] / natoms : 0.0;
   for (long i = 0; i < natoms; i++)
     for (int c = 0; c < 3; c++) atom_p[3 * i + c] -= tot[c];

@@ -382,3 +382,86 @@ def test_press_keys_and_file_format(host, tmp_path):
     assert rows.shape == (n, 12)
     assert np.array_equal(rows[:, 3:6], s.x)
     assert np.array_equal(rows[:, 6:], want)
+
+
+def _polar(rand, sigma):
+    """Knuth's polar method (vol. 2, 3rd ed., p. 122) on a uniform stream."""
+    while True:
+        x = -1 + 2 * rand()
+        y = -1 + 2 * rand()
+        r2 = x * x + y * y
+        if not (r2 > 1.0 or r2 == 0):
+            return sigma * y * np.sqrt(-2.0 * np.log(r2) / r2)
+
+
+def test_maxwell_cell_order_and_stream(host, tmp_path):
+    """maxwell (src/imd_maxwell.c:44-323) draws three normal variates per
+    atom in IMD's cell traversal order from the drand48 stream that
+    read_atoms seeded (src/imd_io_3d.c:87), then removes the mean momentum.
+    The stream is libc's own drand48, the traversal order is the oracle's
+    (its binning is pinned by the grid tests), and the atoms are written
+    shuffled so that input order and cell order differ."""
+    eam = W.johnson_eam((W.CU, W.NI), rc=5.0, npts=400)
+    s = W.fcc_system((4, 5, 4), 3.615, eam.masses, W.kT(300.0), 13,
+                     type_fractions=(0.6, 0.4), displace=0.05)
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(s.n)
+    pos = s.x[perm].copy()
+    pos[::7, 0] += s.box[0, 0]  # outside the box: back_into_box before binning
+    types, masses = s.type[perm], s.mass[perm]
+    path = str(tmp_path / "nov.conf")
+    with open(path, "w") as f:
+        f.write("#F A 1 1 1 3 0 0\n#C number type mass x y z\n")
+        for d, v in zip("XYZ", s.box):
+            f.write("#%s %.17g %.17g %.17g\n" % (d, *v))
+        f.write("#E\n")
+        for k in range(s.n):
+            f.write("%d %d %.17g %.17g %.17g %.17g\n" % (
+                k, types[k], masses[k], *pos[k]))
+    host.init_parameters()
+    C.c_int.in_dll(host, "ntypes").value = 2
+    C.c_long.in_dll(host, "seed").value = -77
+    C.c_double.in_dll(host, "nbl_margin").value = 0.4
+    rc_list = max(eam.pair.end.max(), eam.rho.end.max())
+    C.c_double.in_dll(host, "cellsz").value = rc_list
+    assert host.read_atoms(path.encode()) == 0
+    assert C.c_int.in_dll(host, "do_maxwell").value == 1
+    T = W.kT(300.0)
+    host.maxwell.argtypes = [C.c_double]
+    host.maxwell(T)
+    n = s.n
+    p = np.ctypeslib.as_array(C.POINTER(C.c_double).in_dll(host, "atom_p"),
+                              (n, 3)).copy()
+    # the cell of each atom as IMD bins it at read time (oracle), then the
+    # inner cells x-major, y, z, and input order within a cell
+    sysobj = W.System(x=pos, p=np.zeros_like(pos), type=types.astype(np.int32),
+                      mass=masses, box=s.box, pbc=(1, 1, 1),
+                      id=np.arange(n, dtype=np.int64))
+    pair = O.OracleTable.from_table(eam.pair)
+    rho = O.OracleTable.from_table(eam.rho)
+    emb = O.OracleTable.from_table(eam.embed, radial=False)
+    orc = O.OracleSystem(sysobj, pair, rho, emb, nbl_margin=0.4, ntypes=2)
+    orc.calc_forces()
+    g = orc.grid()
+    c = orc.cells()
+    key = (c[:, 0] * g[1] + c[:, 1]) * g[2] + c[:, 2]
+    order = np.argsort(key, kind="stable")
+    assert not np.array_equal(order, np.arange(n))
+    libc = C.CDLL(None)
+    libc.drand48.restype = C.c_double
+    libc.srand48.argtypes = [C.c_long]
+    libc.srand48(-77)
+    want = np.zeros((n, 3))
+    for i in order:
+        sig = np.sqrt(T * masses[i])
+        for d in range(3):
+            want[i, d] = _polar(libc.drand48, sig)
+    tot = np.zeros(3)
+    for i in order:
+        tot += want[i]
+    want -= tot / n
+    assert np.abs(p - want).max() <= 1e-15 * np.abs(want).max()
+    assert np.abs(p.sum(axis=0)).max() < 1e-12 * np.abs(p).sum()
+    # the variance is kT m per component (n = 320 atoms: 3 sigma bands)
+    z = p / np.sqrt(T * masses)[:, None]
+    assert abs(z.var() - 1.0) < 3 * np.sqrt(2.0 / z.size)
