@@ -80,6 +80,8 @@ struct imdg_ctx {
   bool cubic = false;   // 4-point or spline tables (cubic segments)
   bool eeam = false;    // EEAM energy term present (tab[3])
   int xcd_remap = 1;
+  int tile_cap = IMDG_TILE_CAP;  // records of a list-build tile (a smaller
+                                 // cap sends more blocks to the fallback)
   // tables
   HostTable tab[4];  // pair, rho, embed, EEAM energy term (emod)
   bool tables_set = false;
@@ -730,6 +732,7 @@ BuildArgs build_args(imdg_ctx* c) {
   b.ncells = c->ncells;
   b.nstile = c->nstile;
   b.kmax = c->kmax;
+  b.tile_cap = c->tile_cap;
   return b;
 }
 
@@ -1127,6 +1130,8 @@ int imdg_ctx_create(const imdg_config* cfg, imdg_ctx** out) {
   const char* graph = getenv("IMDG_GRAPH");
   c->use_graph = graph ? atoi(graph) != 0 : false;
   if (const char* e = getenv("IMDG_SPLIT_MOVE")) c->split_move = atoi(e) != 0;
+  if (const char* e = getenv("IMDG_TILE_LIMIT"))
+    c->tile_cap = std::max(0, std::min(atoi(e), IMDG_TILE_CAP));
   if (const char* e = getenv("IMDG_FINAL_KERNEL"))
     c->final_inline = atoi(e) == 0;
   if (cfg->lanes_per_atom > 0)
@@ -1922,8 +1927,16 @@ int imdg_get_info(imdg_ctx* c, imdg_info* info) {
   info->kmax_seen = h.kmax_seen;
   info->steps_done = h.steps_done;
   info->is_short = h.is_short;
+  if (h.is_short) {
+    // reported once: the flag covers the work since the previous call
+    static const int zero = 0;
+    HIPCHK(c, hipMemcpyAsync(&c->d_ctl->is_short, &zero, sizeof(int),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   info->nbl_count = st.nbl_count;
   info->tiles_fallback = h.tiles_fallback;
+  info->exact_pairs = h.exact_pairs;
   std::vector<int> hn(c->nslots);
   HIPCHK(c, hipMemcpy(hn.data(), c->d_nnb, sizeof(int) * c->nslots,
                       hipMemcpyDeviceToHost));

@@ -433,7 +433,7 @@ __device__ void build_tile(const BuildArgs& a, int b, TileLds& S) {
   }
   int tot = __shfl(pos, IMDG_WAVE - 1, IMDG_WAVE);
   pos -= len;
-  ok = ok && tot <= IMDG_TILE_CAP;
+  ok = ok && tot <= a.tile_cap;
   if (ok && l < nrun) T.run[l] = make_int4(run[l].x, len, pos, 0);
   if (l == 0) {
     if (!ok) {
@@ -612,6 +612,8 @@ __device__ __forceinline__ void list_cell(
       band[u] = valid && !in[u] && r2 < L.hi;
     }
     if (band[0] || band[1] || band[2] || band[3]) {
+      atomicAdd(&a.ctl->exact_pairs,
+                (int)band[0] + (int)band[1] + (int)band[2] + (int)band[3]);
 #pragma unroll
       for (int u = 0; u < 4; u++)
         if (band[u])

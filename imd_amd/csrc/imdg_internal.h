@@ -121,7 +121,8 @@ struct Ctl {
                           // its last block (frozen never changes mid-kernel)
   int kmax_seen;
   int is_short;
-  int tiles_fallback;     // blocks whose tile exceeded IMDG_TILE_CAP
+  int tiles_fallback;     // blocks whose tile exceeded the tile cap
+  int exact_pairs;        // list candidates decided by the FP64 expression
   unsigned int done[DC_N];
   long steps_done;
 };
@@ -249,6 +250,7 @@ struct BuildArgs {
   uint32_t* nbr;
   int* nnb;
   int* blkmax;            // longest list of every block (k_list)
+  int tile_cap;           // records a tile may hold (<= IMDG_TILE_CAP)
   int nrep;
   int nslots;
   int nblk;

@@ -27,6 +27,7 @@
 
 static imdg_ctx *ctx = NULL;
 static double t_start = -1.0;
+double main_loop_seconds = 0.0;
 
 void (*move_atoms)(void) = NULL;
 
@@ -122,7 +123,16 @@ void imd_gpu_reconfigure(void) {
   imd_gpu_init();
 }
 
-static void pull_thermo(void) {
+/* IMD's warnings after a force evaluation with a pair closer than a
+   table's begin (src/imd_forces_nbl.c:982, 1323): is_short stays set from
+   the pair loop into the EAM loop there, so an EAM system prints both */
+static void warn_short(const imdg_info *info, int steps) {
+  if (!info->is_short) return;
+  fprintf(stderr, "Short distance, pair, step %d!\n", steps);
+  if (have_eam) fprintf(stderr, "\n Short distance, EAM, step %d!\n", steps);
+}
+
+static void pull_thermo(int steps) {
   imdg_thermo th;
   check(imdg_get_thermo(ctx, 0, &th), "imdg_get_thermo");
   tot_pot_energy = th.epot;
@@ -132,11 +142,12 @@ static void pull_thermo(void) {
   imdg_info info;
   check(imdg_get_info(ctx, &info), "imdg_get_info");
   nbl_count = (int)info.nbl_count;
+  if (steps >= 0) warn_short(&info, steps);
 }
 
 void calc_forces(int steps) {
   check(imdg_calc_forces(ctx, steps), "calc_forces");
-  pull_thermo();
+  pull_thermo(steps);
 }
 
 void make_nblist(void) { check(imdg_make_nblist(ctx), "make_nblist"); }
@@ -151,12 +162,12 @@ void deallocate_nblist(void) {
 
 void move_atoms_nve(void) {
   check(imdg_move_atoms(ctx), "move_atoms");
-  pull_thermo();
+  pull_thermo(-1);
 }
 
 void move_atoms_nvt(void) {
   check(imdg_move_atoms(ctx), "move_atoms");
-  pull_thermo();
+  pull_thermo(-1);
 }
 
 void fetch_atoms(void) {
@@ -286,6 +297,13 @@ int main_loop(void) {
       tot_kin_energy = hist[0].ekin;
       virial = hist[0].virial;
       eta = hist[0].eta;
+      {
+        /* the device flags a short distance per fused run: the warning
+           names the run's last step */
+        imdg_info info;
+        check(imdg_get_info(ctx, &info), "imdg_get_info");
+        warn_short(&info, last);
+      }
       if (step_outputs(last)) finished = 1;
       done += n;
       done_steps += n;
@@ -294,6 +312,7 @@ int main_loop(void) {
     }
   }
   double el = wall() - t0;
+  main_loop_seconds += el;
   imdg_info info;
   check(imdg_get_info(ctx, &info), "imdg_get_info");
   nbl_count = (int)info.nbl_count;

@@ -79,6 +79,7 @@ extern long *atom_number;
 /* thermodynamic results */
 extern double tot_pot_energy, tot_kin_energy, virial, volume;
 extern int nbl_count;
+extern double main_loop_seconds;     /* wall time spent in main_loop */
 
 extern void (*move_atoms)(void);
 

@@ -126,12 +126,19 @@ typedef struct {
   int    kmax_seen;     /* largest list length built so far */
   long   nbl_count;     /* list builds of replica 0 */
   long   steps_done;    /* device step counter */
-  int    is_short;      /* some r^2 fell below a table's begin */
+  int    is_short;      /* some r^2 fell below a table's begin since the
+                           previous imdg_get_info (which clears it): IMD's
+                           is_short of calc_forces
+                           (src/imd_forces_nbl.c:982, 1323) */
   int    lanes_per_atom;/* lanes per atom of the force loop */
   double list_bytes;    /* bytes of list storage */
   double mean_k;        /* mean full-list length per real atom */
   int    tiles_fallback;/* blocks whose neighbour tile did not fit in LDS
                            (their neighbours are gathered from HBM) */
+  long   exact_pairs;   /* list candidates (so far) within the FP32 screening band
+                           of the list radius, decided by IMD's FP64
+                           expression r^2 < cellsz
+                           (src/imd_forces_nbl.c:249-261) */
 } imdg_info;
 
 /* lifecycle */

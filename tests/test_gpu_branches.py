@@ -1,0 +1,123 @@
+"""Branches the ordinary parity cases do not reach.
+
+* The list build's fallback for blocks whose neighbour tile does not fit in
+  LDS (their candidates are read from HBM): a lowered tile cap
+  (IMDG_TILE_LIMIT) sends blocks there; the pair sets and forces still
+  match the oracle, and a fused run gives bit-identical results with and
+  without the fallback (the list is the same).
+* Pairs closer than a table's begin: IMD clamps r^2 to begin and sets
+  is_short (src/potaccess.h:329-334, src/imd_forces_nbl.c:982, 1323); the
+  forces at the clamped argument and the flag must match the oracle, and
+  imdg_get_info reports the flag once.
+"""
+import numpy as np
+import pytest
+
+from imd_amd import imdg as G
+from imd_amd import workloads as W
+
+from helpers import (canon_pairs_gpu, canon_pairs_oracle, eam_tables, lj,
+                     oracle_eam, oracle_lj, rel_rms)
+
+pytestmark = pytest.mark.gpu
+
+BARS = {G.FP64: (1e-10, 1e-10), G.MIXED: (1e-5, 1e-6)}
+
+
+def _static(gpu, orc, precision, has_eam=True):
+    tol_f, tol_s = BARS[precision]
+    ga, oa = gpu.atoms(), orc.atoms()
+    assert rel_rms(ga["f"], oa["f"]) <= tol_f
+    assert rel_rms(ga["epot"], oa["epot"]) <= tol_s
+    if has_eam:
+        assert rel_rms(ga["rho"], oa["rho"]) <= tol_s
+        assert rel_rms(ga["df"], oa["df"]) <= tol_s
+    th = gpu.thermo()
+    assert abs(th["epot"] - orc.scalar("epot")) <= tol_s * abs(orc.scalar("epot"))
+    assert abs(th["virial"] - orc.scalar("virial")) <= \
+        tol_s * orc.scalar("virial_abs")
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.MIXED])
+def test_tile_fallback(precision, monkeypatch):
+    eam = eam_tables()
+    s = W.eam_crystal(n_cells=8, temp_k=900.0, seed=41, displace=0.04)
+    orc = oracle_eam(s, eam, nbl_margin=0.4, timestep=W.FS)
+    orc.calc_forces()
+    monkeypatch.setenv("IMDG_TILE_LIMIT", "400")
+    low = G.eam_context(eam, [s], precision=precision)
+    monkeypatch.delenv("IMDG_TILE_LIMIT")
+    with low:
+        low.calc_forces()
+        info = low.info()
+        assert info["tiles_fallback"] > 0
+        assert canon_pairs_gpu(low.pairs()) == canon_pairs_oracle(orc.pairs())
+        _static(low, orc, precision)
+        low.run(30)
+        a_low = low.atoms()
+        low.make_nblist()
+        n_fb = low.info()["tiles_fallback"]
+    with G.eam_context(eam, [s], precision=precision) as full:
+        full.calc_forces()
+        assert full.info()["tiles_fallback"] == 0
+        full.run(30)
+        a_full = full.atoms()
+    # a later rebuild goes through the fallback again
+    assert n_fb > info["tiles_fallback"]
+    for k in ("x", "p", "f", "epot"):
+        assert np.array_equal(a_low[k], a_full[k]), k
+
+
+def _bring_close(s, i, dist):
+    """Move the nearest neighbour of atom i (minimum image) to distance dist
+    along the line that joins them."""
+    L = np.diag(s.box)
+    d = s.x - s.x[i]
+    d -= L * np.round(d / L)
+    r = np.sqrt(np.sum(d * d, axis=1))
+    r[i] = np.inf
+    j = int(np.argmin(r))
+    x = s.x.copy()
+    x[j] = s.x[i] + d[j] * (dist / r[j])
+    x[j] -= L * np.floor(x[j] / L)
+    s.x = x
+    return j
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.MIXED])
+def test_short_distance_eam(precision):
+    eam = eam_tables()
+    s = W.eam_crystal(n_cells=5, temp_k=300.0, seed=43, displace=0.02)
+    _bring_close(s, 5, 0.95)          # r^2 = 0.9025 below begin 1.0
+    orc = oracle_eam(s, eam, nbl_margin=0.4, timestep=W.FS)
+    orc.calc_forces()
+    assert orc.scalar("is_short") == 1
+    with G.eam_context(eam, [s], precision=precision) as gpu:
+        gpu.calc_forces()
+        assert canon_pairs_gpu(gpu.pairs()) == canon_pairs_oracle(orc.pairs())
+        _static(gpu, orc, precision)
+        assert gpu.info()["is_short"] == 1
+        assert gpu.info()["is_short"] == 0      # reported once
+        gpu.calc_forces()
+        assert gpu.info()["is_short"] == 1
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.MIXED])
+def test_short_distance_pair_only(precision):
+    s = W.fcc_system((6, 6, 6), 1.5496, [1.0], 0.5, 44, displace=0.02)
+    _bring_close(s, 10, 0.45)         # r^2 = 0.2025 below begin 0.25
+    orc = oracle_lj(s, nbl_margin=0.3, timestep=0.002)
+    orc.calc_forces()
+    assert orc.scalar("is_short") == 1
+    with G.pair_context(lj(), [s], precision=precision) as gpu:
+        gpu.calc_forces()
+        _static(gpu, orc, precision, has_eam=False)
+        assert gpu.info()["is_short"] == 1
+
+
+def test_no_short_distance_flag_in_a_crystal():
+    eam = eam_tables()
+    s = W.eam_crystal(n_cells=5, temp_k=300.0, seed=45, displace=0.02)
+    with G.eam_context(eam, [s], precision=G.MIXED) as gpu:
+        gpu.run(5)
+        assert gpu.info()["is_short"] == 0

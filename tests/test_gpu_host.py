@@ -313,3 +313,93 @@ def test_eeam_run_matches_oracle(tmp_path):
     rows = _rows(str(tmp_path / "run.00001.chkpt"))
     P, _ = orc.eeam()
     assert np.allclose(rows[:, 11], P, rtol=1e-9)
+
+
+def test_imd_gpu_maxwell_start_matches_oracle(tmp_path):
+    """A configuration without velocities: imd_gpu draws Maxwell momenta
+    (src/imd_maxwell.c:44-301: three variates per atom from the drand48
+    stream seeded at read time, cells in IMD's traversal order, atoms of a
+    cell in input order, then the mean momentum removed) and runs 30 NVE
+    steps; the oracle starts from the same draw, made here with libc's
+    drand48, and its thermo must match the .eng rows."""
+    import ctypes as C
+    from test_host import _polar
+    eam = eam_tables()
+    s = W.eam_crystal(n_cells=5, temp_k=0.0, seed=27, displace=0.04)
+    perm = np.random.default_rng(4).permutation(s.n)
+    x = s.x[perm].copy()
+    files = eam.write(str(tmp_path / "pot"))
+    cfg = str(tmp_path / "nov.conf")
+    with open(cfg, "w") as f:
+        f.write("#F A 1 1 1 3 0 0\n#C number type mass x y z\n")
+        for d, v in zip("XYZ", s.box):
+            f.write("#%s %.17g %.17g %.17g\n" % (d, *v))
+        f.write("#E\n")
+        for k in range(s.n):
+            f.write("%d 0 %.17g %.17g %.17g %.17g\n" % (k, s.mass[0], *x[k]))
+    kT = W.kT(500.0)
+    nsteps = 30
+    lines = _lines(s, files, cfg, str(tmp_path / "run"), "nve", nsteps,
+                   extra=["starttemp %.17g" % kT, "seed -4242"])
+    _run(tmp_path, lines)
+    eng = _read_eng(str(tmp_path / "run.eng"))
+    # the same draw on the CPU: the oracle's cells (pinned against the
+    # grid closed forms) give the traversal order
+    sysobj = W.System(x=x, p=np.zeros_like(x), type=np.zeros(s.n, np.int32),
+                      mass=s.mass.copy(), box=s.box, pbc=(1, 1, 1),
+                      id=np.arange(s.n, dtype=np.int64))
+    probe = oracle_eam(sysobj, eam, nbl_margin=0.4, timestep=W.FS)
+    probe.calc_forces()
+    g, c = probe.grid(), probe.cells()
+    order = np.argsort((c[:, 0] * g[1] + c[:, 1]) * g[2] + c[:, 2],
+                       kind="stable")
+    libc = C.CDLL(None)
+    libc.drand48.restype = C.c_double
+    libc.srand48.argtypes = [C.c_long]
+    libc.srand48(-4242)
+    p = np.zeros((s.n, 3))
+    for i in order:
+        sig = np.sqrt(kT * s.mass[i])
+        for d in range(3):
+            p[i, d] = _polar(libc.drand48, sig)
+    tot = np.zeros(3)
+    for i in order:
+        tot += p[i]
+    p -= tot / s.n
+    sysobj.p = p
+    orc = oracle_eam(sysobj, eam, nbl_margin=0.4, timestep=W.FS)
+    th = orc.run(nsteps + 1)
+    assert eng.shape[0] == nsteps // 10 + 1
+    for row in eng:
+        step = int(round(row[0] / W.FS))
+        epot = th[step, 0] / s.n
+        temp = 2.0 * th[step, 1] / (3 * s.n)
+        assert abs(row[1] - epot) <= 1e-9 * abs(epot), (step, row[1], epot)
+        assert abs(row[2] - temp) <= 1e-5 * abs(temp), (step, row[2], temp)
+    # about the requested temperature at the start (320 atoms)
+    assert abs(2.0 * th[0, 1] / (3 * s.n) / kT - 1.0) < 0.25
+
+
+def test_imd_gpu_short_distance_warning_and_summary(tmp_path):
+    """A pair closer than the tables' begin: imd_gpu prints IMD's
+    'Short distance' warnings (src/imd_forces_nbl.c:982, 1323) and the
+    closing summary lines of src/imd.c:318-364."""
+    eam, s, files, cfg = _setup(tmp_path)
+    L = np.diag(s.box)
+    d = s.x - s.x[0]
+    d -= L * np.round(d / L)
+    r = np.sqrt(np.sum(d * d, axis=1))
+    r[0] = np.inf
+    j = int(np.argmin(r))
+    x = s.x.copy()
+    x[j] = s.x[0] + d[j] * (0.95 / r[j])
+    x[j] -= L * np.floor(x[j] / L)
+    s.x = x
+    cfg = s.write_config(str(tmp_path / "short.chkpt"))
+    lines = _lines(s, files, cfg, str(tmp_path / "run"), "nve", 20)
+    r = _run(tmp_path, lines)
+    assert "Short distance, pair, step" in r.stderr
+    assert "Short distance, EAM, step" in r.stderr
+    assert "Neighbor list update every" in r.stdout
+    assert "Did 21 steps with %d atoms." % s.n in r.stdout
+    assert "seconds per step and atom" in r.stdout

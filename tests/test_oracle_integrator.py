@@ -3,8 +3,11 @@
 Free particles (no partner inside the cutoff) have the closed-form
 trajectory x_n = x_0 + n dt p / m, the skin test then fires at a step that
 is known in advance, NVT with eta = 0 and 1/tau^2 = 0 is NVE, the
-Nose-Hoover thermostat drives the temperature to its target, and leapfrog's
-energy error shrinks as dt^2.
+Nose-Hoover thermostat drives the temperature to its target, leapfrog's
+energy error shrinks as dt^2, and for free particles with eta != 0 the
+Nose-Hoover recurrence (friction form, Ekin, eta update) is followed step by
+step in numpy, tight enough that an explicit friction step, a missing
+1/(1 + eta dt/2) or an eta update from E1 fail.
 """
 import numpy as np
 import pytest
@@ -112,3 +115,74 @@ def test_leapfrog_energy_error_scales_as_dt_squared():
         spread.append(e.std())
     ratio = spread[0] / spread[1]
     assert 2.5 < ratio < 6.0
+
+
+def nose_hoover_free(s, dt, T, eta0, isq, n, friction="imd", eta_from="e2"):
+    """Nose-Hoover for free particles (F = 0) in numpy, step by step:
+    reib = 1 - eta dt/2, einsd = 1/(1 + eta dt/2), p <- (p reib + dt F)
+    einsd, x <- x + (dt/m) p, Ekin = (E1 + E2)/4 and eta <- eta + dt (E2 /
+    (3N T) - 1) / tau^2, E1 and E2 = sum |p|^2/m before and after the
+    momentum update (src/imd_integrate.c:907-908, 1020-1027, 1103,
+    1140-1141).  friction="euler" and eta_from="e1" are the plausible
+    mistakes the pin must reject."""
+    m = s.mass[:, None]
+    p = s.p.astype(np.float64).copy()
+    x = s.x.astype(np.float64).copy()
+    eta = eta0
+    ek, etas = [], []
+    for _ in range(n):
+        reib = 1.0 - eta * dt / 2.0
+        einsd = 1.0 / (1.0 + eta * dt / 2.0)
+        e1 = np.sum(p * p / m)
+        if friction == "imd":
+            p = (p * reib + dt * 0.0) * einsd
+        elif friction == "euler":
+            p = p * (1.0 - eta * dt) + dt * 0.0
+        else:                       # the division by 1 + eta dt/2 left out
+            p = p * reib + dt * 0.0
+        e2 = np.sum(p * p / m)
+        x = x + (dt / m) * p
+        ek.append((e1 + e2) / 4.0)
+        eta += dt * ((e2 if eta_from == "e2" else e1) / (3 * s.n * T) - 1.0) * isq
+        etas.append(eta)
+    return x, p, np.array(ek), np.array(etas)
+
+
+NH_FREE = dict(dt=0.5, eta0=0.3, tau=6.0)
+
+
+def nh_free_case():
+    s = free_system(n=64, vmax=0.05, seed=21)
+    # target between the initial temperature and zero: the friction cools,
+    # eta relaxes through zero and changes sign within the run
+    T = 0.5 * np.sum(s.p ** 2 / s.mass[:, None]) / (3 * s.n)
+    return s, T
+
+
+def test_nose_hoover_free_particles_closed_form():
+    s, T = nh_free_case()
+    dt, eta0 = NH_FREE["dt"], NH_FREE["eta0"]
+    isq = 1.0 / NH_FREE["tau"] ** 2
+    n = 50
+    orc, _ = oracle_for(s, ensemble=O.ORC_NVT, dt=dt, temperature=T,
+                        eta=eta0, isq_tau_eta=isq)
+    th = orc.run(n)
+    a = orc.atoms()
+    x, p, ek, etas = nose_hoover_free(s, dt, T, eta0, isq, n)
+    # the recurrence moves eta and Ekin by O(1) relative amounts
+    assert etas.min() < 0.0 < etas.max()
+    assert ek[-1] < 0.6 * ek[0]
+    assert np.allclose(th[:, 3], etas, rtol=1e-14, atol=1e-15)
+    assert np.allclose(th[:, 1], ek, rtol=1e-14, atol=0)
+    assert np.allclose(a["p"], p, rtol=1e-13, atol=1e-18)
+    L = 30.0
+    d = a["x"] - x
+    d -= L * np.round(d / L)
+    assert np.max(np.abs(d)) < 1e-12
+    assert np.all(th[:, 0] == 0.0)
+    # each plausible mistake misses the bar by orders of magnitude
+    for kw in (dict(friction="euler"), dict(friction="no_einsd"),
+               dict(eta_from="e1")):
+        _, pw, ekw, etaw = nose_hoover_free(s, dt, T, eta0, isq, n, **kw)
+        assert np.max(np.abs(etaw - th[:, 3])) > 1e-6
+        assert np.max(np.abs(ekw - th[:, 1]) / ek[0]) > 1e-6
