@@ -42,12 +42,21 @@ def algorithmic_bytes(kind: str, k: float, mixed: bool) -> float:
       48 B p read/write, 4 B type, 16 B FP32 record out, 24 B x at the last
       build).
     FP64: density 4k+48, force 4k+72, integrate + skin test 148.
-    The fused force kernel carries force + integrate: 4k+188 | 4k+220.
+    "fused": the force kernel of a timed step, which integrates in the same
+    launch and keeps the forces in registers (per-atom forces and energies
+    are stored on a run's last step only): 4k+20 of the force part plus the
+    integration without its force read, 4k+160 (mixed) | 4k+164 (FP64).
+    "step": one whole step as the metric defines it, density + force +
+    integrate: 8k+212 (mixed) | 8k+268 (FP64).
     """
     if kind == "density":
         return 4 * k + (24 if mixed else 48)
     if kind == "force":
         return 4 * k + (36 + 152 if mixed else 72 + 148)
+    if kind == "fused":
+        return 4 * k + (20 + 140 if mixed else 40 + 124)
+    if kind == "step":
+        return 8 * k + (212 if mixed else 268)
     raise ValueError(kind)
 
 
@@ -171,21 +180,41 @@ def measure_traffic(args, kernel_key: str = "k_force"):
                 fetch_bytes=res["FETCH_SIZE"], write_bytes=res["WRITE_SIZE"])
 
 
+DRIFT_CONFIGS = {
+    "eam256k": "fcc EAM crystal 40^3 cells (256,000 atoms), 600 K, dt 1 fs, "
+               "skin 0.4 A, NVE",
+    "eam32k": "fcc EAM crystal 20^3 cells (32,000 atoms, a subset of the "
+              "256,000-atom workload), 600 K, dt 1 fs, skin 0.4 A, NVE",
+    "lj4000": "fcc LJ crystal 10^3 cells (4,000 atoms), kT 0.7, dt 0.002, "
+              "skin 0.3, NVE",
+}
+
+
 def drift_run(args):
-    """NVE energy drift: the 256,000-atom EAM crystal (40^3 cells, 600 K,
-    displacements N(0, 0.02 A), seed 1001, dt 1 fs, skin 0.4 A) for
-    args.drift steps in each precision.  E_tot per atom = (Epot + Ekin) / N
-    every 10 steps; drift = least-squares slope x 1000 steps / |<E_tot>|."""
+    """NVE energy drift of one input in each precision over args.drift
+    steps.  E_tot per atom = (Epot + Ekin) / N every 10 steps; drift =
+    least-squares slope x 1000 steps / |<E_tot>|.  The 32,000- and
+    4,000-atom inputs are the ones tests/golden/drift_oracle.py runs through
+    the oracle (the reference algorithm); its drift is printed beside."""
     from imd_amd import imdg as G
-    eam = W.johnson_eam()
-    s = W.eam_crystal(n_cells=40, temp_k=600.0, seed=1001, displace=0.02)
+    name = args.drift_config
+    if name == "lj4000":
+        s = W.lj_crystal()
+        make = lambda prec: G.pair_context(W.lj_table(), [s], precision=prec,
+                                           ensemble=G.NVE, timestep=0.002,
+                                           nbl_margin=0.3)
+    else:
+        eam = W.johnson_eam()
+        s = W.eam_crystal(n_cells=40 if name == "eam256k" else 20,
+                          temp_k=600.0, seed=1001, displace=0.02)
+        make = lambda prec: G.eam_context(eam, [s], precision=prec,
+                                          ensemble=G.NVE, timestep=W.FS,
+                                          nbl_margin=0.4)
     out = {"metric": "NVE total-energy drift per 1000 steps (relative)",
-           "config": {"workload": "fcc EAM crystal 40^3 cells (%d atoms), "
-                                  "600 K, dt 1 fs, skin 0.4 A, NVE" % s.n,
+           "config": {"workload": DRIFT_CONFIGS[name], "name": name,
                       "steps": args.drift, "sample_every": 10}}
-    for prec, name in ((G.MIXED, "mixed"), (G.FP64, "fp64")):
-        ctx = G.eam_context(eam, [s], precision=prec, ensemble=G.NVE,
-                            timestep=W.FS, nbl_margin=0.4)
+    for prec, label in ((G.MIXED, "mixed"), (G.FP64, "fp64")):
+        ctx = make(prec)
         e, done, t0 = [], 0, time.perf_counter()
         while done < args.drift:
             n = min(1000, args.drift - done)
@@ -198,41 +227,122 @@ def drift_run(args):
         ctx.close()
         t = np.array([v[0] for v in e], dtype=np.float64)
         y = np.array([v[1] for v in e], dtype=np.float64)
-        slope = np.polyfit(t, y, 1)[0]
-        out[name] = {"drift_per_1000_steps": float(slope * 1000 / abs(y.mean())),
-                     "etot_per_atom_mean": float(y.mean()),
-                     "etot_per_atom_span": float(y.max() - y.min()),
-                     "atom_steps_per_s": s.n * args.drift / el}
+        slope, icept = np.polyfit(t, y, 1)
+        resid = y - (slope * t + icept)
+        out[label] = {"drift_per_1000_steps": float(slope * 1000 / abs(y.mean())),
+                      "etot_per_atom_mean": float(y.mean()),
+                      "etot_per_atom_span": float(y.max() - y.min()),
+                      "etot_resid_rms": float(np.sqrt(np.mean(resid ** 2))),
+                      "atom_steps_per_s": s.n * args.drift / el}
+    ref = os.path.join(ROOT, "tests", "golden", "drift_oracle.json")
+    if name != "eam256k" and os.path.exists(ref):
+        out["oracle"] = json.load(open(ref)).get(name)
     print(json.dumps(out))
     return 0
 
 
-def precision_errors(G, eam, s):
-    """Per-quantity error of the mixed path against the FP64 path on the
-    same configuration (one force evaluation each): RMS relative error and
-    max absolute error over atoms for forces, rho, 2F'(rho) and per-atom
-    energy; relative error of the total energy and of the virial (the
-    configurational part of the pressure)."""
+def state_errors(G, eam, s):
+    """Mixed against FP64 on one state, each quantity from the GPU: forces,
+    rho, 2F'(rho) and the per-atom energy from one force evaluation with the
+    full tables; F(rho) per atom from an evaluation without the pair table
+    (the per-atom energy is then the embedding term alone); the pressure per
+    atom (trace of the per-atom stress tensor / 3, energy units) and the
+    total pressure P = sum of the traces / (3 V), after one NVE step from
+    the state (the stress is computed after a step)."""
     res = {}
+    vol = abs(np.linalg.det(np.asarray(s.box, dtype=np.float64)))
     for prec in (G.FP64, G.MIXED):
         ctx = G.eam_context(eam, [s], precision=prec, ensemble=G.NVE,
                             timestep=W.FS, nbl_margin=0.4)
         ctx.calc_forces()
-        res[prec] = (ctx.atoms(), ctx.thermo())
+        a, th = ctx.atoms(), ctx.thermo()
+        ctx.set_stress(True)
+        ctx.move_atoms()
+        per, tot = ctx.stress()
         ctx.close()
-    (a64, t64), (a32, t32) = res[G.FP64], res[G.MIXED]
+        emb = G.Context(precision=prec, ensemble=G.NVE, ntypes=1,
+                        timestep=W.FS, nbl_margin=0.4)
+        emb.set_tables(None, G.TableArrays.from_format2(eam.rho),
+                       G.TableArrays.from_format2(eam.embed))
+        emb.set_systems([s])
+        emb.calc_forces()
+        femb = emb.atoms()["epot"]
+        emb.close()
+        res[prec] = dict(f=a["f"], rho=a["rho"], df=a["df"], epot=a["epot"],
+                         femb=femb, press=per[:, :3].sum(axis=1) / 3.0,
+                         ptot=tot[0, :3].sum() / (3.0 * vol),
+                         virial=th["virial"], etot=th["epot"])
+    r64, r32 = res[G.FP64], res[G.MIXED]
     out = {}
-    for key, name in (("f", "force"), ("rho", "rho"), ("df", "2F'(rho)"),
-                      ("epot", "energy per atom")):
-        ref = np.asarray(a64[key], dtype=np.float64)
-        d = np.asarray(a32[key], dtype=np.float64) - ref
+    for key, name in (("f", "force"), ("rho", "rho"), ("femb", "F(rho)"),
+                      ("df", "2F'(rho)"), ("epot", "energy per atom"),
+                      ("press", "pressure per atom")):
+        ref = np.asarray(r64[key], dtype=np.float64)
+        d = np.asarray(r32[key], dtype=np.float64) - ref
         scale = np.sqrt(np.mean(ref * ref))
         out[name] = {"rms_rel": float(np.sqrt(np.mean(d * d)) / scale),
                      "max_abs": float(np.max(np.abs(d))),
+                     "max_rel": float(np.max(np.abs(d)) / scale),
                      "rms_ref": float(scale)}
-    out["total energy"] = {"rel": abs(t32["epot"] - t64["epot"]) / abs(t64["epot"])}
-    out["virial"] = {"rel": abs(t32["virial"] - t64["virial"]) / abs(t64["virial"])}
+    out["total energy"] = {"rel": abs(r32["etot"] - r64["etot"]) / abs(r64["etot"])}
+    out["virial"] = {"rel": abs(r32["virial"] - r64["virial"]) / abs(r64["virial"])}
+    out["pressure"] = {"rel": abs(r32["ptot"] - r64["ptot"]) / abs(r64["ptot"]),
+                       "value": r64["ptot"]}
     return out
+
+
+def precision_errors(G, eam, s):
+    """Per-quantity error of the mixed path against the FP64 path on the
+    initial configuration (state_errors)."""
+    return state_errors(G, eam, s)
+
+
+def precision_study(args):
+    """Per-quantity error of the mixed path against the FP64 path along the
+    2M-atom NVT run: the mixed run advances args.precision_study steps, and
+    every args.sample_every steps its state (positions, momenta) is
+    evaluated by both precisions (state_errors).  Prints one JSON line with
+    every sample and the largest value of each error over the samples."""
+    import dataclasses
+    from imd_amd import imdg as G
+    eam, s = make_workload(args.cells, args.seed, args.temp)
+    isq = 1.0 / (args.tau_fs * W.FS) ** 2
+    ctx = G.eam_context(eam, [s], precision=G.MIXED, ensemble=G.NVT,
+                        timestep=W.FS, nbl_margin=0.4)
+    ctx.set_nvt(0, W.kT(args.temp), 0.0, isq)
+    samples, done = [], 0
+    while True:
+        a = ctx.atoms()
+        st = dataclasses.replace(s, x=a["x"], p=a["p"])
+        err = state_errors(G, eam, st)
+        samples.append({"step": done, "errors": err})
+        print(json.dumps({"sample": done, "force_rms_rel":
+                          err["force"]["rms_rel"]}), file=sys.stderr,
+              flush=True)
+        if done >= args.precision_study:
+            break
+        n = min(args.sample_every, args.precision_study - done)
+        ctx.run(n)
+        done += n
+    ctx.close()
+    worst = {}
+    for smp in samples:
+        for q, e in smp["errors"].items():
+            w = worst.setdefault(q, {})
+            for k, v in e.items():
+                if k in ("rms_ref", "value"):
+                    continue
+                w[k] = max(w.get(k, 0.0), v)
+    print(json.dumps({"metric": "mixed vs FP64 per-quantity error along the "
+                                "NVT run",
+                      "config": {"workload": "fcc EAM crystal %d^3 cells (%d "
+                                             "atoms), NVT %g K, dt 1 fs"
+                                             % (args.cells, s.n, args.temp),
+                                 "steps": args.precision_study,
+                                 "sample_every": args.sample_every},
+                      "max_over_samples": worst, "samples": samples}))
+    return 0
+
 
 
 CONFIGS = {
@@ -369,6 +479,12 @@ def main(argv=None):
     ap.add_argument("--drift", type=int, default=0, metavar="STEPS",
                     help="only measure the NVE energy drift over STEPS steps "
                          "(both precisions) and exit")
+    ap.add_argument("--drift-config", choices=sorted(DRIFT_CONFIGS),
+                    default="eam256k", help="input of the --drift run")
+    ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
+                    help="only sample the mixed-vs-FP64 errors along STEPS "
+                         "steps of the NVT run and exit")
+    ap.add_argument("--sample-every", type=int, default=1000)
     ap.add_argument("--config", choices=sorted(CONFIGS), default=None,
                     help="measure one of the other configurations instead "
                          "of the 2M-atom metric workload")
@@ -386,6 +502,8 @@ def main(argv=None):
         return copy_probe(args.copy_probe)
     if args.drift:
         return drift_run(args)
+    if args.precision_study:
+        return precision_study(args)
     if args.config:
         return config_run(args)
 
@@ -454,14 +572,19 @@ def main(argv=None):
     if kt and kt.get("force", (0, 0))[1] > 0:
         ms, n = kt["force"]
         avg_s = ms / n / 1e3
-        byts = algorithmic_bytes("force", k_mean, mixed) * natoms
+        byts = algorithmic_bytes("fused", k_mean, mixed) * natoms
         ach = byts / avg_s / 1e9
+        step_ach = value * algorithmic_bytes("step", k_mean, mixed) / 1e9
         roofline = {"bound": "hbm", "achieved": round(ach, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                     "kernel": "k_force (pair+EAM force, leapfrog, skin test)",
                     "avg_launch_us": round(avg_s * 1e6, 2),
-                    "bytes_per_launch": int(byts)}
+                    "bytes_per_launch": int(byts),
+                    "step_achieved": round(step_ach, 1),
+                    "step_frac": round(step_ach / HBM_PEAK_GBS, 4),
+                    "step_bytes_per_atom": round(
+                        algorithmic_bytes("step", k_mean, mixed), 1)}
         if "density" in kt and kt["density"][1] > 0:
             dms, dn = kt["density"]
             roofline["density_avg_launch_us"] = round(dms / dn * 1e3, 2)

@@ -120,6 +120,8 @@ struct imdg_ctx {
   ThermoRec* d_ring = nullptr;
   void* d_tabs = nullptr;        // TabDev<R>[4]
   void* d_knots = nullptr;       // knots of all tables
+  void* d_knots64 = nullptr;     // FP64 knots of embedding and EEAM term
+  void* d_tab64 = nullptr;       // Tab64[2] (FP32 step kernels)
   // launch plan of the persistent step kernels
   int ncu = 0;
   int nchunk = 0;
@@ -358,6 +360,12 @@ int upload_tables(imdg_ctx* c) {
       td[t].end[col] = (R)h.end[col];
       td[t].invstep[col] = (R)h.invstep[col];
       td[t].twoistep[col] = (R)(2 * h.invstep[col]);
+      {
+        const double is = h.invstep[col], c0 = -h.begin[col] * is;
+        td[t].ilo[col] = (R)(is - (double)(R)is);
+        td[t].t0[col] = (R)c0;
+        td[t].t0lo[col] = (R)(c0 - (double)(R)c0);
+      }
       if (c->cubic) {
         std::vector<double> seg;
         if (!cubic_segments(h, col, t < 2, c->cfg.interpolation, seg))
@@ -392,9 +400,11 @@ int upload_tables(imdg_ctx* c) {
   // dv = p1 - p0 and d2v = p2 - 2 p1 + p0 as PAIR_INT2 forms them
   // (src/potaccess.h:347-349), in FP64 and then rounded once.
   //   FP64: S[k] = (phi p0, phi dv, phi d2v, rho dv), then rho d2v[k].
-  //   FP32: Q[k] = (phi p0, phi A, phi B, phi D), then (rho A, rho D)[k],
+  //   FP32: Q[k] = (phi p0, phi A, phi B, phi A_lo), then (rho A, rho D)[k],
   //         with A = dv - d2v/2, B = d2v/2, D = d2v: value p0 + chi (A +
-  //         B chi) and gradient / (2 istep) A + D chi.  Segment len-1 is all
+  //         B chi) and gradient / (2 istep) A + D chi = A + B (2 chi)
+  //         (2B = D exactly in FP32); A_lo = dv - d2v/2 - A, the part of A
+  //         that FP32 drops, enters the virial sum.  Segment len-1 is all
   //         zero: the reference evaluates it only at r^2 >= end, where the
   //         shifted radial tables vanish, so a clamp of the table argument
   //         replaces the cutoff tests.
@@ -444,7 +454,7 @@ int upload_tables(imdg_ctx* c) {
           a.x = (R)at(hp, k);
           a.y = (R)(dp - 0.5 * ddp);
           b.x = (R)(0.5 * ddp);
-          b.y = (R)ddp;
+          b.y = (R)((dp - 0.5 * ddp) - (double)a.y);
           r.x = (R)(dr - 0.5 * ddr);
           r.y = (R)ddr;
         }
@@ -482,7 +492,7 @@ int upload_tables(imdg_ctx* c) {
   }
   // FP32, several species, every pair and rho column on one grid: per type
   // pair p = it*nt + jt and segment k a force record of 32 B, (phi p0, phi
-  // A, phi B, phi D) of pair column p and (A, D) of rho columns p (density
+  // A, phi B, phi A_lo) of pair column p and (A, D) of rho columns p (density
   // at an it-atom due to a jt-atom) and jt*nt + it, and a density record
   // (rho p0, A, B, 0) of rho column p; segment len-1 zero, as for one
   // species (the table argument is clamped instead of the cutoff tests)
@@ -530,7 +540,7 @@ int upload_tables(imdg_ctx* c) {
           seg(hp, it * nt + jt, k, ph);
           seg(hr, it * nt + jt, k, rc);
           seg(hr, jt * nt + it, k, rt);
-          push4(ph[0], ph[1], ph[2], ph[3]);
+          push4(ph[0], ph[1], ph[2], ph[1] - (double)(R)ph[1]);
           push4(rc[1], rc[3], rt[1], rt[3]);
         }
     c->knots_dpolyn = (int)kn.size();
@@ -540,6 +550,50 @@ int upload_tables(imdg_ctx* c) {
         seg(hr, p, k, rc);
         push4(rc[0], rc[1], rc[2], 0.0);
       }
+  }
+  // the FP32 step kernels evaluate the embedding function and the EEAM
+  // energy term (once per atom) in FP64: their knots or cubic segments again
+  // in FP64
+  if (sizeof(R) == 4) {
+    std::vector<double2> k64;
+    Tab64 t64[2];
+    memset(t64, 0, sizeof t64);
+    for (int q = 0; q < 2; q++) {
+      HostTable& h = c->tab[2 + q];
+      if (!h.present) continue;
+      for (int col = 0; col < h.ncols && col < IMDG_MAXTYPES; col++) {
+        t64[q].off[col] = (int)k64.size();
+        t64[q].len[col] = h.len[col];
+        t64[q].begin[col] = h.begin[col];
+        t64[q].end[col] = h.end[col];
+        t64[q].invstep[col] = h.invstep[col];
+        t64[q].twoistep[col] = 2 * h.invstep[col];
+        if (c->cubic) {
+          std::vector<double> seg;
+          if (!cubic_segments(h, col, false, c->cfg.interpolation, seg))
+            return fail(c, IMDG_ERR_NOTABLE,
+                        "column %d of table %d is too short for the "
+                        "interpolation", col, 2 + q);
+          for (size_t k = 0; k < seg.size(); k += 2)
+            k64.push_back(make_double2(seg[k], seg[k + 1]));
+          continue;
+        }
+        const int nc = h.ncols;
+        for (int k = 0; k <= h.len[col]; k++) {
+          double p0 = h.table[(size_t)k * nc + col];
+          double p1 = h.table[(size_t)(k + 1) * nc + col];
+          k64.push_back(make_double2(p0, p1 - p0));
+        }
+      }
+    }
+    int rc64 = dalloc(c, &c->d_knots64, k64.size() * sizeof(double2));
+    if (rc64) return rc64;
+    if (!k64.empty())
+      HIPCHK(c, hipMemcpy(c->d_knots64, k64.data(),
+                          k64.size() * sizeof(double2), hipMemcpyHostToDevice));
+    rc64 = dalloc(c, &c->d_tab64, sizeof t64);
+    if (rc64) return rc64;
+    HIPCHK(c, hipMemcpy(c->d_tab64, t64, sizeof t64, hipMemcpyHostToDevice));
   }
   int rc = dalloc(c, &c->d_knots, kn.size() * sizeof(R2));
   if (rc) return rc;
@@ -661,6 +715,8 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.nbr = c->d_nbr;
   a.nnb = c->d_nnb;
   a.knots = (const typename V2<R>::T*)c->d_knots;
+  a.knots64 = (const double2*)c->d_knots64;
+  a.tab64 = (const Tab64*)c->d_tab64;
   a.joint = c->knots_joint;
   a.dpoly = c->knots_dpoly;
   a.jointn = c->knots_jointn;

@@ -144,6 +144,26 @@ struct TabDev {
   int len[IMDG_MAXCOLS];  // usable segments (rows of the IMD column)
   R begin[IMDG_MAXCOLS], end[IMDG_MAXCOLS], invstep[IMDG_MAXCOLS];
   R twoistep[IMDG_MAXCOLS];
+  // the table argument t = (x - begin) istep as t0 + x istep with the
+  // constants split into R parts, istep = invstep + ilo and -begin istep =
+  // t0 + t0lo (ilo = t0lo = 0 in FP64): the FP32 loops take the segment
+  // from fma(x, invstep, t0) and add fma(x, ilo, t0lo) to its fraction, so
+  // the rounding of the two constants shifts no fraction (a shift common
+  // to every pair of a shell does not average out in the virial)
+  R ilo[IMDG_MAXCOLS], t0[IMDG_MAXCOLS], t0lo[IMDG_MAXCOLS];
+};
+
+// The embedding function (and the EEAM energy term) in FP64 for the FP32
+// step kernels: F and 2F' are evaluated once per atom, and an FP32
+// derivative would scale the EAM part of every pair's force by the
+// rounding of 2 istep, a common factor that the virial's cancellation of
+// pair and EAM parts magnifies.  Knots (or cubic segments) as for the
+// force-precision tables, in FP64.
+struct Tab64 {
+  int off[IMDG_MAXTYPES];
+  int len[IMDG_MAXTYPES];
+  double begin[IMDG_MAXTYPES], end[IMDG_MAXTYPES], invstep[IMDG_MAXTYPES];
+  double twoistep[IMDG_MAXTYPES];
 };
 
 // Column parameters as the NT > 1 neighbour loops read them from LDS.
@@ -218,6 +238,8 @@ struct StepArgs {
   int eeam;               // EEAM: second density P = sum rho^2, energy E(P)
   R* eeP;                 // EEAM P per slot (written with store)
   R* eedm;                // EEAM 2E'(P) per slot (read by the force loop)
+  const double2* knots64; // FP64 knots of the embedding (tab64[0]) and the
+  const Tab64* tab64;     // EEAM energy term (tab64[1]); FP32 kernels only
   int lpa_den, lpa_frc;   // lanes per slot of the density / force loops
   int thr_den, thr_frc;   // threads per block of the density / force loops
 };

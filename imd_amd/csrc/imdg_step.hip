@@ -76,19 +76,43 @@ __device__ __forceinline__ Sched make_sched(int nchunk, int xcd_remap) {
 }
 
 // Cell-offset vectors of a replica, in the force precision, into the
-// wavefront's own LDS row (one 4-vector per code: one LDS load per pair).
+// wavefront's own LDS row (one 4-vector per code: one LDS load per pair),
+// and twice what rounding to the force precision dropped from them
+// (olo).  Every pair with the same cell offset would otherwise see the same
+// error in its separation, a small uniform strain per direction that the
+// virial's cancellation of large terms magnifies; the FP32 one-grid loops
+// fold 2 d.olo into r^2 (sep_r2).
 template <typename R4>
-__device__ __forceinline__ void wave_offsets(const RepDev& Rp, R4* off) {
+__device__ __forceinline__ void wave_offsets(const RepDev& Rp, R4* off,
+                                             R4* olo) {
+  typedef decltype(off->x) R;
   int l = threadIdx.x & (IMDG_WAVE - 1);
   if (l < IMDG_NCODES) {
-    R4 o;
-    o.x = Rp.offs[l][0];
-    o.y = Rp.offs[l][1];
-    o.z = Rp.offs[l][2];
+    R4 o, q;
+    o.x = (R)Rp.offs[l][0];
+    o.y = (R)Rp.offs[l][1];
+    o.z = (R)Rp.offs[l][2];
     o.w = 0;
+    q.x = (R)(2.0 * (Rp.offs[l][0] - (double)o.x));
+    q.y = (R)(2.0 * (Rp.offs[l][1] - (double)o.y));
+    q.z = (R)(2.0 * (Rp.offs[l][2] - (double)o.z));
+    q.w = 0;
     off[l] = o;
+    olo[l] = q;
   }
   wave_lds_sync();
+}
+
+// r^2 of a separation d formed with the rounded cell offset, with 2 d.olo
+// (olo: twice the offset's rounding remainder) added inside the FMA chain,
+// where the partial sums still resolve it: the rounding of r^2 is then
+// unbiased over the pairs of a shell.
+__device__ __forceinline__ float sep_r2(float dx, float dy, float dz,
+                                        const float4& ol) {
+  float c = dx * ol.x;
+  c = fmaf(dy, ol.y, c);
+  c = fmaf(dz, ol.z, c);
+  return fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, c)));
 }
 
 // Wavefront reduction of the step partials (sum; max for PB_MAXD).
@@ -605,6 +629,33 @@ struct ForceCols {
   R rb, re, ris, r2is;
 };
 
+// Table argument of the FP32 segment tables (one grid): t = (r^2 - begin)
+// istep as fma(r^2, istep, t0) clamped to [0, len - 1/2] (med3), segment
+// k = (int)t, fraction chi = fract(t) + fma(r^2, ilo, t0lo): the low parts
+// of the split constants (TabDev::ilo, t0lo) go into the fraction, where
+// FP32 still resolves them, instead of shifting every pair of a shell the
+// same way.
+struct TArg {
+  float istep, ilo, t0, t0lo, tz;
+};
+
+template <typename R>
+__device__ __forceinline__ TArg targ(const TabDev<R>& t) {
+  TArg a;
+  a.istep = (float)t.invstep[0];
+  a.ilo = (float)t.ilo[0];
+  a.t0 = (float)t.t0[0];
+  a.t0lo = (float)t.t0lo[0];
+  a.tz = (float)t.len[0] - 0.5f;
+  return a;
+}
+
+__device__ __forceinline__ int tseg(float r2, const TArg& A, float& chi) {
+  float t = __builtin_amdgcn_fmed3f(fmaf(r2, A.istep, A.t0), 0.0f, A.tz);
+  chi = __builtin_amdgcn_fractf(t) + fmaf(r2, A.ilo, A.t0lo);
+  return (int)t;
+}
+
 // Four neighbours of the density loop: records and offsets, the prefetch,
 // r^2 and segments, all knot loads together, accumulation with mask
 // multiplies (no branches around the loads).
@@ -686,11 +737,12 @@ __device__ __forceinline__ void density_group(
 template <typename PF>
 __device__ __forceinline__ void density_group_poly(
     uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
-    const float4* off, const float4& ui, const float4* __restrict__ DQ,
-    float tc0, float istep, float tz, float& rho, float& r2min) {
+    const float4* off, const float4* olo, const float4& ui,
+    const float4* __restrict__ DQ, const TArg& ta, float& rho,
+    float& r2min) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
   float ux[4], uy[4], uz[4];
-  float4 o[4];
+  float4 o[4], ol[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     // the type is not needed: gather 12 B
@@ -700,6 +752,7 @@ __device__ __forceinline__ void density_group_poly(
     uy[u] = xy.y;
     uz[u] = p[2];
     o[u] = off[en[u] >> IMDG_IDX_BITS];
+    ol[u] = olo[en[u] >> IMDG_IDX_BITS];
   }
   __builtin_amdgcn_sched_barrier(0);
   slot = pf.load();
@@ -710,10 +763,8 @@ __device__ __forceinline__ void density_group_poly(
     float dx = (ux[u] - ui.x) + o[u].x;
     float dy = (uy[u] - ui.y) + o[u].y;
     float dz = (uz[u] - ui.z) + o[u].z;
-    r2[u] = dx * dx + dy * dy + dz * dz;
-    float t = __builtin_amdgcn_fmed3f(fmaf(r2[u], istep, tc0), 0.0f, tz);
-    kk[u] = (int)t;
-    chi[u] = __builtin_amdgcn_fractf(t);
+    r2[u] = sep_r2(dx, dy, dz, ol[u]);
+    kk[u] = tseg(r2[u], ta, chi[u]);
   }
   float4 q[4];
 #pragma unroll
@@ -920,24 +971,27 @@ __device__ __forceinline__ void force_group_sg(
 }
 
 // FP32 variant of force_group_sg on the polynomial segment table: Q[k] =
-// (phi p0, phi A, phi B, phi D), Rr[k] = (rho A, rho D); value p0 + chi (A +
-// B chi), gradients / (2 istep) A + D chi.  The table argument t = (r^2 -
-// begin) istep is clamped to [0, len - 1/2] (med3): r^2 below begin reads
-// segment 0 at chi = 0 like the reference, r^2 at or beyond end reads the
-// zero segment len - 1, so no cutoff test is needed.
+// (phi p0, phi A, phi B, phi A_lo), Rr[k] = (rho A, rho D); value p0 + chi
+// (A + B chi), gradients / (2 istep) A + B (2 chi) (pair; 2B is phi's D
+// exactly) and A + D chi (rho).  The table argument is clamped to [0, len -
+// 1/2] (tseg): r^2 below begin reads segment 0 at chi = 0 like the
+// reference, r^2 at or beyond end reads the zero segment len - 1, so no
+// cutoff test is needed.  svlo sums r^2 A_lo, the part of the pair
+// gradient that FP32 drops from A, for the virial.
 template <typename PF>
 __device__ __forceinline__ void force_group_sgf(
     uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
-    const float4* off, const float4& ui, float hFi,
-    const float4* __restrict__ Q, const float2* __restrict__ Rr, float tc0,
-    float istep, float tz, float& sx, float& sy, float& sz, float& ep,
-    float& svir, float& r2min) {
+    const float4* off, const float4* olo, const float4& ui, float hFi,
+    const float4* __restrict__ Q, const float2* __restrict__ Rr,
+    const TArg& ta, float& sx, float& sy, float& sz, float& ep, float& svir,
+    float& svlo) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
-  float4 uj[4], o[4];
+  float4 uj[4], o[4], ol[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     uj[u] = rec[en[u] & IMDG_IDX_MASK];
     o[u] = off[en[u] >> IMDG_IDX_BITS];
+    ol[u] = olo[en[u] >> IMDG_IDX_BITS];
   }
   __builtin_amdgcn_sched_barrier(0);
   slot = pf.load();
@@ -948,10 +1002,8 @@ __device__ __forceinline__ void force_group_sgf(
     dx[u] = (uj[u].x - ui.x) + o[u].x;
     dy[u] = (uj[u].y - ui.y) + o[u].y;
     dz[u] = (uj[u].z - ui.z) + o[u].z;
-    r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-    float t = __builtin_amdgcn_fmed3f(fmaf(r2[u], istep, tc0), 0.0f, tz);
-    kk[u] = (int)t;
-    chi[u] = __builtin_amdgcn_fractf(t);
+    r2[u] = sep_r2(dx[u], dy[u], dz[u], ol[u]);
+    kk[u] = tseg(r2[u], ta, chi[u]);
   }
   float4 q[4];
   float2 rr[4];
@@ -962,7 +1014,7 @@ __device__ __forceinline__ void force_group_sgf(
   }
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    float gp = fmaf(chi[u], q[u].w, q[u].y);
+    float gp = fmaf(chi[u] + chi[u], q[u].z, q[u].y);
     float gr = fmaf(chi[u], rr[u].y, rr[u].x);
     float vp = fmaf(chi[u], fmaf(chi[u], q[u].z, q[u].y), q[u].x);
     float h = fmaf(0.5f, uj[u].w, hFi);
@@ -972,30 +1024,32 @@ __device__ __forceinline__ void force_group_sgf(
     sy = fmaf(dy[u], g, sy);
     sz = fmaf(dz[u], g, sz);
     svir = fmaf(r2[u], g, svir);
-    r2min = fminf(r2min, r2[u]);
+    svlo = fmaf(r2[u], q[u].w, svlo);
   }
 }
 
 // Several species on one grid (FP32): the record pair S[2 (p len + k)] of
-// type pair p = it*nt + jt is (phi p0, phi A, phi B, phi D) of pair column p
-// and (A, D) of rho columns p and jt*nt + it; gradients in units of
-// 1/(2 istep) as in force_group_sgf.  rho'_j (column p) goes with 2F'_i,
+// type pair p = it*nt + jt is (phi p0, phi A, phi B, phi A_lo) of pair
+// column p and (A, D) of rho columns p and jt*nt + it; gradients in units
+// of 1/(2 istep) and the virial's A_lo sum as in force_group_sgf.  rho'_j (column p) goes with 2F'_i,
 // rho'_i (column jt*nt + it) with 2F'_j (src/imd_forces_nbl.c:1156-1203).
 template <typename PF>
 __device__ __forceinline__ void force_group_sgn(
     uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
     const signed char* __restrict__ atype, const float4* off,
-    const float4& ui, float hFi, const float4* __restrict__ S, int pbase,
-    int len, float tc0, float istep, float tz, float& sx, float& sy,
-    float& sz, float& ep, float& svir, float& r2min) {
+    const float4* olo, const float4& ui, float hFi,
+    const float4* __restrict__ S, int pbase, int len, const TArg& ta,
+    float& sx, float& sy, float& sz, float& ep, float& svir, float& svlo,
+    float& r2min) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
-  float4 uj[4], o[4];
+  float4 uj[4], o[4], ol[4];
   int jt[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     uj[u] = rec[en[u] & IMDG_IDX_MASK];
     jt[u] = atype[en[u] & IMDG_IDX_MASK];
     o[u] = off[en[u] >> IMDG_IDX_BITS];
+    ol[u] = olo[en[u] >> IMDG_IDX_BITS];
   }
   __builtin_amdgcn_sched_barrier(0);
   slot = pf.load();
@@ -1006,10 +1060,8 @@ __device__ __forceinline__ void force_group_sgn(
     dx[u] = (uj[u].x - ui.x) + o[u].x;
     dy[u] = (uj[u].y - ui.y) + o[u].y;
     dz[u] = (uj[u].z - ui.z) + o[u].z;
-    r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-    float t = __builtin_amdgcn_fmed3f(fmaf(r2[u], istep, tc0), 0.0f, tz);
-    kk[u] = 2 * ((pbase + jt[u]) * len + (int)t);
-    chi[u] = __builtin_amdgcn_fractf(t);
+    r2[u] = sep_r2(dx[u], dy[u], dz[u], ol[u]);
+    kk[u] = 2 * ((pbase + jt[u]) * len + tseg(r2[u], ta, chi[u]));
   }
   float4 q[4], rr[4];
 #pragma unroll
@@ -1019,7 +1071,7 @@ __device__ __forceinline__ void force_group_sgn(
   }
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    float gp = fmaf(chi[u], q[u].w, q[u].y);
+    float gp = fmaf(chi[u] + chi[u], q[u].z, q[u].y);
     float grj = fmaf(chi[u], rr[u].y, rr[u].x);
     float gri = fmaf(chi[u], rr[u].w, rr[u].z);
     float vp = fmaf(chi[u], fmaf(chi[u], q[u].z, q[u].y), q[u].x);
@@ -1029,6 +1081,7 @@ __device__ __forceinline__ void force_group_sgn(
     sy = fmaf(dy[u], g, sy);
     sz = fmaf(dz[u], g, sz);
     svir = fmaf(r2[u], g, svir);
+    svlo = fmaf(r2[u], q[u].w, svlo);
     r2min = fminf(r2min, r2[u]);
   }
 }
@@ -1038,15 +1091,16 @@ __device__ __forceinline__ void force_group_sgn(
 template <typename PF>
 __device__ __forceinline__ void density_group_polyn(
     uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
-    const float4* off, const float4& ui, const float4* __restrict__ DQ,
-    int pbase, int len, float tc0, float istep, float tz, float& rho,
-    float& r2min) {
+    const float4* off, const float4* olo, const float4& ui,
+    const float4* __restrict__ DQ, int pbase, int len, const TArg& ta,
+    float& rho, float& r2min) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
-  float4 uj[4], o[4];
+  float4 uj[4], o[4], ol[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     uj[u] = rec[en[u] & IMDG_IDX_MASK];
     o[u] = off[en[u] >> IMDG_IDX_BITS];
+    ol[u] = olo[en[u] >> IMDG_IDX_BITS];
   }
   __builtin_amdgcn_sched_barrier(0);
   slot = pf.load();
@@ -1057,10 +1111,8 @@ __device__ __forceinline__ void density_group_polyn(
     float dx = (uj[u].x - ui.x) + o[u].x;
     float dy = (uj[u].y - ui.y) + o[u].y;
     float dz = (uj[u].z - ui.z) + o[u].z;
-    r2[u] = dx * dx + dy * dy + dz * dz;
-    float t = __builtin_amdgcn_fmed3f(fmaf(r2[u], istep, tc0), 0.0f, tz);
-    kk[u] = (pbase + (int)uj[u].w) * len + (int)t;
-    chi[u] = __builtin_amdgcn_fractf(t);
+    r2[u] = sep_r2(dx, dy, dz, ol[u]);
+    kk[u] = (pbase + (int)uj[u].w) * len + tseg(r2[u], ta, chi[u]);
   }
   float4 q[4];
 #pragma unroll
@@ -1088,6 +1140,7 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
   typedef typename V2<R>::T R2;
   extern __shared__ __align__(16) unsigned char dyn_lds[];
   __shared__ R4 woff[IMDG_STEP_WAVES][IMDG_NCODES];
+  __shared__ R4 wolo[IMDG_STEP_WAVES][IMDG_NCODES];
   __shared__ ColPar<R> cpr[IMDG_MAXCOLS];
   if (a.ctl->frozen) return;
   const TabDev<R>& tr = tabs[1];
@@ -1107,13 +1160,14 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
   const int lane = threadIdx.x & (IMDG_WAVE - 1);
   const int ga = lane / L, gs = lane % L;
   R4* off = woff[threadIdx.x / IMDG_WAVE];
+  R4* olo = wolo[threadIdx.x / IMDG_WAVE];
   int cur = -1, is_short = 0;
   double acc = 0.0;
   for (int c = S.c; c < S.c1; c += S.stride) {
     int r = a.blk_rep[c / (IMDG_BLOCK / IMDG_CHUNK)];
     const RepDev& Rp = a.rep[r];
     if (r != cur) {
-      wave_offsets(Rp, off);
+      wave_offsets(Rp, off, olo);
       cur = r;
     }
     R rho_keep = (R)0, pee_keep = (R)0;
@@ -1130,11 +1184,11 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
         if constexpr (sizeof(R) == 4 && NT1 && !CUB && !EE) {
           const float4* __restrict__ DQ =
               reinterpret_cast<const float4*>(K + (a.dpoly - klo));
-          const float tc0 = -b0 * is0, tz = (float)len0 - 0.5f;
+          const TArg ta = targ(tr);
           list_walk<L>(nw, a.nnb[i],
                        [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                         density_group_poly(e4, pf, slot, rec, off, ui, DQ,
-                                            tc0, is0, tz, rho, r2min);
+                         density_group_poly(e4, pf, slot, rec, off, olo, ui,
+                                            DQ, ta, rho, r2min);
                        });
         } else {
           auto generic = [&]() {
@@ -1150,12 +1204,12 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
               // several species on one grid: segment records in HBM/L2
               const float4* __restrict__ DQ =
                   reinterpret_cast<const float4*>(a.knots + a.dpolyn);
-              const float tc0 = -b0 * is0, tz = (float)len0 - 0.5f;
+              const TArg ta = targ(tr);
               list_walk<L>(nw, a.nnb[i],
                            [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                             density_group_polyn(e4, pf, slot, rec, off, ui,
-                                                 DQ, it * nt, len0, tc0, is0,
-                                                 tz, rho, r2min);
+                             density_group_polyn(e4, pf, slot, rec, off, olo,
+                                                 ui, DQ, it * nt, len0, ta,
+                                                 rho, r2min);
                            });
               is_short |= (r2min < b0);
             } else {
@@ -1183,27 +1237,57 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
       int it = NT1 ? 0 : (int)ui.w;
       R rho = rho_keep;
       // embedding: PAIR_INT on the embedding column of the type, argument
-      // rho (knots in HBM; one lookup per atom)
-      const R2* __restrict__ ke = a.knots + te.off[it];
-      R chi;
-      int k = tab_knot<CUB>(seg_locate(rho, te.begin[it], te.end[it],
-                                       te.invstep[it], te.len[it], chi));
-      R2 p = ke[k], q = ke[k + 1];
-      R F = tab_val<CUB, R>(p, q, chi);
-      R dF = tab_grad<CUB, R>(p, q, chi, te.twoistep[it]);
+      // rho (knots in HBM; one lookup per atom), in FP64 (the FP32 kernels
+      // read FP64 knots of their own)
+      double Fd, dFd;
+      if constexpr (sizeof(R) == 4) {
+        const Tab64& E = a.tab64[0];
+        const double2* __restrict__ ke = a.knots64 + E.off[it];
+        double chi;
+        int k = tab_knot<CUB>(seg_locate((double)rho, E.begin[it], E.end[it],
+                                         E.invstep[it], E.len[it], chi));
+        double2 p = ke[k], q = ke[k + 1];
+        Fd = tab_val<CUB, double>(p, q, chi);
+        dFd = tab_grad<CUB, double>(p, q, chi, E.twoistep[it]);
+      } else {
+        const R2* __restrict__ ke = a.knots + te.off[it];
+        R chi;
+        int k = tab_knot<CUB>(seg_locate(rho, te.begin[it], te.end[it],
+                                         te.invstep[it], te.len[it], chi));
+        R2 p = ke[k], q = ke[k + 1];
+        Fd = tab_val<CUB, R>(p, q, chi);
+        dFd = tab_grad<CUB, R>(p, q, chi, te.twoistep[it]);
+      }
       if (EE) {
         // EEAM energy term E(P) and EAM_DM = 2E'(P)
         // (src/imd_forces_nbl.c:1090-1095)
-        const TabDev<R>& tm = tabs[3];
-        const R2* __restrict__ km = a.knots + tm.off[it];
-        R chm;
-        int kq = tab_knot<CUB>(seg_locate(pee_keep, tm.begin[it], tm.end[it],
-                                          tm.invstep[it], tm.len[it], chm));
-        R2 pm = km[kq], qm = km[kq + 1];
-        F += tab_val<CUB, R>(pm, qm, chm);
-        a.eedm[i] = tab_grad<CUB, R>(pm, qm, chm, tm.twoistep[it]);
+        double Em, dEm;
+        if constexpr (sizeof(R) == 4) {
+          const Tab64& E = a.tab64[1];
+          const double2* __restrict__ km = a.knots64 + E.off[it];
+          double chm;
+          int kq = tab_knot<CUB>(seg_locate((double)pee_keep, E.begin[it],
+                                            E.end[it], E.invstep[it],
+                                            E.len[it], chm));
+          double2 pm = km[kq], qm = km[kq + 1];
+          Em = tab_val<CUB, double>(pm, qm, chm);
+          dEm = tab_grad<CUB, double>(pm, qm, chm, E.twoistep[it]);
+        } else {
+          const TabDev<R>& tm = tabs[3];
+          const R2* __restrict__ km = a.knots + tm.off[it];
+          R chm;
+          int kq = tab_knot<CUB>(seg_locate(pee_keep, tm.begin[it],
+                                            tm.end[it], tm.invstep[it],
+                                            tm.len[it], chm));
+          R2 pm = km[kq], qm = km[kq + 1];
+          Em = tab_val<CUB, R>(pm, qm, chm);
+          dEm = tab_grad<CUB, R>(pm, qm, chm, tm.twoistep[it]);
+        }
+        Fd += Em;
+        a.eedm[i] = (R)dEm;
         if (a.store) a.eeP[i] = pee_keep;
       }
+      const R F = (R)Fd, dF = (R)dFd;
       R4 rb;
       rb.x = ui.x;
       rb.y = ui.y;
@@ -1214,7 +1298,7 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
         a.rho[i] = rho;
         a.epot[i] = F;
       }
-      emb = (double)F;
+      emb = Fd;
     }
     if (a.part_chunk) {
       double s = wave_sum(emb);
@@ -1248,6 +1332,7 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
   typedef typename V2<R>::T R2;
   extern __shared__ __align__(16) unsigned char dyn_lds[];
   __shared__ R4 woff[IMDG_STEP_WAVES][IMDG_NCODES];
+  __shared__ R4 wolo[IMDG_STEP_WAVES][IMDG_NCODES];
   __shared__ ColPar<R> cpp[IMDG_MAXCOLS], cpr[IMDG_MAXCOLS];
   if (a.ctl->frozen) return;
   const TabDev<R>& tp = tabs[0];
@@ -1277,21 +1362,23 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
   const int lane = threadIdx.x & (IMDG_WAVE - 1);
   const int ga = lane / L, gs = lane % L;
   R4* off = woff[threadIdx.x / IMDG_WAVE];
+  R4* olo = wolo[threadIdx.x / IMDG_WAVE];
   int cur = -1, is_short = 0;
   double acc[PB_N] = {0, 0, 0, 0, 0, 0};
   for (int c = S.c; c < S.c1; c += S.stride) {
     int r = a.blk_rep[c / (IMDG_BLOCK / IMDG_CHUNK)];
     const RepDev& Rp = a.rep[r];
     if (r != cur) {
-      wave_offsets(Rp, off);
+      wave_offsets(Rp, off, olo);
       cur = r;
     }
-    // the slot's sums kept by this lane (pass gs)
-    R kx = 0, ky = 0, kz = 0, kep = 0, kvir = 0;
+    // the slot's sums kept by this lane (pass gs); vlo: the virial's sum
+    // of the pair gradients' FP32 remainders (FP32 one-grid loops)
+    R kx = 0, ky = 0, kz = 0, kep = 0, kvir = 0, kvlo = 0;
 #pragma unroll 1
     for (int p = 0; p < L; p++) {
       int i = c * IMDG_CHUNK + p * (IMDG_WAVE / L) + ga;
-      R fx = 0, fy = 0, fz = 0, ep = 0, vir = 0;
+      R fx = 0, fy = 0, fz = 0, ep = 0, vir = 0, vlo = 0;
       if (i - Rp.atom_off < Rp.natoms) {
         R4 ui = rec[i];
         int it = NT1 ? 0 : (int)atype[i];
@@ -1302,12 +1389,12 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
           R r2min = (R)3.0e38;
           const float4* __restrict__ S =
               reinterpret_cast<const float4*>(a.knots + a.jointn);
-          const float tc0 = -pb * pis, tz = (float)plen - 0.5f;
+          const TArg ta = targ(tp);
           list_walk<L>(nw, a.nnb[i],
                        [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                         force_group_sgn(e4, pf, slot, rec, atype, off, ui,
-                                         0.5f * dFi, S, it * nt, plen, tc0,
-                                         pis, tz, fx, fy, fz, ep, vir, r2min);
+                         force_group_sgn(e4, pf, slot, rec, atype, off, olo,
+                                         ui, 0.5f * dFi, S, it * nt, plen, ta,
+                                         fx, fy, fz, ep, vir, vlo, r2min);
                        });
           is_short |= (r2min < pb);
         } else if constexpr (MODE == FM_SG && !CUB && !EE) {
@@ -1318,12 +1405,12 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
                 reinterpret_cast<const float4*>(K + (a.joint - klo));
             const float2* __restrict__ Rr =
                 reinterpret_cast<const float2*>(K + (a.joint - klo) + 2 * plen);
-            const float tc0 = -pb * pis, tz = (float)plen - 0.5f;
+            const TArg ta = targ(tp);
             list_walk<L>(nw, a.nnb[i],
                          [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                           force_group_sgf(e4, pf, slot, rec, off, ui,
-                                           0.5f * dFi, Q, Rr, tc0, pis, tz,
-                                           fx, fy, fz, ep, vir, r2min);
+                           force_group_sgf(e4, pf, slot, rec, off, olo, ui,
+                                           0.5f * dFi, Q, Rr, ta, fx, fy, fz,
+                                           ep, vir, vlo);
                          });
           } else {
             const R4* __restrict__ S =
@@ -1357,25 +1444,31 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
       fz = lane_group_sum<L>(fz);
       ep = lane_group_sum<L>(ep);
       vir = lane_group_sum<L>(vir);
+      if (sizeof(R) == 4) vlo = lane_group_sum<L>(vlo);
       if (gs == p) {
         kx = fx;
         ky = fy;
         kz = fz;
         kep = ep;
         kvir = vir;
+        kvlo = vlo;
       }
     }
-    if ((MODE == FM_SG || (MODE == FM_SGN && sizeof(R) == 4)) && !CUB && !EE) {
+    // the one-grid loops sum in units of 1/(2 istep); the virial is scaled
+    // in FP64 with its remainder sum added
+    constexpr bool ONEGRID =
+        (MODE == FM_SG || (MODE == FM_SGN && sizeof(R) == 4)) && !CUB && !EE;
+    if (ONEGRID) {
       kx *= p2is;
       ky *= p2is;
       kz *= p2is;
-      kvir *= p2is;
     }
     kx = to_slot_lane<L>(kx, lane);
     ky = to_slot_lane<L>(ky, lane);
     kz = to_slot_lane<L>(kz, lane);
     kep = to_slot_lane<L>(kep, lane);
     kvir = to_slot_lane<L>(kvir, lane);
+    if (sizeof(R) == 4) kvlo = to_slot_lane<L>(kvlo, lane);
     int i = c * IMDG_CHUNK + lane;
     double v[PB_N] = {0, 0, 0, 0, 0, 0};
     if (i - Rp.atom_off < Rp.natoms) {
@@ -1391,7 +1484,9 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
         a.frc[i] = fv;
       }
       v[PB_EPAIR] = 0.5 * (double)kep;
-      v[PB_VIRIAL] = -0.5 * (double)kvir;
+      v[PB_VIRIAL] = ONEGRID ? -0.5 * ((double)kvir + (double)kvlo) *
+                                   (double)p2is
+                             : -0.5 * (double)kvir;
       if (INTEGRATE)
         integrate_slot(a, Rp, r, i, (double)kx, (double)ky, (double)kz, v);
     }

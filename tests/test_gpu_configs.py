@@ -106,7 +106,10 @@ def _oracle(s, eam, **kw):
     return orc
 
 
-def _traj_bars(precision, hist, th, ga, oa, box):
+def _traj_bars(precision, hist, th, ga, oa, box, p_bar=1e-5):
+    """FP64: the oracle's trajectory to rounding.  Mixed: energies to 1e-6
+    and momenta to p_bar (the FP32 force error, ~1e-6 relative, separates
+    the trajectories slowly: 1e-5 over tens of steps, more over 1,000)."""
     if precision == G.FP64:
         d = ga["x"] - oa["x"]
         L = np.diag(box)
@@ -118,7 +121,7 @@ def _traj_bars(precision, hist, th, ga, oa, box):
     else:
         assert np.allclose(hist[:, 1], th[:, 0], rtol=1e-6, atol=0)
         assert np.allclose(hist[:, 2], th[:, 1], rtol=1e-6, atol=0)
-        assert rel_rms(ga["p"], oa["p"]) <= 1e-5
+        assert rel_rms(ga["p"], oa["p"]) <= p_bar
     assert np.array_equal(hist[:, 6] > 0, th[:, 4] > 0)
 
 
@@ -132,7 +135,7 @@ def test_lj_4000_atoms_1000_steps(precision):
         hist = np.concatenate([gpu.thermo_history(k, 250)[:, 0, :]
                                for k in range(0, 1000, 250)])
         ga = gpu.atoms()
-    _traj_bars(precision, hist, th, ga, orc.atoms(), s.box)
+    _traj_bars(precision, hist, th, ga, orc.atoms(), s.box, p_bar=1e-4)
 
 
 @pytest.mark.parametrize("precision", [G.FP64, G.MIXED])
@@ -212,7 +215,12 @@ def test_64_replicas_static_and_20_graph_steps(precision, monkeypatch):
             total += len(mine)
             tol_f, tol_s = BARS[precision]
             oa = orc.atoms()
-            assert rel_rms(ga_all["f"][sl], oa["f"]) <= tol_f
+            # the replicas start on the perfect lattice (vacancies aside),
+            # where the net forces cancel to ~1e-14 eV/A: forces are
+            # compared on the scale of a thermal force (0.1 eV/A) there
+            fscale = max(float(np.sqrt(np.mean(oa["f"] ** 2))), 0.1)
+            assert np.sqrt(np.mean((ga_all["f"][sl] - oa["f"]) ** 2)) <= \
+                tol_f * fscale
             assert rel_rms(ga_all["rho"][sl], oa["rho"]) <= tol_s
             assert rel_rms(ga_all["epot"][sl], oa["epot"]) <= tol_s
             th = gpu.thermo(r)
