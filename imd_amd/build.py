@@ -31,6 +31,7 @@ HIP_UNITS = [
     ("imdg_step.hip", ["-DIMDG_STEP_FP32"], "imdg_step_f32.o"),
     ("imdg_step.hip", ["-DIMDG_STEP_FP64"], "imdg_step_f64.o"),
     ("imdg_api.hip", [], "imdg_api.o"),
+    ("imdg_noalloc.hip", [], "imdg_noalloc.o"),
 ]
 HIP_SOURCES = sorted({u[0] for u in HIP_UNITS})
 HIPCC_FLAGS = [

@@ -43,6 +43,7 @@ template <typename R>
 hipError_t launch_stress(const StepArgs<R>&, const TabDev<R>*, double*,
                          double*, int, hipStream_t);
 void launch_build(const BuildArgs&, int, hipStream_t);
+const char* noalloc_message();
 void launch_check(const RepDev*, Ctl*, const int*, const double4*,
                   const double4*, const double4*, double*, int, hipStream_t);
 }  // namespace imdg
@@ -1161,7 +1162,11 @@ long imdg_struct_size(const char* name) {
 }
 
 const char* imdg_last_error(const imdg_ctx* ctx) {
-  return ctx ? ctx->err.c_str() : "null context";
+  if (ctx) return ctx->err.c_str();
+  // a context-less call (imdg_check_nblist_noalloc) keeps its message per
+  // host thread
+  const char* m = imdg::noalloc_message();
+  return (m && m[0]) ? m : "null context";
 }
 
 int imdg_ctx_create(const imdg_config* cfg, imdg_ctx** out) {

@@ -30,7 +30,7 @@ SYMBOLS = [
     "imdg_get_pairs", "imdg_get_grid", "imdg_get_info", "imdg_profile",
     "imdg_kernel_times", "imdg_struct_size", "imdg_get_tuning",
     "imdg_set_tuning", "imdg_autotune", "imdg_set_stress", "imdg_get_stress",
-    "imdg_set_eeam_table", "imdg_get_eeam",
+    "imdg_set_eeam_table", "imdg_get_eeam", "imdg_check_nblist_noalloc",
 ]
 
 

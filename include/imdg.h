@@ -277,6 +277,25 @@ int  imdg_set_tuning(imdg_ctx *ctx, const imdg_tuning *t);
    *chosen (may be NULL) receives the result. */
 int  imdg_autotune(imdg_ctx *ctx, int reps, imdg_tuning *chosen);
 
+/* check_nblist on caller-owned device arrays, allocating nothing.  No
+   counterpart in the reference (IMD's check_nblist works on its own cell
+   arrays): it is for callers that cannot allocate device memory, such as
+   code under HIP graph capture or a host loop that keeps its positions on
+   the device.  x and x_nbl are device pointers to n positions, 3 doubles
+   each (now and at the last list build); the kernels leave
+   max |x - x_nbl|^2 in *max_disp2 and IMD's decision max > (nbl_margin /
+   2)^2 in *invalid (device pointers; either may be NULL, not both), on
+   `stream` (a hipStream_t, NULL for the null stream), without host
+   synchronisation.  The reduction goes through a fixed device array of
+   65,536 doubles per GPU: n above 16,777,216 (65,536 blocks of 256 atoms)
+   is rejected with IMDG_ERR_ARG before anything is launched.  Calls that
+   overlap on one GPU share that array and race.  imdg_last_error(NULL)
+   gives the message of a failed call on the calling host thread. */
+int  imdg_check_nblist_noalloc(int device, const double *x,
+                               const double *x_nbl, long n,
+                               double nbl_margin, double *max_disp2,
+                               int *invalid, void *stream);
+
 /* sizeof() of a public struct by name ("imdg_config", "imdg_table",
    "imdg_replica", "imdg_thermo", "imdg_info", "imdg_tuning"), -1 for an
    unknown name: lets
