@@ -133,6 +133,8 @@ struct imdg_ctx {
   int knots_dpoly = -1;                // FP32 rho segment polynomials
   int knots_jointn = -1;               // FP32, several species, one grid:
   int knots_dpolyn = -1;               // force and rho segment records
+  int* d_cmeta = nullptr;              // chunk metadata, 16-bit lists
+  bool list16 = false;                 // lists in the 16-bit format
   int force_mode = 0;                  // 0 one species, 1 shared grid,
                                        // 2 several, 3 several on one grid
   int lpa_den = 0, lpa_frc = 0;        // lanes per slot of the two loops
@@ -717,6 +719,8 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.nnb = c->d_nnb;
   a.knots = (const typename V2<R>::T*)c->d_knots;
   a.knots64 = (const double2*)c->d_knots64;
+  a.cmeta = c->d_cmeta;
+  a.list16 = c->list16 ? 1 : 0;
   a.tab64 = (const Tab64*)c->d_tab64;
   a.joint = c->knots_joint;
   a.dpoly = c->knots_dpoly;
@@ -790,6 +794,8 @@ BuildArgs build_args(imdg_ctx* c) {
   b.nstile = c->nstile;
   b.kmax = c->kmax;
   b.tile_cap = c->tile_cap;
+  b.cmeta = c->d_cmeta;
+  b.list16 = c->list16 ? 1 : 0;
   return b;
 }
 
@@ -1404,6 +1410,16 @@ int imdg_set_atoms(imdg_ctx* c, int nrep, const imdg_replica* reps,
   c->units_cap = 0;  // partials are allocated below, for any plan
   if ((rc = c->mixed ? plan_steps<float>(c) : plan_steps<double>(c)))
     return rc;
+  // the 16-bit list format where only the one-grid FP32 loops read the
+  // list: their walks (and the stress kernel's) decode it
+  c->list16 = c->mixed && c->tab[1].present && !c->cubic && !c->eeam &&
+              (c->cfg.ntypes == 1
+                   ? (c->force_mode == 1 && c->knots_dpoly >= 0)
+                   : (c->force_mode == 3 && c->knots_dpolyn >= 0));
+  if (const char* e = getenv("IMDG_LIST16"))
+    if (atoi(e) == 0) c->list16 = false;
+  c->d_cmeta = nullptr;
+  if (c->list16) ALLOC(c->d_cmeta, sizeof(int) * IMDG_META * c->nchunk);
   {
     // room for any launch plan: at most 32 resident wavefronts per CU
     c->units_cap = std::max(c->units_max, 32 * c->ncu);
@@ -1919,6 +1935,13 @@ int imdg_get_pairs(imdg_ctx* c, long* out, long cap, long* npairs) {
   HIPCHK(c, hipMemcpy(hc.data(), c->d_acell, sizeof(int) * ns,
                       hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(hl.data(), c->d_nbr, c->nbr_bytes, hipMemcpyDeviceToHost));
+  std::vector<int> meta;
+  if (c->list16) {
+    meta.resize((size_t)IMDG_META * c->nchunk);
+    HIPCHK(c, hipMemcpy(meta.data(), c->d_cmeta, sizeof(int) * meta.size(),
+                        hipMemcpyDeviceToHost));
+  }
+  const uint16_t* hl16 = reinterpret_cast<const uint16_t*>(hl.data());
   long np = 0;
   for (int r = 0; r < c->nrep; r++) {
     const RepDev& R = c->rep[r];
@@ -1928,11 +1951,24 @@ int imdg_get_pairs(imdg_ctx* c, long* out, long cap, long* npairs) {
       int lc = hc[s] - R.cell_off;
       int ci[3] = {lc / (R.gdim[1] * R.gdim[2]), (lc / R.gdim[2]) % R.gdim[1],
                    lc % R.gdim[2]};
+      const int* cm = c->list16 ? &meta[(size_t)(s >> 6) * IMDG_META] : nullptr;
+      const bool w16 = cm && cm[IMDG_META_WIDE] == 0;
       for (int m = 0; m < hn[s]; m++) {
-        size_t at = (((size_t)(s >> 6) * (c->kmax >> 2) + (m >> 2)) * IMDG_WAVE +
-                     (s & 63)) * IMDG_GROUP + (m & 3);
-        uint32_t e = hl[at];
-        int idx = (int)(e & IMDG_IDX_MASK), code = (int)(e >> IMDG_IDX_BITS);
+        int idx, code;
+        if (w16) {
+          size_t at = (size_t)(s >> 6) * c->kmax * IMDG_WAVE * 2 +
+                      ((size_t)(m >> 2) * IMDG_WAVE + (s & 63)) * IMDG_GROUP +
+                      (m & 3);
+          uint32_t e = hl16[at];
+          code = (int)(e >> IMDG_K16_BITS);
+          idx = cm[code] + (int)(e & IMDG_K16_MASK);
+        } else {
+          size_t at = (((size_t)(s >> 6) * (c->kmax >> 2) + (m >> 2)) *
+                           IMDG_WAVE + (s & 63)) * IMDG_GROUP + (m & 3);
+          uint32_t e = hl[at];
+          idx = (int)(e & IMDG_IDX_MASK);
+          code = (int)(e >> IMDG_IDX_BITS);
+        }
         if (code == IMDG_CODE_PAD) continue;
         int j = idx;
         long jj = v.aid[j] - c->in_off[r];

@@ -545,6 +545,9 @@ struct ListLane {
   float3 ui;
   float lo, hi;
   uint32_t* lst;  // first entry of the atom's grouped list
+  char* lst16;    // ... in the 16-bit format
+  const int* cb;  // the chunk's base slot per code (16-bit format, LDS)
+  bool w16;       // the chunk is stored in the 16-bit format
   int cnt, kmax;
   int cx, cy, cz;  // the atom's cell (image shifts of the FP64 decision)
 };
@@ -558,6 +561,21 @@ __device__ __forceinline__ void list_append(ListLane& L, uint32_t e) {
   const unsigned off = ((m >> 2) << 10) | ((m & 3u) << 2);
   *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(L.lst) + off) = e;
   L.cnt++;
+}
+
+// The same in the chunk's format: 32-bit slot | code, or 16-bit code |
+// slot - base[code] (groups of 64 lanes 512 B apart).
+__device__ __forceinline__ void list_put(ListLane& L, uint32_t code,
+                                         int slot) {
+  if (L.w16) {
+    const unsigned m = (unsigned)min(L.cnt, L.kmax - 1);
+    const unsigned off = ((m >> 2) << 9) | ((m & 3u) << 1);
+    *reinterpret_cast<uint16_t*>(L.lst16 + off) =
+        (uint16_t)((code << IMDG_K16_BITS) | (unsigned)(slot - L.cb[code]));
+    L.cnt++;
+  } else {
+    list_append(L, (uint32_t)slot | (code << IMDG_IDX_BITS));
+  }
 }
 
 // Image shift of neighbour cell offset o along an axis: -1 / +1 when the
@@ -626,8 +644,7 @@ __device__ __forceinline__ void list_cell(
     }
 #pragma unroll
     for (int u = 0; u < 4; u++)
-      if (in[u])
-        list_append(L, (uint32_t)(jb + t0 + u) | (code << IMDG_IDX_BITS));
+      if (in[u]) list_put(L, code, jb + t0 + u);
   }
 }
 
@@ -647,6 +664,7 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
   __shared__ float off[3 * IMDG_NCODES];
   __shared__ int tinfo[2];
   __shared__ int wmax[IMDG_NWAVE];
+  __shared__ int cbase[IMDG_NWAVE][IMDG_META];
   if (!a.ctl->any_rebuild || a.ctl->frozen) return;
   int b = blockIdx.x;
   int r = a.blk_rep[b];
@@ -688,18 +706,20 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
     }
     __syncthreads();
     int i = b * IMDG_BLOCK + threadIdx.x;
-    if (i - R.atom_off < R.natoms) {
-      int cx, cy, cz;
-      cell_coords(R, a.acell[i], cx, cy, cz);
+    const bool mine = i - R.atom_off < R.natoms;
+    {
+      int cx = 0, cy = 0, cz = 0;
+      if (mine) cell_coords(R, a.acell[i], cx, cy, cz);
       int g0 = R.gdim[0], g1 = R.gdim[1], g2 = R.gdim[2];
       ListLane L;
       L.i = i;
-      L.ui = rec_xyz<MIXED>(a, i);
+      L.ui = mine ? rec_xyz<MIXED>(a, i) : make_float3(0.f, 0.f, 0.f);
       float cs = (float)R.cellsz;
       L.lo = cs * (float)(1.0 - IMDG_SCREEN_BAND);
       L.hi = cs * (float)(1.0 + IMDG_SCREEN_BAND);
       L.lst = a.nbr + (((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63)) *
                           IMDG_GROUP;
+      L.lst16 = const_cast<char*>(list16_of(a.nbr, i, kmax));
       L.cnt = 0;
       L.kmax = kmax;
       L.cx = cx;
@@ -707,7 +727,8 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
       L.cz = cz;
       // the 27 neighbour cells (code k = (ox+1)*9 + (oy+1)*3 + (oz+1)):
       // first slot, count, tile position and image shift, all loads in
-      // flight together; a cell beyond an open face has count 0
+      // flight together; a cell beyond an open face has count 0, and so
+      // has every cell of a lane without an atom
       int jb[27], jn[27], tb[27];
 #pragma unroll
       for (int k = 0; k < 27; k++) {
@@ -716,8 +737,8 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
         int sx = n0 < 0 ? -1 : (n0 >= g0 ? 1 : 0);
         int sy = n1 < 0 ? -1 : (n1 >= g1 ? 1 : 0);
         int sz = n2 < 0 ? -1 : (n2 >= g2 ? 1 : 0);
-        bool valid = (sx == 0 || R.pbc[0] == 1) && (sy == 0 || R.pbc[1] == 1) &&
-                     (sz == 0 || R.pbc[2] == 1);
+        bool valid = mine && (sx == 0 || R.pbc[0] == 1) &&
+                     (sy == 0 || R.pbc[1] == 1) && (sz == 0 || R.pbc[2] == 1);
         n0 -= sx * g0;
         n1 -= sy * g1;
         n2 -= sz * g2;
@@ -726,6 +747,32 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
         jb[k] = cb;
         jn[k] = valid ? cn : 0;
       }
+      // 16-bit format: the chunk's (wavefront's) base slot per code, and
+      // whether every neighbour cell run lies within 2048 slots of it
+      L.w16 = false;
+      if (a.list16) {
+        const int w = threadIdx.x / IMDG_WAVE, lane = threadIdx.x & 63;
+        const int chunk = i >> 6;
+        int wide = 0;
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+          int lo = wave_min_i(jn[k] > 0 ? jb[k] : 0x7fffffff);
+          int hi = wave_max_i(jn[k] > 0 ? jb[k] + jn[k] : 0);
+          if (lo == 0x7fffffff) lo = hi = 0;
+          wide |= (hi - lo > IMDG_K16_SPAN) ? 1 : 0;
+          if (lane == 0) cbase[w][k] = lo;
+        }
+        if (lane == 0) {
+          cbase[w][IMDG_CODE_PAD] = chunk * IMDG_CHUNK;
+          cbase[w][IMDG_META_WIDE] = wide;
+        }
+        wave_lds_sync();
+        if (lane <= IMDG_META_WIDE)
+          a.cmeta[(size_t)chunk * IMDG_META + lane] = cbase[w][lane];
+        L.cb = cbase[w];
+        L.w16 = wide == 0;
+      }
+      if (!mine) goto list_done;
       if (tok) {
         // tile positions: a search for the centre cell of each of the 9
         // columns; the z neighbours follow it in slot order (cells are
@@ -761,16 +808,16 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
       // list fits whenever the list does)
       int cnt = L.cnt;
       if (cnt <= kmax) {
-        uint32_t pad = (uint32_t)i | ((uint32_t)IMDG_CODE_PAD << IMDG_IDX_BITS);
         int mend = ((cnt + IMDG_LIST_PAD - 1) / IMDG_LIST_PAD) * IMDG_LIST_PAD;
         for (int m = cnt; m < mend; m++) {
           L.cnt = m;
-          list_append(L, pad);
+          list_put(L, IMDG_CODE_PAD, i);
         }
       }
       a.nnb[i] = cnt < kmax ? cnt : kmax;
       seen = cnt;
     }
+  list_done:;
   }
   // block max of the list length (k_list_done reduces the blocks: one
   // atomic word updated by every block would serialise them)

@@ -17,6 +17,18 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, IMDG_WAVE));
+  return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, IMDG_WAVE));
+  return v;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, IMDG_WAVE));
@@ -124,6 +136,14 @@ __device__ __forceinline__ const uint4* list_of(const uint32_t* nbr, int i,
                                                 int kmax) {
   return reinterpret_cast<const uint4*>(nbr) +
          ((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63);
+}
+
+// Byte address of slot i's first 16-bit entry (group 0) in the 16-bit
+// format; group g is at + g * 512 bytes, entry m % 4 at + 2 (m % 4).
+__device__ __forceinline__ const char* list16_of(const uint32_t* nbr, int i,
+                                                 int kmax) {
+  return reinterpret_cast<const char*>(nbr) + (size_t)(i >> 6) * kmax * 256 +
+         (size_t)(i & 63) * 8;
 }
 
 __device__ __forceinline__ void cell_coords(const RepDev& R, int cell,

@@ -23,6 +23,16 @@
 //    multiple of 8 with entries pointing at the atom itself through the
 //    far-away code IMDG_CODE_PAD; stored four per lane: entries 4c..4c+3 of
 //    slot i form the uint4 at ((i / 64) * (kmax / 4) + c) * 64 + i % 64;
+//  * nbr in the 16-bit format (FP32 contexts whose loops read one-grid
+//    segment tables): per 64-slot chunk, for each cell offset code, a base
+//    slot (the lowest first slot of that neighbour cell over the chunk's
+//    atoms); an entry is code << 11 | (slot - base[code]), 2 bytes, stored
+//    in the first half of the chunk's list region: entries 4g..4g+3 of slot
+//    i at halfword ((i / 64) * kmax + g * 64 + i % 64) * 4 + (entry % 4).
+//    A chunk whose neighbour cells span more than 2048 slots for some code
+//    (a chunk across a plane of the grid, or very dense cells) keeps the
+//    32-bit format; cmeta[chunk * IMDG_META + code] holds the bases and
+//    cmeta[chunk * IMDG_META + IMDG_META_WIDE] marks such chunks;
 //  * tiles: for every block of IMDG_BLOCK slots, the slot runs of all
 //    cells adjacent to the block's cells, staged into LDS by the list build;
 //  * tables as knots (p0_k, dv_k = t[k+1] - t[k]), k = 0..len, of IMD's
@@ -51,6 +61,11 @@
 #define IMDG_PAD_SHIFT 1.0e6
 #define IMDG_GROUP 4          // list entries per lane and load
 #define IMDG_LIST_PAD 16      // lists padded to whole bodies of 4 lanes
+#define IMDG_K16_BITS 11      // 16-bit entries: offset bits below the code
+#define IMDG_K16_MASK ((1u << IMDG_K16_BITS) - 1u)
+#define IMDG_K16_SPAN (1 << IMDG_K16_BITS)
+#define IMDG_META 32          // ints of chunk metadata: bases, wide flag
+#define IMDG_META_WIDE 28
 #define IMDG_TILE_CAP 2048    // records of one block's LDS tile
 #define IMDG_MAXRUN 40        // slot runs of one tile
 
@@ -238,6 +253,8 @@ struct StepArgs {
   int eeam;               // EEAM: second density P = sum rho^2, energy E(P)
   R* eeP;                 // EEAM P per slot (written with store)
   R* eedm;                // EEAM 2E'(P) per slot (read by the force loop)
+  const int* cmeta;       // chunk metadata of the 16-bit list format
+  int list16;             // the list may hold 16-bit chunks
   const double2* knots64; // FP64 knots of the embedding (tab64[0]) and the
   const Tab64* tab64;     // EEAM energy term (tab64[1]); FP32 kernels only
   int lpa_den, lpa_frc;   // lanes per slot of the density / force loops
@@ -273,6 +290,8 @@ struct BuildArgs {
   int* nnb;
   int* blkmax;            // longest list of every block (k_list)
   int tile_cap;           // records a tile may hold (<= IMDG_TILE_CAP)
+  int* cmeta;             // chunk metadata of the 16-bit list format
+  int list16;             // build the 16-bit format where a chunk allows
   int nrep;
   int nslots;
   int nblk;

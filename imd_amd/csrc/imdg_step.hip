@@ -549,6 +549,7 @@ __device__ __forceinline__ R lane_group_min(R v) {
 // reads words s*(4/L) .. s*(4/L) + 4/L - 1 of each of them.
 template <int L>
 struct ListRef {
+  static constexpr bool W16 = false;
   const uint32_t* p;  // the lane's first word of the body
   __device__ __forceinline__ uint4 load() const {
     constexpr int G = IMDG_WAVE * IMDG_GROUP;  // words per group
@@ -611,6 +612,107 @@ __device__ __forceinline__ void list_walk(const uint32_t* w, int nnb, F body) {
       ready(A);
     }
   }
+}
+
+// The 16-bit format (a chunk's entries code << 11 | slot - base[code],
+// halfword groups of 64 lanes): the same lane pattern, each load half as
+// wide.  load() returns the loaded words as they are (so the wait for them
+// stays where the walk places it); unpack16 spreads them to four entries.
+template <int L>
+struct ListRef16 {
+  static constexpr bool W16 = true;
+  const uint16_t* p;  // the lane's first halfword of the body
+  __device__ __forceinline__ uint4 load() const {
+    constexpr int G = IMDG_WAVE * IMDG_GROUP;  // halfwords per group
+    if (L == 1) {
+      uint2 x = *reinterpret_cast<const uint2*>(p);
+      return make_uint4(x.x, x.y, 0u, 0u);
+    }
+    if (L == 2) {
+      uint32_t x = *reinterpret_cast<const uint32_t*>(p);
+      uint32_t y = *reinterpret_cast<const uint32_t*>(p + G);
+      return make_uint4(x, y, 0u, 0u);
+    }
+    return make_uint4(p[0], p[G], p[2 * G], p[3 * G]);
+  }
+};
+
+template <int L>
+__device__ __forceinline__ uint4 unpack16(uint4 r) {
+  if (L == 4) return r;
+  return make_uint4(r.x & 0xffffu, r.x >> 16, r.y & 0xffffu, r.y >> 16);
+}
+
+template <int L>
+__device__ __forceinline__ const uint16_t* list_words16(const uint32_t* nbr,
+                                                        int i, int kmax,
+                                                        int s) {
+  return reinterpret_cast<const uint16_t*>(list16_of(nbr, i, kmax)) +
+         s * (IMDG_GROUP / L);
+}
+
+// list_walk on the 16-bit format.
+template <int L, typename F>
+__device__ __forceinline__ void list_walk16(const uint16_t* w, int nnb,
+                                            F body) {
+  constexpr int E = IMDG_GROUP * L;
+  constexpr size_t BW = (size_t)L * IMDG_WAVE * IMDG_GROUP;  // halfwords
+  int nbody = (nnb + E - 1) / E;
+  if (L <= 2) nbody = (nbody + 1) & ~1;
+  const int blast = nbody > 0 ? nbody - 1 : 0;
+  uint4 A = ListRef16<L>{w}.load();
+  uint4 B = ListRef16<L>{w + (size_t)min(1, blast) * BW}.load();
+  A = make_uint4(nbody > 0 ? A.x : 0u, nbody > 0 ? A.y : 0u,
+                 nbody > 0 ? A.z : 0u, nbody > 0 ? A.w : 0u);
+  B = make_uint4(nbody > 0 ? B.x : 0u, nbody > 0 ? B.y : 0u,
+                 nbody > 0 ? B.z : 0u, nbody > 0 ? B.w : 0u);
+  ready(A);
+  for (int h = 0; 2 * h < nbody; h++) {
+    body(unpack16<L>(A), ListRef16<L>{w + (size_t)min(2 * h + 2, blast) * BW},
+         A);
+    if (L <= 2 || 2 * h + 1 < nbody) {
+      body(unpack16<L>(B),
+           ListRef16<L>{w + (size_t)min(2 * h + 3, blast) * BW}, B);
+      ready(A);
+    }
+  }
+}
+
+// Cell offset code and neighbour slot of a list entry in either format
+// (16-bit: the base slot of the entry's code is the w of the wavefront's
+// offset row, set per chunk); o receives the code's offset row.
+template <bool W16>
+__device__ __forceinline__ uint32_t entry_code(uint32_t e) {
+  return W16 ? e >> IMDG_K16_BITS : e >> IMDG_IDX_BITS;
+}
+template <bool W16>
+__device__ __forceinline__ int entry_slot(uint32_t e, const float4* off,
+                                          float4& o) {
+  o = off[entry_code<W16>(e)];
+  return W16 ? __float_as_int(o.w) + (int)(e & IMDG_K16_MASK)
+             : (int)(e & IMDG_IDX_MASK);
+}
+
+// The chunk's base slots into the w of the wavefront's offset row and its
+// format (true: 16-bit entries).  Called by every lane of the wavefront.
+__device__ __forceinline__ bool chunk_bases(const int* __restrict__ cmeta,
+                                            int c, int lane, float4* off) {
+  const int v = cmeta[(size_t)c * IMDG_META + min(lane, IMDG_META_WIDE)];
+  if (lane < IMDG_NCODES) off[lane].w = __int_as_float(v);
+  const int wide = __shfl(v, IMDG_META_WIDE, IMDG_WAVE);
+  wave_lds_sync();
+  return wide == 0;
+}
+
+// One walk over a slot's list in the chunk's format.
+template <int L, typename F>
+__device__ __forceinline__ void walk_either(bool w16, const uint32_t* nbr,
+                                            int i, int kmax, int s, int nnb,
+                                            F body) {
+  if (w16)
+    list_walk16<L>(list_words16<L>(nbr, i, kmax, s), nnb, body);
+  else
+    list_walk<L>(list_words<L>(nbr, i, kmax, s), nnb, body);
 }
 
 // Single-species column of the density loop.
@@ -746,16 +848,17 @@ __device__ __forceinline__ void density_group_poly(
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     // the type is not needed: gather 12 B
-    const float* p = reinterpret_cast<const float*>(rec + (en[u] & IMDG_IDX_MASK));
+    const int j = entry_slot<PF::W16>(en[u], off, o[u]);
+    const float* p = reinterpret_cast<const float*>(rec + j);
     float2 xy = *reinterpret_cast<const float2*>(p);
     ux[u] = xy.x;
     uy[u] = xy.y;
     uz[u] = p[2];
-    o[u] = off[en[u] >> IMDG_IDX_BITS];
-    ol[u] = olo[en[u] >> IMDG_IDX_BITS];
   }
   __builtin_amdgcn_sched_barrier(0);
   slot = pf.load();
+#pragma unroll
+  for (int u = 0; u < 4; u++) ol[u] = olo[entry_code<PF::W16>(en[u])];
   float r2[4], chi[4];
   int kk[4];
 #pragma unroll
@@ -988,13 +1091,12 @@ __device__ __forceinline__ void force_group_sgf(
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
   float4 uj[4], o[4], ol[4];
 #pragma unroll
-  for (int u = 0; u < 4; u++) {
-    uj[u] = rec[en[u] & IMDG_IDX_MASK];
-    o[u] = off[en[u] >> IMDG_IDX_BITS];
-    ol[u] = olo[en[u] >> IMDG_IDX_BITS];
-  }
+  for (int u = 0; u < 4; u++)
+    uj[u] = rec[entry_slot<PF::W16>(en[u], off, o[u])];
   __builtin_amdgcn_sched_barrier(0);
   slot = pf.load();
+#pragma unroll
+  for (int u = 0; u < 4; u++) ol[u] = olo[entry_code<PF::W16>(en[u])];
   float dx[4], dy[4], dz[4], r2[4], chi[4];
   int kk[4];
 #pragma unroll
@@ -1046,13 +1148,14 @@ __device__ __forceinline__ void force_group_sgn(
   int jt[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    uj[u] = rec[en[u] & IMDG_IDX_MASK];
-    jt[u] = atype[en[u] & IMDG_IDX_MASK];
-    o[u] = off[en[u] >> IMDG_IDX_BITS];
-    ol[u] = olo[en[u] >> IMDG_IDX_BITS];
+    const int j = entry_slot<PF::W16>(en[u], off, o[u]);
+    uj[u] = rec[j];
+    jt[u] = atype[j];
   }
   __builtin_amdgcn_sched_barrier(0);
   slot = pf.load();
+#pragma unroll
+  for (int u = 0; u < 4; u++) ol[u] = olo[entry_code<PF::W16>(en[u])];
   float dx[4], dy[4], dz[4], r2[4], chi[4];
   int kk[4];
 #pragma unroll
@@ -1097,13 +1200,12 @@ __device__ __forceinline__ void density_group_polyn(
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
   float4 uj[4], o[4], ol[4];
 #pragma unroll
-  for (int u = 0; u < 4; u++) {
-    uj[u] = rec[en[u] & IMDG_IDX_MASK];
-    o[u] = off[en[u] >> IMDG_IDX_BITS];
-    ol[u] = olo[en[u] >> IMDG_IDX_BITS];
-  }
+  for (int u = 0; u < 4; u++)
+    uj[u] = rec[entry_slot<PF::W16>(en[u], off, o[u])];
   __builtin_amdgcn_sched_barrier(0);
   slot = pf.load();
+#pragma unroll
+  for (int u = 0; u < 4; u++) ol[u] = olo[entry_code<PF::W16>(en[u])];
   float r2[4], chi[4];
   int kk[4];
 #pragma unroll
@@ -1170,6 +1272,9 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
       wave_offsets(Rp, off, olo);
       cur = r;
     }
+    bool w16 = false;
+    if constexpr (sizeof(R) == 4)
+      if (a.list16) w16 = chunk_bases(a.cmeta, c, lane, off);
     R rho_keep = (R)0, pee_keep = (R)0;
 #pragma unroll 1
     for (int p = 0; p < L; p++) {
@@ -1185,11 +1290,11 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
           const float4* __restrict__ DQ =
               reinterpret_cast<const float4*>(K + (a.dpoly - klo));
           const TArg ta = targ(tr);
-          list_walk<L>(nw, a.nnb[i],
-                       [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                         density_group_poly(e4, pf, slot, rec, off, olo, ui,
-                                            DQ, ta, rho, r2min);
-                       });
+          walk_either<L>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
+                         [&](uint4 e4, auto pf, uint4& slot) {
+                           density_group_poly(e4, pf, slot, rec, off, olo,
+                                              ui, DQ, ta, rho, r2min);
+                         });
         } else {
           auto generic = [&]() {
             list_walk<L>(nw, a.nnb[i],
@@ -1205,12 +1310,12 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
               const float4* __restrict__ DQ =
                   reinterpret_cast<const float4*>(a.knots + a.dpolyn);
               const TArg ta = targ(tr);
-              list_walk<L>(nw, a.nnb[i],
-                           [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                             density_group_polyn(e4, pf, slot, rec, off, olo,
-                                                 ui, DQ, it * nt, len0, ta,
-                                                 rho, r2min);
-                           });
+              walk_either<L>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
+                             [&](uint4 e4, auto pf, uint4& slot) {
+                               density_group_polyn(e4, pf, slot, rec, off,
+                                                   olo, ui, DQ, it * nt, len0,
+                                                   ta, rho, r2min);
+                             });
               is_short |= (r2min < b0);
             } else {
               generic();
@@ -1372,6 +1477,9 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
       wave_offsets(Rp, off, olo);
       cur = r;
     }
+    bool w16 = false;
+    if constexpr (sizeof(R) == 4)
+      if (a.list16) w16 = chunk_bases(a.cmeta, c, lane, off);
     // the slot's sums kept by this lane (pass gs); vlo: the virial's sum
     // of the pair gradients' FP32 remainders (FP32 one-grid loops)
     R kx = 0, ky = 0, kz = 0, kep = 0, kvir = 0, kvlo = 0;
@@ -1390,12 +1498,13 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
           const float4* __restrict__ S =
               reinterpret_cast<const float4*>(a.knots + a.jointn);
           const TArg ta = targ(tp);
-          list_walk<L>(nw, a.nnb[i],
-                       [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                         force_group_sgn(e4, pf, slot, rec, atype, off, olo,
-                                         ui, 0.5f * dFi, S, it * nt, plen, ta,
-                                         fx, fy, fz, ep, vir, vlo, r2min);
-                       });
+          walk_either<L>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
+                         [&](uint4 e4, auto pf, uint4& slot) {
+                           force_group_sgn(e4, pf, slot, rec, atype, off, olo,
+                                           ui, 0.5f * dFi, S, it * nt, plen,
+                                           ta, fx, fy, fz, ep, vir, vlo,
+                                           r2min);
+                         });
           is_short |= (r2min < pb);
         } else if constexpr (MODE == FM_SG && !CUB && !EE) {
           // one grid: sums in units of 1/(2 istep), scaled once per slot
@@ -1406,12 +1515,12 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
             const float2* __restrict__ Rr =
                 reinterpret_cast<const float2*>(K + (a.joint - klo) + 2 * plen);
             const TArg ta = targ(tp);
-            list_walk<L>(nw, a.nnb[i],
-                         [&](uint4 e4, ListRef<L> pf, uint4& slot) {
-                           force_group_sgf(e4, pf, slot, rec, off, olo, ui,
-                                           0.5f * dFi, Q, Rr, ta, fx, fy, fz,
-                                           ep, vir, vlo);
-                         });
+            walk_either<L>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
+                           [&](uint4 e4, auto pf, uint4& slot) {
+                             force_group_sgf(e4, pf, slot, rec, off, olo, ui,
+                                             0.5f * dFi, Q, Rr, ta, fx, fy,
+                                             fz, ep, vir, vlo);
+                           });
           } else {
             const R4* __restrict__ S =
                 reinterpret_cast<const R4*>(K + (a.joint - klo));
@@ -1557,6 +1666,7 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
   typedef typename V2<R>::T R2;
   __shared__ R4 off[IMDG_NCODES];
   __shared__ double red[IMDG_NWAVE][6];
+  __shared__ int sbase[IMDG_NWAVE][IMDG_META];
   if (a.ctl->frozen) return;
   const int b = blockIdx.x;
   const RepDev& Rp = a.rep[a.blk_rep[b]];
@@ -1568,7 +1678,13 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
     o.w = 0;
     off[threadIdx.x] = o;
   }
+  // the 16-bit list format: base slots and format of the block's chunks
+  const int wv = threadIdx.x / IMDG_WAVE, lane = threadIdx.x & (IMDG_WAVE - 1);
+  if (a.list16 && lane <= IMDG_META_WIDE)
+    sbase[wv][lane] =
+        a.cmeta[(size_t)(b * IMDG_NWAVE + wv) * IMDG_META + lane];
   __syncthreads();
+  const bool w16 = a.list16 && sbase[wv][IMDG_META_WIDE] == 0;
   const TabDev<R>& tp = tabs[0];
   const TabDev<R>& tr = tabs[1];
   const bool hp = a.has_pair != 0, he = a.has_eam != 0;
@@ -1597,17 +1713,28 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
     const R dmi = ee ? a.eedm[i] : (R)0;
     R t[6] = {0, 0, 0, 0, 0, 0};
     const uint4* lw = list_of(a.nbr, i, a.kmax);
+    const char* lw16 = list16_of(a.nbr, i, a.kmax);
     const int ngroup = (a.nnb[i] + IMDG_GROUP - 1) / IMDG_GROUP;
     for (int gq = 0; gq < ngroup; gq++) {
-      uint4 e4 = lw[(size_t)gq * IMDG_WAVE];
+      uint4 e4;
+      if (w16)
+        e4 = unpack16<1>(make_uint4(
+            reinterpret_cast<const uint2*>(lw16 + (size_t)gq * 512)->x,
+            reinterpret_cast<const uint2*>(lw16 + (size_t)gq * 512)->y, 0u,
+            0u));
+      else
+        e4 = lw[(size_t)gq * IMDG_WAVE];
       uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         // entries past the list's end are padding: a far-away image,
         // outside every table
-        const int j = en[u] & IMDG_IDX_MASK;
+        const uint32_t code =
+            w16 ? en[u] >> IMDG_K16_BITS : en[u] >> IMDG_IDX_BITS;
+        const int j = w16 ? sbase[wv][code] + (int)(en[u] & IMDG_K16_MASK)
+                          : (int)(en[u] & IMDG_IDX_MASK);
         const R4 uj = rec[j];
-        const R4 o = off[en[u] >> IMDG_IDX_BITS];
+        const R4 o = off[code];
         const R dx = (uj.x - ui.x) + o.x;
         const R dy = (uj.y - ui.y) + o.y;
         const R dz = (uj.z - ui.z) + o.z;
