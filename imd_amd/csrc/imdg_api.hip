@@ -1047,7 +1047,8 @@ int read_slots(imdg_ctx* c, SlotView& v, bool state) {
 // and the list; the force loop's per-atom energies build on the density
 // loop's, so the caller finishes with one regular force evaluation.
 template <typename R>
-int time_loop(imdg_ctx* c, bool density, int reps, double* ms) {
+int time_loop(imdg_ctx* c, bool density, int reps, double* ms,
+              bool integrate = false) {
   StepArgs<R> a = step_args<R>(c);
   const TabDev<R>* t = (const TabDev<R>*)c->d_tabs;
   bool nt1 = c->cfg.ntypes == 1;
@@ -1059,9 +1060,9 @@ int time_loop(imdg_ctx* c, bool density, int reps, double* ms) {
     }
     a.lds_lo = c->lds_frc_lo;
     a.lds_n = c->lds_frc_n;
-    a.unitsB = c->grid_frc[0];
-    return launch_force<R>(a, t, c->force_mode, false, c->grid_frc[0],
-                           c->stream);
+    const int grid = c->grid_frc[integrate ? 1 : 0];
+    a.unitsB = grid;
+    return launch_force<R>(a, t, c->force_mode, integrate, grid, c->stream);
   };
   hipEvent_t e0 = get_event(c), e1 = get_event(c);
   hipError_t e = launch();
@@ -1115,22 +1116,60 @@ int autotune_t(imdg_ctx* c, int reps, imdg_tuning* out) {
     if ((rc = time_loop<R>(c, true, 0, &ms))) return rc;  // records for B
   }
   {
+    // the step's force launch integrates as well (unless the move is a
+    // launch of its own): time that variant, on a copy of the state it
+    // changes (positions, momenta, (u, type) records), put back afterwards
+    const bool integ = !c->split_move;
+    const size_t ns = c->nslots, rb = (c->mixed ? 16 : 32) * ns;
+    void *sp = nullptr, *sm = nullptr, *sr = nullptr;
+    int pc = 0;
+    if (integ) {
+      Ctl h;
+      if ((rc = read_ctl(c, &h))) return rc;
+      pc = h.pos_cur;
+      if (hipMalloc(&sp, 32 * ns) != hipSuccess ||
+          hipMalloc(&sm, 32 * ns) != hipSuccess ||
+          hipMalloc(&sr, rb) != hipSuccess) {
+        hipFree(sp);
+        hipFree(sm);
+        hipFree(sr);
+        return fail(c, IMDG_ERR_NOMEM, "autotune: no room for a state copy");
+      }
+      HIPCHK(c, hipMemcpyAsync(sp, c->d_pos[pc], 32 * ns,
+                               hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(sm, c->d_mom[pc], 32 * ns,
+                               hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(sr, c->d_reca, rb, hipMemcpyDeviceToDevice,
+                               c->stream));
+    }
     int bl = c->lpa_frc, bt = c->thr_frc;
     best_f = 1e300;
-    for (int li = 0; li < 3; li++)
+    for (int li = 0; li < 3 && rc == IMDG_OK; li++)
       for (int ti = 0; ti < 4; ti++) {
         if (thr_f[ti] > fmax) continue;
         c->lpa_frc = lanes[li];
         c->thr_frc = thr_f[ti];
         if (plan(c) != IMDG_OK) continue;
         double ms;
-        if ((rc = time_loop<R>(c, false, reps, &ms))) return rc;
+        if ((rc = time_loop<R>(c, false, reps, &ms, integ))) break;
         if (ms < best_f) {
           best_f = ms;
           bl = lanes[li];
           bt = thr_f[ti];
         }
       }
+    if (integ) {
+      hipMemcpyAsync(c->d_pos[pc], sp, 32 * ns, hipMemcpyDeviceToDevice,
+                     c->stream);
+      hipMemcpyAsync(c->d_mom[pc], sm, 32 * ns, hipMemcpyDeviceToDevice,
+                     c->stream);
+      hipMemcpyAsync(c->d_reca, sr, rb, hipMemcpyDeviceToDevice, c->stream);
+      hipStreamSynchronize(c->stream);
+      hipFree(sp);
+      hipFree(sm);
+      hipFree(sr);
+    }
+    if (rc) return rc;
     c->lpa_frc = bl;
     c->thr_frc = bt;
   }

@@ -753,15 +753,17 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
       if (a.list16) {
         const int w = threadIdx.x / IMDG_WAVE, lane = threadIdx.x & 63;
         const int chunk = i >> 6;
-        int wide = 0;
+        // every lane checks its own runs against the base, so a chunk is
+        // stored in 16 bits only where every entry's offset fits
+        bool bad = false;
 #pragma unroll
         for (int k = 0; k < 27; k++) {
           int lo = wave_min_i(jn[k] > 0 ? jb[k] : 0x7fffffff);
-          int hi = wave_max_i(jn[k] > 0 ? jb[k] + jn[k] : 0);
-          if (lo == 0x7fffffff) lo = hi = 0;
-          wide |= (hi - lo > IMDG_K16_SPAN) ? 1 : 0;
+          if (lo == 0x7fffffff) lo = 0;
+          bad |= jn[k] > 0 && (jb[k] < lo || jb[k] + jn[k] - lo > IMDG_K16_SPAN);
           if (lane == 0) cbase[w][k] = lo;
         }
+        const int wide = __ballot(bad) != 0ull ? 1 : 0;
         if (lane == 0) {
           cbase[w][IMDG_CODE_PAD] = chunk * IMDG_CHUNK;
           cbase[w][IMDG_META_WIDE] = wide;

@@ -17,17 +17,24 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Minimum over the 64 lanes (every lane active) through DPP
+// moves (quad permutes, row mirrors, row broadcasts of lanes 15 and 31:
+// VALU operations without a trip through LDS), read from lane 63.
+#define IMDG_DPP_REDUCE(OP, v)                                                \
+  do {                                                                        \
+    v = OP(v, __builtin_amdgcn_update_dpp(v, v, 0xb1, 0xf, 0xf, false));      \
+    v = OP(v, __builtin_amdgcn_update_dpp(v, v, 0x4e, 0xf, 0xf, false));      \
+    v = OP(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xf, 0xf, false));     \
+    v = OP(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xf, 0xf, false));     \
+    v = OP(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xa, 0xf, false));     \
+    v = OP(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xc, 0xf, false));     \
+  } while (0)
+
 __device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, IMDG_WAVE));
-  return v;
+  IMDG_DPP_REDUCE(min, v);
+  return __builtin_amdgcn_readlane(v, 63);
 }
 
-__device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, IMDG_WAVE));
-  return v;
-}
 
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
