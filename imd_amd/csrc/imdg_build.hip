@@ -437,10 +437,11 @@ __device__ void build_tile(const BuildArgs& a, int b, TileLds& S) {
   if (ok && l < nrun) T.run[l] = make_int4(run[l].x, len, pos, 0);
   if (l == 0) {
     if (!ok) {
+      // counted by k_list_done (one counter word updated by every block
+      // would serialise the blocks)
       T.ok = 0;
       T.nrun = 0;
       T.natoms = 0;
-      atomicAdd(&a.ctl->tiles_fallback, 1);
     } else {
       T.ok = 1;
       T.nrun = nrun;
@@ -544,37 +545,40 @@ struct ListLane {
   int i;
   float3 ui;
   float lo, hi;
-  uint32_t* lst;  // first entry of the atom's grouped list
-  char* lst16;    // ... in the 16-bit format
-  const int* cb;  // the chunk's base slot per code (16-bit format, LDS)
-  bool w16;       // the chunk is stored in the 16-bit format
+  char* lst;  // the lane's first entry (group 0) in the chunk's format
   int cnt, kmax;
   int cx, cy, cz;  // the atom's cell (image shifts of the FP64 decision)
+  uint32_t g0, g1, g2, g3;  // entries of the group being filled
 };
 
-// Entry m of the lane's list: group m/4 of the lane, slot m%4 (a 32-bit
-// byte offset from the lane's first entry: groups of 64 lanes are 1 KiB
-// apart).  Past kmax the last slot is overwritten: such a list has
-// overflowed, and the build is repeated with a longer one.
-__device__ __forceinline__ void list_append(ListLane& L, uint32_t e) {
-  const unsigned m = (unsigned)min(L.cnt, L.kmax - 1);
-  const unsigned off = ((m >> 2) << 10) | ((m & 3u) << 2);
-  *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(L.lst) + off) = e;
-  L.cnt++;
-}
-
-// The same in the chunk's format: 32-bit slot | code, or 16-bit code |
-// slot - base[code] (groups of 64 lanes 512 B apart).
-__device__ __forceinline__ void list_put(ListLane& L, uint32_t code,
-                                         int slot) {
-  if (L.w16) {
-    const unsigned m = (unsigned)min(L.cnt, L.kmax - 1);
-    const unsigned off = ((m >> 2) << 9) | ((m & 3u) << 1);
-    *reinterpret_cast<uint16_t*>(L.lst16 + off) =
-        (uint16_t)((code << IMDG_K16_BITS) | (unsigned)(slot - L.cb[code]));
-    L.cnt++;
+// Append entry e.  The four entries of a group (slots m%4 of group m/4)
+// are collected in registers and written with one store when the group is
+// full: 8 B in the 16-bit format (code << 11 | slot - base of the code;
+// groups of 64 lanes 512 B apart), 16 B in the 32-bit one (slot | code <<
+// 27; 1 KiB apart).  One scattered 2- or 4-byte store per entry would cost
+// an L2 request each.  Entries past kmax are only counted: the list has
+// overflowed and is built again, longer.
+template <bool W16>
+__device__ __forceinline__ void list_put(ListLane& L, uint32_t e) {
+  if (W16) {
+    // shift e in at the top of the 64-bit register (g1:g0): after four
+    // appends g0 = e0 | e1 << 16, g1 = e2 | e3 << 16
+    L.g0 = __builtin_amdgcn_alignbit(L.g1, L.g0, 16);
+    L.g1 = __builtin_amdgcn_alignbit(e, L.g1, 16);
   } else {
-    list_append(L, (uint32_t)slot | (code << IMDG_IDX_BITS));
+    L.g0 = L.g1;
+    L.g1 = L.g2;
+    L.g2 = L.g3;
+    L.g3 = e;
+  }
+  L.cnt++;
+  if ((L.cnt & 3) == 0 && L.cnt <= L.kmax) {
+    const unsigned g = (unsigned)(L.cnt >> 2) - 1u;
+    if (W16)
+      *reinterpret_cast<uint2*>(L.lst + (g << 9)) = make_uint2(L.g0, L.g1);
+    else
+      *reinterpret_cast<uint4*>(L.lst + (g << 10)) =
+          make_uint4(L.g0, L.g1, L.g2, L.g3);
   }
 }
 
@@ -595,16 +599,17 @@ typedef __attribute__((address_space(3))) const float lds_float;
 // flight together): FP32 screen, FP64 decision inside the band (one branch
 // per four candidates; the band is rarely hit), append.  The separation is
 // u_j + (o - u_i), with o - u_i formed once per cell: two roundings of a
-// few Angstrom, far inside the screening band.
-template <bool MIXED, bool TOK>
+// few Angstrom, far inside the screening band.  Candidate t of the cell is
+// entry ebase + t.
+template <bool MIXED, bool TOK, bool W16>
 __device__ __forceinline__ void list_cell(
     const BuildArgs& a, const RepDev& R, const double4* __restrict__ pos,
     lds_float3* trec, ListLane& L, int jb, int jn, int tb, float o0,
-    float o1, float o2, bool self, bool owner, uint32_t code) {
+    float o1, float o2, bool self, bool owner, int code, uint32_t ebase) {
   const float w0 = o0 - L.ui.x, w1 = o1 - L.ui.y, w2 = o2 - L.ui.z;
   // the cell's run in the tile: four records per pass at immediate offsets
   // from one address (up to three records past the run: inside the tile
-  // array of IMDG_TILE_CAP + 4 records, and masked out below)
+  // array of tile_cap + 4 records, and masked out below)
   lds_char* run = (lds_char*)trec + __umul24((unsigned)tb, 12u);
   for (int t0 = 0; t0 < jn; t0 += 4) {
     float3 uj[4];
@@ -636,15 +641,91 @@ __device__ __forceinline__ void list_cell(
       for (int u = 0; u < 4; u++)
         if (band[u])
           in[u] = exact_pair(R, pos, L.i, jb + t0 + u,
-                             wrap_shift(L.cx, (int)code / 9 - 1, R.gdim[0]),
-                             wrap_shift(L.cy, ((int)code / 3) % 3 - 1,
-                                        R.gdim[1]),
-                             wrap_shift(L.cz, (int)code % 3 - 1, R.gdim[2]),
+                             wrap_shift(L.cx, code / 9 - 1, R.gdim[0]),
+                             wrap_shift(L.cy, (code / 3) % 3 - 1, R.gdim[1]),
+                             wrap_shift(L.cz, code % 3 - 1, R.gdim[2]),
                              owner);
     }
 #pragma unroll
     for (int u = 0; u < 4; u++)
-      if (in[u]) list_put(L, code, jb + t0 + u);
+      if (in[u]) list_put<W16>(L, ebase + (uint32_t)(t0 + u));
+  }
+}
+
+// The candidate loops of one atom: the 9 neighbour columns (ox, oy) in a
+// rolled loop, the 3 cells of a column (oz) unrolled, code
+// k = (ox+1)*9 + (oy+1)*3 + (oz+1).  First slot and count of a column's
+// cells are loaded together; a cell beyond an open face has count 0.  Tile
+// positions: a search for the column's centre cell; its z neighbours
+// follow it in slot order (cells are sorted z fastest) and so in the tile,
+// whose runs are laid out in slot order without gaps over the block's
+// neighbour cells, unless the column wraps through a periodic z face.  cb:
+// the chunk's base slot per code (16-bit format).
+template <bool MIXED, bool TOK, bool W16>
+__device__ __forceinline__ void list_atom(const BuildArgs& a, const RepDev& R,
+                                          const double4* __restrict__ pos,
+                                          lds_float3* trec,
+                                          const int4* truns, int nrun,
+                                          const float* off, const int* cb,
+                                          ListLane& L) {
+  const int g0 = R.gdim[0], g1 = R.gdim[1], g2 = R.gdim[2];
+  const bool p0 = R.pbc[0] == 1, p1 = R.pbc[1] == 1, p2 = R.pbc[2] == 1;
+#pragma unroll 1
+  for (int col = 0; col < 9; col++) {
+    int n0 = L.cx + col / 3 - 1, n1 = L.cy + col % 3 - 1;
+    const int sx = n0 < 0 ? -1 : (n0 >= g0 ? 1 : 0);
+    const int sy = n1 < 0 ? -1 : (n1 >= g1 ? 1 : 0);
+    const bool cvalid = ((sx == 0) | p0) & ((sy == 0) | p1);
+    n0 -= sx * g0;
+    n1 -= sy * g1;
+    const int cbase = R.cell_off + (n0 * g1 + n1) * g2;
+    int jb[3], jn[3], tb[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      int n2 = L.cz + q - 1;
+      const int sz = n2 < 0 ? -1 : (n2 >= g2 ? 1 : 0);
+      const bool valid = cvalid & ((sz == 0) | p2);
+      n2 -= sz * g2;
+      const int nc = valid ? cbase + n2 : R.cell_off;
+      jb[q] = a.cell_beg[nc];
+      jn[q] = valid ? a.cell_num[nc] : 0;
+      tb[q] = 0;
+    }
+    if (TOK) {
+      tb[1] = tile_index(truns, nrun, jb[1]);
+#pragma unroll
+      for (int q = 0; q < 3; q += 2) {
+        const int n2 = L.cz + q - 1;
+        if (n2 < 0 || n2 >= g2)
+          tb[q] = jn[q] > 0 ? tile_index(truns, nrun, jb[q]) : 0;
+        else
+          tb[q] = tb[1] + (jb[q] - jb[1]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const int code = col * 3 + q;
+      const uint32_t ebase =
+          W16 ? ((uint32_t)code << IMDG_K16_BITS) | (uint32_t)(jb[q] - cb[code])
+              : (uint32_t)jb[q] | ((uint32_t)code << IMDG_IDX_BITS);
+      list_cell<MIXED, TOK, W16>(a, R, pos, trec, L, jb[q], jn[q], tb[q],
+                                 off[3 * code], off[3 * code + 1],
+                                 off[3 * code + 2], code == 13, code >= 13,
+                                 code, ebase);
+    }
+  }
+  // pad to a multiple of IMDG_LIST_PAD entries with far-away copies of the
+  // atom itself (kmax is a multiple of IMDG_LIST_PAD, so the padded list
+  // fits whenever the list does; the padding completes the last group)
+  const int cnt = L.cnt;
+  if (cnt <= L.kmax) {
+    const uint32_t pad =
+        W16 ? ((uint32_t)IMDG_CODE_PAD << IMDG_K16_BITS) |
+                  (uint32_t)(L.i - cb[IMDG_CODE_PAD])
+            : (uint32_t)L.i | ((uint32_t)IMDG_CODE_PAD << IMDG_IDX_BITS);
+    const int mend = ((cnt + IMDG_LIST_PAD - 1) / IMDG_LIST_PAD) * IMDG_LIST_PAD;
+    for (int m = cnt; m < mend; m++) list_put<W16>(L, pad);
+    L.cnt = cnt;
   }
 }
 
@@ -656,10 +737,15 @@ __device__ __forceinline__ void list_cell(
 // for candidates far from the list radius), so each unordered pair is in
 // the list of both atoms exactly when the reference has it in its half
 // list.  One block per 128-slot block; the FP32 positions of its tile are
-// staged in LDS (12 B each); entries are written four at a time.
+// staged in LDS (12 B each, tile_cap + 4 records of dynamic LDS: the cap
+// trades the blocks that fall back to loads from HBM/L2 against the blocks
+// resident per CU).  The format of a chunk (wavefront) is
+// uniform: 16-bit entries when every neighbour cell run of every code lies
+// within IMDG_K16_SPAN slots of the chunk's base for that code.
 template <bool MIXED>
 __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
-  __shared__ float3 trec[IMDG_TILE_CAP + 4];
+  extern __shared__ __align__(16) unsigned char list_lds[];
+  float3* trec = reinterpret_cast<float3*>(list_lds);
   __shared__ int4 truns[IMDG_MAXRUN];
   __shared__ float off[3 * IMDG_NCODES];
   __shared__ int tinfo[2];
@@ -683,7 +769,7 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
     for (int k = threadIdx.x; k < 3 * IMDG_NCODES; k += blockDim.x)
       off[k] = (float)((&R.offs[0][0])[k]);
     __syncthreads();
-    int nrun = tinfo[0], tok = tinfo[1];
+    const int nrun = tinfo[0], tok = tinfo[1];
     if (tok) {
       int tot = T.natoms;
       // all of a thread's loads in flight together: 8 records per pass
@@ -705,121 +791,82 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
       }
     }
     __syncthreads();
-    int i = b * IMDG_BLOCK + threadIdx.x;
+    const int i = b * IMDG_BLOCK + threadIdx.x;
     const bool mine = i - R.atom_off < R.natoms;
-    {
-      int cx = 0, cy = 0, cz = 0;
-      if (mine) cell_coords(R, a.acell[i], cx, cy, cz);
-      int g0 = R.gdim[0], g1 = R.gdim[1], g2 = R.gdim[2];
-      ListLane L;
-      L.i = i;
-      L.ui = mine ? rec_xyz<MIXED>(a, i) : make_float3(0.f, 0.f, 0.f);
-      float cs = (float)R.cellsz;
-      L.lo = cs * (float)(1.0 - IMDG_SCREEN_BAND);
-      L.hi = cs * (float)(1.0 + IMDG_SCREEN_BAND);
-      L.lst = a.nbr + (((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63)) *
-                          IMDG_GROUP;
-      L.lst16 = const_cast<char*>(list16_of(a.nbr, i, kmax));
-      L.cnt = 0;
-      L.kmax = kmax;
-      L.cx = cx;
-      L.cy = cy;
-      L.cz = cz;
-      // the 27 neighbour cells (code k = (ox+1)*9 + (oy+1)*3 + (oz+1)):
-      // first slot, count, tile position and image shift, all loads in
-      // flight together; a cell beyond an open face has count 0, and so
-      // has every cell of a lane without an atom
-      int jb[27], jn[27], tb[27];
+    int cx = 0, cy = 0, cz = 0;
+    if (mine) cell_coords(R, a.acell[i], cx, cy, cz);
+    // 16-bit format: the chunk's (wavefront's) base slot per code, and
+    // whether every neighbour cell run lies within IMDG_K16_SPAN slots of
+    // it (a lane without an atom has no runs)
+    bool w16 = false;
+    const int w = threadIdx.x / IMDG_WAVE, lane = threadIdx.x & 63;
+    if (a.list16) {
+      const int g0 = R.gdim[0], g1 = R.gdim[1], g2 = R.gdim[2];
+      const bool p0 = R.pbc[0] == 1, p1 = R.pbc[1] == 1, p2 = R.pbc[2] == 1;
+      const int chunk = i >> 6;
+      bool bad = false;
 #pragma unroll
       for (int k = 0; k < 27; k++) {
-        const int ox = k / 9 - 1, oy = (k / 3) % 3 - 1, oz = k % 3 - 1;
-        int n0 = cx + ox, n1 = cy + oy, n2 = cz + oz;
+        int n0 = cx + k / 9 - 1, n1 = cy + (k / 3) % 3 - 1, n2 = cz + k % 3 - 1;
         int sx = n0 < 0 ? -1 : (n0 >= g0 ? 1 : 0);
         int sy = n1 < 0 ? -1 : (n1 >= g1 ? 1 : 0);
         int sz = n2 < 0 ? -1 : (n2 >= g2 ? 1 : 0);
-        bool valid = mine && (sx == 0 || R.pbc[0] == 1) &&
-                     (sy == 0 || R.pbc[1] == 1) && (sz == 0 || R.pbc[2] == 1);
-        n0 -= sx * g0;
-        n1 -= sy * g1;
-        n2 -= sz * g2;
-        int nc = valid ? R.cell_off + (n0 * g1 + n1) * g2 + n2 : R.cell_off;
-        int cb = a.cell_beg[nc], cn = a.cell_num[nc];
-        jb[k] = cb;
-        jn[k] = valid ? cn : 0;
+        bool valid = mine & ((sx == 0) | p0) & ((sy == 0) | p1) & ((sz == 0) | p2);
+        int nc = valid ? R.cell_off + ((n0 - sx * g0) * g1 + n1 - sy * g1) * g2 +
+                             n2 - sz * g2
+                       : R.cell_off;
+        int jb = a.cell_beg[nc], jn = valid ? a.cell_num[nc] : 0;
+        int lo = wave_min_i(jn > 0 ? jb : 0x7fffffff);
+        if (lo == 0x7fffffff) lo = 0;
+        bad |= jn > 0 && (jb < lo || jb + jn - lo > IMDG_K16_SPAN);
+        if (lane == 0) cbase[w][k] = lo;
       }
-      // 16-bit format: the chunk's (wavefront's) base slot per code, and
-      // whether every neighbour cell run lies within 2048 slots of it
-      L.w16 = false;
-      if (a.list16) {
-        const int w = threadIdx.x / IMDG_WAVE, lane = threadIdx.x & 63;
-        const int chunk = i >> 6;
-        // every lane checks its own runs against the base, so a chunk is
-        // stored in 16 bits only where every entry's offset fits
-        bool bad = false;
-#pragma unroll
-        for (int k = 0; k < 27; k++) {
-          int lo = wave_min_i(jn[k] > 0 ? jb[k] : 0x7fffffff);
-          if (lo == 0x7fffffff) lo = 0;
-          bad |= jn[k] > 0 && (jb[k] < lo || jb[k] + jn[k] - lo > IMDG_K16_SPAN);
-          if (lane == 0) cbase[w][k] = lo;
-        }
-        const int wide = __ballot(bad) != 0ull ? 1 : 0;
-        if (lane == 0) {
-          cbase[w][IMDG_CODE_PAD] = chunk * IMDG_CHUNK;
-          cbase[w][IMDG_META_WIDE] = wide;
-        }
-        wave_lds_sync();
-        if (lane <= IMDG_META_WIDE)
-          a.cmeta[(size_t)chunk * IMDG_META + lane] = cbase[w][lane];
-        L.cb = cbase[w];
-        L.w16 = wide == 0;
+      const int wide = __ballot(bad) != 0ull ? 1 : 0;
+      if (lane == 0) {
+        cbase[w][IMDG_CODE_PAD] = chunk * IMDG_CHUNK;
+        cbase[w][IMDG_META_WIDE] = wide;
       }
-      if (!mine) goto list_done;
-      if (tok) {
-        // tile positions: a search for the centre cell of each of the 9
-        // columns; the z neighbours follow it in slot order (cells are
-        // sorted z fastest) and so in the tile, whose runs are laid out in
-        // slot order without gaps over the block's neighbour cells, unless
-        // the column wraps through a periodic z face
-#pragma unroll
-        for (int k = 1; k < 27; k += 3) tb[k] = tile_index(truns, nrun, jb[k]);
-#pragma unroll
-        for (int k = 0; k < 27; k++) {
-          const int oz = k % 3 - 1;
-          if (oz == 0) continue;
-          const int n2 = cz + oz;
-          if (n2 < 0 || n2 >= g2)
-            tb[k] = jn[k] > 0 ? tile_index(truns, nrun, jb[k]) : 0;
-          else
-            tb[k] = tb[k - oz] + (jb[k] - jb[k - oz]);
-        }
-#pragma unroll
-        for (int k = 0; k < 27; k++)
-          list_cell<MIXED, true>(a, R, pos, (lds_float3*)trec, L, jb[k], jn[k], tb[k],
-                                 off[3 * k], off[3 * k + 1], off[3 * k + 2],
-                                 k == 13, k >= 13, (uint32_t)k);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 27; k++)
-          list_cell<MIXED, false>(a, R, pos, (lds_float3*)trec, L, jb[k], jn[k], 0,
-                                  off[3 * k], off[3 * k + 1], off[3 * k + 2],
-                                  k == 13, k >= 13, (uint32_t)k);
-      }
-      // pad to a multiple of IMDG_LIST_PAD entries with far-away copies of
-      // the atom itself (kmax is a multiple of IMDG_LIST_PAD, so the padded
-      // list fits whenever the list does)
-      int cnt = L.cnt;
-      if (cnt <= kmax) {
-        int mend = ((cnt + IMDG_LIST_PAD - 1) / IMDG_LIST_PAD) * IMDG_LIST_PAD;
-        for (int m = cnt; m < mend; m++) {
-          L.cnt = m;
-          list_put(L, IMDG_CODE_PAD, i);
-        }
-      }
-      a.nnb[i] = cnt < kmax ? cnt : kmax;
-      seen = cnt;
+      wave_lds_sync();
+      if (lane <= IMDG_META_WIDE)
+        a.cmeta[(size_t)chunk * IMDG_META + lane] = cbase[w][lane];
+      w16 = __builtin_amdgcn_readfirstlane(wide) == 0;
     }
-  list_done:;
+    if (mine) {
+      ListLane L;
+      L.i = i;
+      L.ui = rec_xyz<MIXED>(a, i);
+      const float cs = (float)R.cellsz;
+      L.lo = cs * (float)(1.0 - IMDG_SCREEN_BAND);
+      L.hi = cs * (float)(1.0 + IMDG_SCREEN_BAND);
+      L.cnt = 0;
+      L.kmax = kmax;
+      L.g0 = L.g1 = L.g2 = L.g3 = 0u;
+      L.cx = cx;
+      L.cy = cy;
+      L.cz = cz;
+      const int* cb = cbase[w];
+      if (w16) {
+        L.lst = const_cast<char*>(list16_of(a.nbr, i, kmax));
+        if (tok)
+          list_atom<MIXED, true, true>(a, R, pos, (lds_float3*)trec, truns,
+                                       nrun, off, cb, L);
+        else
+          list_atom<MIXED, false, true>(a, R, pos, (lds_float3*)trec, truns,
+                                        nrun, off, cb, L);
+      } else {
+        L.lst = reinterpret_cast<char*>(
+            a.nbr + (((size_t)(i >> 6) * (kmax >> 2)) * IMDG_WAVE + (i & 63)) *
+                        IMDG_GROUP);
+        if (tok)
+          list_atom<MIXED, true, false>(a, R, pos, (lds_float3*)trec, truns,
+                                        nrun, off, cb, L);
+        else
+          list_atom<MIXED, false, false>(a, R, pos, (lds_float3*)trec, truns,
+                                         nrun, off, cb, L);
+      }
+      a.nnb[i] = L.cnt < kmax ? L.cnt : kmax;
+      seen = L.cnt;
+    }
   }
   // block max of the list length (k_list_done reduces the blocks: one
   // atomic word updated by every block would serialise them)
@@ -839,17 +886,30 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
 // buffers), otherwise clear the rebuild flags and flip the buffers that
 // k_permute filled.
 __global__ void __launch_bounds__(1024) k_list_done(BuildArgs a) {
-  __shared__ int wmax[1024 / IMDG_WAVE];
+  __shared__ int wmax[1024 / IMDG_WAVE], wfb[1024 / IMDG_WAVE];
   if (!a.ctl->any_rebuild || a.ctl->frozen) return;
-  int m = 0;
+  int m = 0, fb = 0;
   for (int b = threadIdx.x; b < a.nblk; b += blockDim.x)
-    if (a.rst[a.blk_rep[b]].need_rebuild) m = max(m, a.blkmax[b]);
-  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, IMDG_WAVE));
-  if ((threadIdx.x % IMDG_WAVE) == 0) wmax[threadIdx.x / IMDG_WAVE] = m;
+    if (a.rst[a.blk_rep[b]].need_rebuild) {
+      m = max(m, a.blkmax[b]);
+      fb += a.tiles[b].ok ? 0 : 1;
+    }
+  for (int o = 32; o > 0; o >>= 1) {
+    m = max(m, __shfl_xor(m, o, IMDG_WAVE));
+    fb += __shfl_xor(fb, o, IMDG_WAVE);
+  }
+  if ((threadIdx.x % IMDG_WAVE) == 0) {
+    wmax[threadIdx.x / IMDG_WAVE] = m;
+    wfb[threadIdx.x / IMDG_WAVE] = fb;
+  }
   __syncthreads();
   if (threadIdx.x != 0) return;
-  for (int k = 1; k < (int)(blockDim.x / IMDG_WAVE); k++) m = max(m, wmax[k]);
+  for (int k = 1; k < (int)(blockDim.x / IMDG_WAVE); k++) {
+    m = max(m, wmax[k]);
+    fb += wfb[k];
+  }
   m = max(m, wmax[0]);
+  a.ctl->tiles_fallback += fb;
   if (m > a.ctl->kmax_seen) a.ctl->kmax_seen = m;
   if (m > a.kmax) {
     // keep the rebuild flags: the host grows the list and builds again
@@ -902,12 +962,13 @@ void launch_build(const BuildArgs& b, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_place, dim3(b.nblk), dim3(IMDG_BLOCK), 0, st, b);
   hipLaunchKernelGGL(k_cellsort, dim3(grid), dim3(256), 0, st, b);
   hipLaunchKernelGGL(k_permute, dim3(b.nblk), dim3(IMDG_BLOCK), 0, st, b);
+  const size_t tile_lds = (size_t)(b.tile_cap + 4) * sizeof(float3);
   if (b.mixed)
-    hipLaunchKernelGGL((k_list<true>), dim3(b.nblk), dim3(IMDG_BLOCK), 0, st,
-                       b);
+    hipLaunchKernelGGL((k_list<true>), dim3(b.nblk), dim3(IMDG_BLOCK),
+                       tile_lds, st, b);
   else
-    hipLaunchKernelGGL((k_list<false>), dim3(b.nblk), dim3(IMDG_BLOCK), 0, st,
-                       b);
+    hipLaunchKernelGGL((k_list<false>), dim3(b.nblk), dim3(IMDG_BLOCK),
+                       tile_lds, st, b);
   hipLaunchKernelGGL(k_list_done, dim3(1), dim3(1024), 0, st, b);
 }
 
