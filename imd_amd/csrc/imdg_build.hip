@@ -33,6 +33,10 @@ __device__ __forceinline__ int block_replica(const int* blk_rep) {
 // order inside a cell is fixed later by k_cellsort).
 #define BIN_HASH 256
 
+// Flag bit of a block's list-length maximum (blkmax): its tile exceeded the
+// tile cap and the list build read its candidates from HBM/L2.
+#define IMDG_BLK_FALLBACK (1 << 30)
+
 __global__ void __launch_bounds__(IMDG_BLOCK) k_bin(BuildArgs a) {
 #pragma clang fp contract(off)
   __shared__ int hkey[BIN_HASH], hcnt[BIN_HASH], hbase[BIN_HASH];
@@ -868,16 +872,17 @@ __global__ void __launch_bounds__(IMDG_BLOCK) k_list(BuildArgs a) {
       seen = L.cnt;
     }
   }
-  // block max of the list length (k_list_done reduces the blocks: one
+    // block max of the list length (k_list_done reduces the blocks: one
   // atomic word updated by every block would serialise them)
   for (int o = 32; o > 0; o >>= 1)
     seen = max(seen, __shfl_xor(seen, o, IMDG_WAVE));
   if ((threadIdx.x % IMDG_WAVE) == 0) wmax[threadIdx.x / IMDG_WAVE] = seen;
   __syncthreads();
+  // with bit 30: the block's tile fell back to HBM/L2 loads
   if (threadIdx.x == 0) {
     int m = wmax[0];
     for (int k = 1; k < IMDG_NWAVE; k++) m = max(m, wmax[k]);
-    a.blkmax[b] = m;
+    a.blkmax[b] = m | (flagged && !tinfo[1] ? IMDG_BLK_FALLBACK : 0);
   }
 }
 
@@ -891,8 +896,9 @@ __global__ void __launch_bounds__(1024) k_list_done(BuildArgs a) {
   int m = 0, fb = 0;
   for (int b = threadIdx.x; b < a.nblk; b += blockDim.x)
     if (a.rst[a.blk_rep[b]].need_rebuild) {
-      m = max(m, a.blkmax[b]);
-      fb += a.tiles[b].ok ? 0 : 1;
+      const int v = a.blkmax[b];
+      m = max(m, v & ~IMDG_BLK_FALLBACK);
+      fb += v & IMDG_BLK_FALLBACK ? 1 : 0;
     }
   for (int o = 32; o > 0; o >>= 1) {
     m = max(m, __shfl_xor(m, o, IMDG_WAVE));

@@ -678,6 +678,15 @@ __device__ __forceinline__ void list_walk16(const uint16_t* w, int nnb,
   }
 }
 
+// Record j of an FP32 record array, addressed by a 32-bit byte offset from
+// the array: the load takes the array's address from scalar registers and
+// the offset from one vector register (no 64-bit address add per gather).
+__device__ __forceinline__ float4 rec_at(const float4* __restrict__ rec,
+                                         uint32_t j) {
+  return *reinterpret_cast<const float4*>(
+      reinterpret_cast<const char*>(rec) + j * 16u);
+}
+
 // Cell offset code and neighbour slot of a list entry in either format
 // (16-bit: the base slot of the entry's code is the w of the wavefront's
 // offset row, set per chunk); o receives the code's offset row.
@@ -835,7 +844,11 @@ __device__ __forceinline__ void density_group(
 // FP32 single-species variant of density_group on the rho segment
 // polynomials DQ[k] = (p0, A, B, 0): value p0 + chi (A + B chi); the table
 // argument is clamped to [0, len - 1/2] (med3), so r^2 at or beyond end
-// reads the zero segment len - 1 and no cutoff test is needed.
+// reads the zero segment len - 1 and no cutoff test is needed.  r^2 keeps
+// the offset remainder (sep_r2): without it every rho shifts by a uniform
+// ~1e-7 (relative), and through F' the virial by ~1e-4.  Records are
+// addressed by a 32-bit byte offset from the array (scalar base + vector
+// offset, no 64-bit address add).
 template <typename PF>
 __device__ __forceinline__ void density_group_poly(
     uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
@@ -848,8 +861,9 @@ __device__ __forceinline__ void density_group_poly(
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     // the type is not needed: gather 12 B
-    const int j = entry_slot<PF::W16>(en[u], off, o[u]);
-    const float* p = reinterpret_cast<const float*>(rec + j);
+    const uint32_t j = (uint32_t)entry_slot<PF::W16>(en[u], off, o[u]);
+    const float* p = reinterpret_cast<const float*>(
+        reinterpret_cast<const char*>(rec) + j * 16u);
     float2 xy = *reinterpret_cast<const float2*>(p);
     ux[u] = xy.x;
     uy[u] = xy.y;
@@ -1092,7 +1106,7 @@ __device__ __forceinline__ void force_group_sgf(
   float4 uj[4], o[4], ol[4];
 #pragma unroll
   for (int u = 0; u < 4; u++)
-    uj[u] = rec[entry_slot<PF::W16>(en[u], off, o[u])];
+    uj[u] = rec_at(rec, (uint32_t)entry_slot<PF::W16>(en[u], off, o[u]));
   __builtin_amdgcn_sched_barrier(0);
   slot = pf.load();
 #pragma unroll
@@ -1149,7 +1163,7 @@ __device__ __forceinline__ void force_group_sgn(
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int j = entry_slot<PF::W16>(en[u], off, o[u]);
-    uj[u] = rec[j];
+    uj[u] = rec_at(rec, (uint32_t)j);
     jt[u] = atype[j];
   }
   __builtin_amdgcn_sched_barrier(0);
@@ -1201,7 +1215,7 @@ __device__ __forceinline__ void density_group_polyn(
   float4 uj[4], o[4], ol[4];
 #pragma unroll
   for (int u = 0; u < 4; u++)
-    uj[u] = rec[entry_slot<PF::W16>(en[u], off, o[u])];
+    uj[u] = rec_at(rec, (uint32_t)entry_slot<PF::W16>(en[u], off, o[u]));
   __builtin_amdgcn_sched_barrier(0);
   slot = pf.load();
 #pragma unroll
