@@ -81,8 +81,9 @@ struct imdg_ctx {
   bool cubic = false;   // 4-point or spline tables (cubic segments)
   bool eeam = false;    // EEAM energy term present (tab[3])
   int xcd_remap = 1;
-  int tile_cap = IMDG_TILE_CAP;  // records of a list-build tile (a smaller
-                                 // cap sends more blocks to the fallback)
+  int tile_cap = IMDG_TILE_DEFAULT;  // records of a list-build tile (a
+                                     // smaller cap sends more blocks to the
+                                     // fallback)
   // tables
   HostTable tab[4];  // pair, rho, embed, EEAM energy term (emod)
   bool tables_set = false;

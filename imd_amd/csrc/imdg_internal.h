@@ -66,7 +66,13 @@
 #define IMDG_K16_SPAN (1 << IMDG_K16_BITS)
 #define IMDG_META 32          // ints of chunk metadata: bases, wide flag
 #define IMDG_META_WIDE 28
-#define IMDG_TILE_CAP 2048    // records of one block's LDS tile
+#define IMDG_TILE_CAP 2048    // largest tile cap (records of one block's
+                              // LDS tile, IMDG_TILE_LIMIT)
+#define IMDG_TILE_DEFAULT 1536  // default cap: on the 2M-atom EAM crystal
+                                // (tiles of ~1,400-1,900 records) the list
+                                // build is fastest here, with 4 resident
+                                // blocks per CU instead of 3 and about 40%
+                                // of the blocks reading from HBM/L2
 #define IMDG_MAXRUN 40        // slot runs of one tile
 
 // FP64 block partials written by the force/move kernels

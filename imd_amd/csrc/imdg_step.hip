@@ -1255,8 +1255,9 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
   typedef typename V4<R>::T R4;
   typedef typename V2<R>::T R2;
   extern __shared__ __align__(16) unsigned char dyn_lds[];
-  __shared__ R4 woff[IMDG_STEP_WAVES][IMDG_NCODES];
-  __shared__ R4 wolo[IMDG_STEP_WAVES][IMDG_NCODES];
+  // per wavefront: the offset row, then its remainder row (one address
+  // register serves both: the remainder is at a constant distance)
+  __shared__ R4 wofs[IMDG_STEP_WAVES][2][IMDG_NCODES];
   __shared__ ColPar<R> cpr[IMDG_MAXCOLS];
   if (a.ctl->frozen) return;
   const TabDev<R>& tr = tabs[1];
@@ -1275,8 +1276,8 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
   Sched S = make_sched(a.nchunk, a.xcd_remap);
   const int lane = threadIdx.x & (IMDG_WAVE - 1);
   const int ga = lane / L, gs = lane % L;
-  R4* off = woff[threadIdx.x / IMDG_WAVE];
-  R4* olo = wolo[threadIdx.x / IMDG_WAVE];
+  R4* off = wofs[threadIdx.x / IMDG_WAVE][0];
+  R4* olo = wofs[threadIdx.x / IMDG_WAVE][1];
   int cur = -1, is_short = 0;
   double acc = 0.0;
   for (int c = S.c; c < S.c1; c += S.stride) {
@@ -1450,8 +1451,9 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
   typedef typename V4<R>::T R4;
   typedef typename V2<R>::T R2;
   extern __shared__ __align__(16) unsigned char dyn_lds[];
-  __shared__ R4 woff[IMDG_STEP_WAVES][IMDG_NCODES];
-  __shared__ R4 wolo[IMDG_STEP_WAVES][IMDG_NCODES];
+  // per wavefront: the offset row, then its remainder row (one address
+  // register serves both: the remainder is at a constant distance)
+  __shared__ R4 wofs[IMDG_STEP_WAVES][2][IMDG_NCODES];
   __shared__ ColPar<R> cpp[IMDG_MAXCOLS], cpr[IMDG_MAXCOLS];
   if (a.ctl->frozen) return;
   const TabDev<R>& tp = tabs[0];
@@ -1480,8 +1482,8 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
   Sched S = make_sched(a.nchunk, a.xcd_remap);
   const int lane = threadIdx.x & (IMDG_WAVE - 1);
   const int ga = lane / L, gs = lane % L;
-  R4* off = woff[threadIdx.x / IMDG_WAVE];
-  R4* olo = wolo[threadIdx.x / IMDG_WAVE];
+  R4* off = wofs[threadIdx.x / IMDG_WAVE][0];
+  R4* olo = wofs[threadIdx.x / IMDG_WAVE][1];
   int cur = -1, is_short = 0;
   double acc[PB_N] = {0, 0, 0, 0, 0, 0};
   for (int c = S.c; c < S.c1; c += S.stride) {
@@ -2036,10 +2038,14 @@ hipError_t launch_move(const StepArgs<R>& a, int grid, hipStream_t st) {
 template <typename R>
 hipError_t launch_finalize(const StepArgs<R>& a, bool forces, bool moved,
                            hipStream_t st) {
-  // one block per IMDG_STEP_WAVES replicas (one wavefront each)
+  // several replicas: one block per IMDG_STEP_WAVES replicas (one wavefront
+  // each); one replica: one wavefront (its lanes sum the block partials, and
+  // no combine over wavefronts follows: the end of the step is a chain of
+  // dependent operations, and a shorter chain ends it sooner)
   int nblk = a.part_chunk ? (a.nrep + IMDG_STEP_WAVES - 1) / IMDG_STEP_WAVES : 1;
-  hipLaunchKernelGGL((k_finalize<R>), dim3(nblk), dim3(IMDG_STEP_THREADS), 0,
-                     st, a, forces ? 1 : 0, moved ? 1 : 0);
+  int threads = a.part_chunk ? IMDG_STEP_THREADS : IMDG_WAVE;
+  hipLaunchKernelGGL((k_finalize<R>), dim3(nblk), dim3(threads), 0, st, a,
+                     forces ? 1 : 0, moved ? 1 : 0);
   return hipGetLastError();
 }
 

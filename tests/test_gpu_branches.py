@@ -57,7 +57,10 @@ def test_tile_fallback(precision, monkeypatch):
         a_low = low.atoms()
         low.make_nblist()
         n_fb = low.info()["tiles_fallback"]
-    with G.eam_context(eam, [s], precision=precision) as full:
+    monkeypatch.setenv("IMDG_TILE_LIMIT", "2048")     # the largest cap
+    full = G.eam_context(eam, [s], precision=precision)
+    monkeypatch.delenv("IMDG_TILE_LIMIT")
+    with full:
         full.calc_forces()
         assert full.info()["tiles_fallback"] == 0
         full.run(30)
