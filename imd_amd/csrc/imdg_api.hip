@@ -722,6 +722,10 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.knots64 = (const double2*)c->d_knots64;
   a.cmeta = c->d_cmeta;
   a.list16 = c->list16 ? 1 : 0;
+  a.type_bit = (c->mixed && c->cfg.ntypes == 2 && c->tab[1].present &&
+                c->force_mode == 3 && !c->cubic && !c->eeam)
+                   ? 1
+                   : 0;
   a.tab64 = (const Tab64*)c->d_tab64;
   a.joint = c->knots_joint;
   a.dpoly = c->knots_dpoly;

@@ -261,6 +261,9 @@ struct StepArgs {
   R* eedm;                // EEAM 2E'(P) per slot (read by the force loop)
   const int* cmeta;       // chunk metadata of the 16-bit list format
   int list16;             // the list may hold 16-bit chunks
+  int type_bit;           // two species, FP32 shared-grid records: the
+                          // neighbour's type is the lowest mantissa bit of
+                          // its 2F' (rec_b.w), not a gather of atype
   const double2* knots64; // FP64 knots of the embedding (tab64[0]) and the
   const Tab64* tab64;     // EEAM energy term (tab64[1]); FP32 kernels only
   int lpa_den, lpa_frc;   // lanes per slot of the density / force loops

@@ -678,6 +678,21 @@ __device__ __forceinline__ void list_walk16(const uint16_t* w, int nnb,
   }
 }
 
+// 2F' with an atom's type (0 or 1) in the lowest mantissa bit, for the
+// two-species force loop: of the FP32 values whose lowest bit is the type,
+// the one nearest the FP64 value (v itself, or its neighbour on the FP64
+// value's side: at most one unit in the last place, 1.2e-7 relative, and
+// as often up as down).  The force loop reads the type back from the bit.
+__device__ __forceinline__ float fold_type_bit(float v, double exact,
+                                               int type) {
+  const uint32_t b = __float_as_uint(v);
+  if ((b & 1u) == (uint32_t)type) return v;
+  if ((b & 0x7fffffffu) == 0u) return __uint_as_float(b + 1u);  // +-0
+  // b + 1 is the neighbour away from zero, b - 1 the one towards zero
+  const float up = __uint_as_float(b + 1u), dn = __uint_as_float(b - 1u);
+  return fabs((double)up - exact) <= fabs((double)dn - exact) ? up : dn;
+}
+
 // Record j of an FP32 record array, addressed by a 32-bit byte offset from
 // the array: the load takes the array's address from scalar registers and
 // the offset from one vector register (no 64-bit address add per gather).
@@ -1149,7 +1164,7 @@ __device__ __forceinline__ void force_group_sgf(
 // column p and (A, D) of rho columns p and jt*nt + it; gradients in units
 // of 1/(2 istep) and the virial's A_lo sum as in force_group_sgf.  rho'_j (column p) goes with 2F'_i,
 // rho'_i (column jt*nt + it) with 2F'_j (src/imd_forces_nbl.c:1156-1203).
-template <typename PF>
+template <bool TB, typename PF>
 __device__ __forceinline__ void force_group_sgn(
     uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
     const signed char* __restrict__ atype, const float4* off,
@@ -1164,7 +1179,11 @@ __device__ __forceinline__ void force_group_sgn(
   for (int u = 0; u < 4; u++) {
     const int j = entry_slot<PF::W16>(en[u], off, o[u]);
     uj[u] = rec_at(rec, (uint32_t)j);
-    jt[u] = atype[j];
+    if (!TB) jt[u] = atype[j];
+  }
+  if (TB) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) jt[u] = (int)(__float_as_uint(uj[u].w) & 1u);
   }
   __builtin_amdgcn_sched_barrier(0);
   slot = pf.load();
@@ -1413,6 +1432,8 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
       rb.y = ui.y;
       rb.z = ui.z;
       rb.w = dF;
+      if constexpr (sizeof(R) == 4)
+        if (a.type_bit) rb.w = fold_type_bit(dF, dFd, it);
       a.rec_b[i] = rb;
       if (a.store) {
         a.rho[i] = rho;
@@ -1514,13 +1535,22 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
           const float4* __restrict__ S =
               reinterpret_cast<const float4*>(a.knots + a.jointn);
           const TArg ta = targ(tp);
-          walk_either<L>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
-                         [&](uint4 e4, auto pf, uint4& slot) {
-                           force_group_sgn(e4, pf, slot, rec, atype, off, olo,
-                                           ui, 0.5f * dFi, S, it * nt, plen,
-                                           ta, fx, fy, fz, ep, vir, vlo,
-                                           r2min);
-                         });
+          if (a.type_bit)
+            walk_either<L>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
+                           [&](uint4 e4, auto pf, uint4& slot) {
+                             force_group_sgn<true>(
+                                 e4, pf, slot, rec, atype, off, olo, ui,
+                                 0.5f * dFi, S, it * nt, plen, ta, fx, fy, fz,
+                                 ep, vir, vlo, r2min);
+                           });
+          else
+            walk_either<L>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
+                           [&](uint4 e4, auto pf, uint4& slot) {
+                             force_group_sgn<false>(
+                                 e4, pf, slot, rec, atype, off, olo, ui,
+                                 0.5f * dFi, S, it * nt, plen, ta, fx, fy, fz,
+                                 ep, vir, vlo, r2min);
+                           });
           is_short |= (r2min < pb);
         } else if constexpr (MODE == FM_SG && !CUB && !EE) {
           // one grid: sums in units of 1/(2 istep), scaled once per slot
