@@ -604,12 +604,13 @@ typedef __attribute__((address_space(3))) const float lds_float;
 // per four candidates; the band is rarely hit), append.  The separation is
 // u_j + (o - u_i), with o - u_i formed once per cell: two roundings of a
 // few Angstrom, far inside the screening band.  Candidate t of the cell is
-// entry ebase + t.
-template <bool MIXED, bool TOK, bool W16>
+// entry ebase + t.  SELF: the atom's own cell (code 13), where the atom is
+// a candidate of itself.
+template <bool MIXED, bool TOK, bool W16, bool SELF>
 __device__ __forceinline__ void list_cell(
     const BuildArgs& a, const RepDev& R, const double4* __restrict__ pos,
     lds_float3* trec, ListLane& L, int jb, int jn, int tb, float o0,
-    float o1, float o2, bool self, bool owner, int code, uint32_t ebase) {
+    float o1, float o2, bool owner, int code, uint32_t ebase) {
   const float w0 = o0 - L.ui.x, w1 = o1 - L.ui.y, w2 = o2 - L.ui.z;
   // the cell's run in the tile: four records per pass at immediate offsets
   // from one address (up to three records past the run: inside the tile
@@ -630,7 +631,7 @@ __device__ __forceinline__ void list_cell(
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       int j = jb + t0 + u;
-      bool valid = (t0 + u < jn) && !(self && j == L.i);
+      bool valid = (t0 + u < jn) && !(SELF && j == L.i);
       float d0 = uj[u].x + w0;
       float d1 = uj[u].y + w1;
       float d2 = uj[u].z + w2;
@@ -712,10 +713,14 @@ __device__ __forceinline__ void list_atom(const BuildArgs& a, const RepDev& R,
       const uint32_t ebase =
           W16 ? ((uint32_t)code << IMDG_K16_BITS) | (uint32_t)(jb[q] - cb[code])
               : (uint32_t)jb[q] | ((uint32_t)code << IMDG_IDX_BITS);
-      list_cell<MIXED, TOK, W16>(a, R, pos, trec, L, jb[q], jn[q], tb[q],
-                                 off[3 * code], off[3 * code + 1],
-                                 off[3 * code + 2], code == 13, code >= 13,
-                                 code, ebase);
+      if (q == 1 && col == 4)  // code 13 (q is a constant here)
+        list_cell<MIXED, TOK, W16, true>(
+            a, R, pos, trec, L, jb[q], jn[q], tb[q], off[3 * code],
+            off[3 * code + 1], off[3 * code + 2], true, code, ebase);
+      else
+        list_cell<MIXED, TOK, W16, false>(
+            a, R, pos, trec, L, jb[q], jn[q], tb[q], off[3 * code],
+            off[3 * code + 1], off[3 * code + 2], code >= 13, code, ebase);
     }
   }
   // pad to a multiple of IMDG_LIST_PAD entries with far-away copies of the
