@@ -297,6 +297,41 @@ def precision_errors(G, eam, s):
     return state_errors(G, eam, s)
 
 
+def rebuild_probe(args):
+    """Host-timed cost of the list rebuild on the 2M-atom crystal: steps of
+    the crystal at 1 K (no skin test fires, so the rebuild launches of every
+    step return at once) and forced full rebuilds (make_nblist: the whole
+    chain, then a host synchronisation).  One JSON line."""
+    from imd_amd import imdg as G
+    eam, s = make_workload(args.cells, args.seed, 1.0)
+    ctx = G.eam_context(eam, [s], precision=G.MIXED, ensemble=G.NVE,
+                        timestep=W.FS, nbl_margin=0.4)
+    ctx.run(args.warmup)
+    tuning = ctx.tuning() if args.no_autotune else ctx.autotune(reps=10)
+    ctx.run(2)
+    nb0 = ctx.info()["nbl_count"]
+    t0 = time.perf_counter()
+    ctx.run(args.steps)
+    t_step = (time.perf_counter() - t0) / args.steps
+    nb1 = ctx.info()["nbl_count"]
+    ctx.make_nblist()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.make_nblist()
+    t_build = (time.perf_counter() - t0) / reps
+    ctx.close()
+    print(json.dumps({"metric": "list rebuild cost (host-timed)",
+                      "config": {"workload": "fcc EAM crystal %d^3 cells "
+                                 "(%d atoms) at 1 K, mixed" % (args.cells, s.n),
+                                 "steps": args.steps},
+                      "launch_shape": _shape(tuning),
+                      "step_us_without_rebuild": 1e6 * t_step,
+                      "rebuilds_during_steps": int(nb1 - nb0),
+                      "full_rebuild_us": 1e6 * t_build}))
+    return 0
+
+
 def precision_study(args):
     """Per-quantity error of the mixed path against the FP64 path along the
     2M-atom NVT run: the mixed run advances args.precision_study steps, and
@@ -417,7 +452,7 @@ def config_run(args):
             else:
                 os.environ["IMDG_GRAPH"] = old
         ctx.run(args.warmup)
-        tuning = ctx.tuning() if args.no_autotune else ctx.autotune(reps=3)
+        tuning = ctx.tuning() if args.no_autotune else ctx.autotune(reps=10)
         ctx.run(2)  # capture the step with the chosen shapes
         nb0 = ctx.info()["nbl_count"]
         ctx.sync()
@@ -485,6 +520,9 @@ def main(argv=None):
                     help="only sample the mixed-vs-FP64 errors along STEPS "
                          "steps of the NVT run and exit")
     ap.add_argument("--sample-every", type=int, default=1000)
+    ap.add_argument("--rebuild-probe", action="store_true",
+                    help="host-timed steps without rebuilds and forced full "
+                         "rebuilds of the 2M-atom crystal")
     ap.add_argument("--config", choices=sorted(CONFIGS), default=None,
                     help="measure one of the other configurations instead "
                          "of the 2M-atom metric workload")
@@ -504,6 +542,8 @@ def main(argv=None):
         return drift_run(args)
     if args.precision_study:
         return precision_study(args)
+    if args.rebuild_probe:
+        return rebuild_probe(args)
     if args.config:
         return config_run(args)
 
@@ -532,7 +572,7 @@ def main(argv=None):
         ctx.run(args.warmup)
         # launch shapes: timed on the warmed-up state (which it leaves as it
         # is), before the timed region
-        tuning = ctx.tuning() if args.no_autotune else ctx.autotune(reps=3)
+        tuning = ctx.tuning() if args.no_autotune else ctx.autotune(reps=10)
         nb0 = ctx.info()["nbl_count"]
         # the timed region records HIP events around the force launches only
         # (the roofline kernel); the other classes are timed afterwards

@@ -81,6 +81,8 @@ struct imdg_ctx {
   bool cubic = false;   // 4-point or spline tables (cubic segments)
   bool eeam = false;    // EEAM energy term present (tab[3])
   int xcd_remap = 1;
+  int list_nt = -1;  // list words non-temporal: -1 by list size
+                     // (IMDG_LIST_NT_BYTES), 0 never, 1 always (IMDG_LIST_NT)
   int tile_cap = IMDG_TILE_DEFAULT;  // records of a list-build tile (a
                                      // smaller cap sends more blocks to the
                                      // fallback)
@@ -114,6 +116,7 @@ struct imdg_ctx {
   Tile* d_tiles = nullptr;
   uint32_t* d_nbr = nullptr;
   size_t nbr_bytes = 0;
+  int list_k = 0;  // entries of the longest list at the last checked build
   int* d_nnb = nullptr;
   void *d_reca = nullptr, *d_recb = nullptr;
   void *d_rho = nullptr, *d_epot = nullptr, *d_frc = nullptr;
@@ -722,6 +725,10 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.knots64 = (const double2*)c->d_knots64;
   a.cmeta = c->d_cmeta;
   a.list16 = c->list16 ? 1 : 0;
+  const size_t list_read =
+      (size_t)c->nslots * std::min(c->list_k, c->kmax) * (c->list16 ? 2 : 4);
+  a.list_nt = c->list_nt < 0 ? (list_read > IMDG_LIST_NT_BYTES ? 1 : 0)
+                             : c->list_nt;
   a.type_bit = (c->mixed && c->cfg.ntypes == 2 && c->tab[1].present &&
                 c->force_mode == 3 && !c->cubic && !c->eeam)
                    ? 1
@@ -980,6 +987,7 @@ int fix_overflow(imdg_ctx* c, int* fixed) {
   Ctl h;
   int rc = read_ctl(c, &h);
   if (rc) return rc;
+  c->list_k = h.kmax_seen;
   if (!h.frozen) return IMDG_OK;
   int newk = std::max(c->kmax + IMDG_LIST_PAD, (int)ceil(h.kmax_seen * 1.2));
   newk = (newk + IMDG_LIST_PAD - 1) / IMDG_LIST_PAD * IMDG_LIST_PAD;
@@ -1243,6 +1251,7 @@ int imdg_ctx_create(const imdg_config* cfg, imdg_ctx** out) {
   if (const char* e = getenv("IMDG_SPLIT_MOVE")) c->split_move = atoi(e) != 0;
   if (const char* e = getenv("IMDG_TILE_LIMIT"))
     c->tile_cap = std::max(0, std::min(atoi(e), IMDG_TILE_CAP));
+  if (const char* e = getenv("IMDG_LIST_NT")) c->list_nt = atoi(e) != 0;
   if (const char* e = getenv("IMDG_FINAL_KERNEL"))
     c->final_inline = atoi(e) == 0;
   if (cfg->lanes_per_atom > 0)

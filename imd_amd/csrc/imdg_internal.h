@@ -66,6 +66,12 @@
 #define IMDG_K16_SPAN (1 << IMDG_K16_BITS)
 #define IMDG_META 32          // ints of chunk metadata: bases, wide flag
 #define IMDG_META_WIDE 28
+// Lists whose words read per step may exceed this (slots x the longest
+// list of the last build x 2 or 4 bytes) are loaded non-temporally by the
+// FP32 neighbour loops: half of MI355X's 256 MB last-level cache, which
+// keeps a smaller list (and the step's other data) from one step to the
+// next.
+#define IMDG_LIST_NT_BYTES (128ull << 20)
 #define IMDG_TILE_CAP 2048    // largest tile cap (records of one block's
                               // LDS tile, IMDG_TILE_LIMIT)
 #define IMDG_TILE_DEFAULT 1536  // default cap: on the 2M-atom EAM crystal
@@ -261,6 +267,8 @@ struct StepArgs {
   R* eedm;                // EEAM 2E'(P) per slot (read by the force loop)
   const int* cmeta;       // chunk metadata of the 16-bit list format
   int list16;             // the list may hold 16-bit chunks
+  int list_nt;            // load the list words non-temporally (a list
+                          // read larger than IMDG_LIST_NT_BYTES)
   int type_bit;           // two species, FP32 shared-grid records: the
                           // neighbour's type is the lowest mantissa bit of
                           // its 2F' (rec_b.w), not a gather of atype

@@ -49,11 +49,15 @@ enum {
 // ---------------------------------------------------------------------------
 
 // The list words (read once per launch, never gathered) are loaded
-// non-temporally, so they pass the L2 without pushing out the neighbour
-// records that the loops gather many times (force 258 -> 252 us, density
-// 194 -> 189 us on the 2M-atom crystal).  Positions and momenta keep the
-// default policy: marked non-temporal they reach the next step's launch
-// from HBM instead of the last-level cache (force 252 -> 265 us).
+// non-temporally when the list is larger than the last-level cache can
+// keep between steps (list_nt; the FP32 one-grid loops have an instance
+// for each policy): they then pass the L2 without pushing out the neighbour
+// records that the loops gather many times (2M-atom crystal, 0.3 GB of
+// list: force 258 -> 252 us, density 194 -> 189 us).  A smaller list is
+// read again from the last-level cache every step, and non-temporal loads
+// would send it to HBM (256,000 atoms: 9% slower).  Positions and momenta
+// keep the default policy: marked non-temporal they reach the next step's
+// launch from HBM (force 252 -> 265 us).
 typedef unsigned int nt_u2 __attribute__((ext_vector_type(2)));
 typedef unsigned int nt_u4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ uint2 nt_load2(const void* p) {
@@ -564,12 +568,21 @@ __device__ __forceinline__ R lane_group_min(R v) {
 // Four list entries of one lane: with L lanes per slot, one body covers L
 // groups of the grouped layout (4L entries of the slot); lane s of the slot
 // reads words s*(4/L) .. s*(4/L) + 4/L - 1 of each of them.
-template <int L>
+template <int L, bool NT = false>
 struct ListRef {
   static constexpr bool W16 = false;
   const uint32_t* p;  // the lane's first word of the body
   __device__ __forceinline__ uint4 load() const {
     constexpr int G = IMDG_WAVE * IMDG_GROUP;  // words per group
+    if (!NT) {  // NT: non-temporal loads (the list_nt policy)
+      if (L == 1) return *reinterpret_cast<const uint4*>(p);
+      if (L == 2) {
+        uint2 x = *reinterpret_cast<const uint2*>(p);
+        uint2 y = *reinterpret_cast<const uint2*>(p + G);
+        return make_uint4(x.x, x.y, y.x, y.y);
+      }
+      return make_uint4(p[0], p[G], p[2 * G], p[3 * G]);
+    }
     if (L == 1) return nt_load4(p);
     if (L == 2) {
       uint2 x = nt_load2(p);
@@ -608,24 +621,24 @@ __device__ __forceinline__ void ready(const uint4& e) {
 // far-away entries to a multiple of 16, so with 1 or 2 lanes per slot the
 // body count is rounded up to even inside the padding and B always runs;
 // with 4 lanes a body is 16 entries and B past the last body is skipped.
-template <int L, typename F>
+template <int L, bool NT = false, typename F>
 __device__ __forceinline__ void list_walk(const uint32_t* w, int nnb, F body) {
   constexpr int E = IMDG_GROUP * L;                    // entries per body
   constexpr size_t BW = (size_t)L * IMDG_WAVE * IMDG_GROUP;  // words per body
   int nbody = (nnb + E - 1) / E;
   if (L <= 2) nbody = (nbody + 1) & ~1;
   const int blast = nbody > 0 ? nbody - 1 : 0;
-  uint4 A = ListRef<L>{w}.load();
-  uint4 B = ListRef<L>{w + (size_t)min(1, blast) * BW}.load();
+  uint4 A = ListRef<L, NT>{w}.load();
+  uint4 B = ListRef<L, NT>{w + (size_t)min(1, blast) * BW}.load();
   A = make_uint4(nbody > 0 ? A.x : 0u, nbody > 0 ? A.y : 0u,
                  nbody > 0 ? A.z : 0u, nbody > 0 ? A.w : 0u);
   B = make_uint4(nbody > 0 ? B.x : 0u, nbody > 0 ? B.y : 0u,
                  nbody > 0 ? B.z : 0u, nbody > 0 ? B.w : 0u);
   ready(A);  // both ways into the loop head then have A's words in hand
   for (int h = 0; 2 * h < nbody; h++) {
-    body(A, ListRef<L>{w + (size_t)min(2 * h + 2, blast) * BW}, A);
+    body(A, ListRef<L, NT>{w + (size_t)min(2 * h + 2, blast) * BW}, A);
     if (L <= 2 || 2 * h + 1 < nbody) {
-      body(B, ListRef<L>{w + (size_t)min(2 * h + 3, blast) * BW}, B);
+      body(B, ListRef<L, NT>{w + (size_t)min(2 * h + 3, blast) * BW}, B);
       // take the wait for the next A here, where only B's prefetch is newer
       // than it (at the loop head the wait counter is merged over the loop
       // entry and would also wait for B's prefetch)
@@ -638,18 +651,31 @@ __device__ __forceinline__ void list_walk(const uint32_t* w, int nnb, F body) {
 // halfword groups of 64 lanes): the same lane pattern, each load half as
 // wide.  load() returns the loaded words as they are (so the wait for them
 // stays where the walk places it); unpack16 spreads them to four entries.
-template <int L>
+template <int L, bool NT = false>
 struct ListRef16 {
   static constexpr bool W16 = true;
   const uint16_t* p;  // the lane's first halfword of the body
   __device__ __forceinline__ uint4 load() const {
     constexpr int G = IMDG_WAVE * IMDG_GROUP;  // halfwords per group
+    if (!NT) {  // NT: non-temporal loads (the list_nt policy)
+      if (L == 1) {
+        uint2 x = *reinterpret_cast<const uint2*>(p);
+        return make_uint4(x.x, x.y, 0u, 0u);
+      }
+      if (L == 2) {
+        uint32_t x = *reinterpret_cast<const uint32_t*>(p);
+        uint32_t y = *reinterpret_cast<const uint32_t*>(p + G);
+        return make_uint4(x, y, 0u, 0u);
+      }
+      return make_uint4(p[0], p[G], p[2 * G], p[3 * G]);
+    }
     if (L == 1) {
       uint2 x = nt_load2(p);
       return make_uint4(x.x, x.y, 0u, 0u);
     }
     if (L == 2) {
-      uint32_t x = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
+      uint32_t x =
+          __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
       uint32_t y =
           __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p + G));
       return make_uint4(x, y, 0u, 0u);
@@ -676,7 +702,7 @@ __device__ __forceinline__ const uint16_t* list_words16(const uint32_t* nbr,
 }
 
 // list_walk on the 16-bit format.
-template <int L, typename F>
+template <int L, bool NT = false, typename F>
 __device__ __forceinline__ void list_walk16(const uint16_t* w, int nnb,
                                             F body) {
   constexpr int E = IMDG_GROUP * L;
@@ -684,19 +710,19 @@ __device__ __forceinline__ void list_walk16(const uint16_t* w, int nnb,
   int nbody = (nnb + E - 1) / E;
   if (L <= 2) nbody = (nbody + 1) & ~1;
   const int blast = nbody > 0 ? nbody - 1 : 0;
-  uint4 A = ListRef16<L>{w}.load();
-  uint4 B = ListRef16<L>{w + (size_t)min(1, blast) * BW}.load();
+  uint4 A = ListRef16<L, NT>{w}.load();
+  uint4 B = ListRef16<L, NT>{w + (size_t)min(1, blast) * BW}.load();
   A = make_uint4(nbody > 0 ? A.x : 0u, nbody > 0 ? A.y : 0u,
                  nbody > 0 ? A.z : 0u, nbody > 0 ? A.w : 0u);
   B = make_uint4(nbody > 0 ? B.x : 0u, nbody > 0 ? B.y : 0u,
                  nbody > 0 ? B.z : 0u, nbody > 0 ? B.w : 0u);
   ready(A);
   for (int h = 0; 2 * h < nbody; h++) {
-    body(unpack16<L>(A), ListRef16<L>{w + (size_t)min(2 * h + 2, blast) * BW},
-         A);
+    body(unpack16<L>(A),
+         ListRef16<L, NT>{w + (size_t)min(2 * h + 2, blast) * BW}, A);
     if (L <= 2 || 2 * h + 1 < nbody) {
       body(unpack16<L>(B),
-           ListRef16<L>{w + (size_t)min(2 * h + 3, blast) * BW}, B);
+           ListRef16<L, NT>{w + (size_t)min(2 * h + 3, blast) * BW}, B);
       ready(A);
     }
   }
@@ -752,15 +778,16 @@ __device__ __forceinline__ bool chunk_bases(const int* __restrict__ cmeta,
   return wide == 0;
 }
 
-// One walk over a slot's list in the chunk's format.
-template <int L, typename F>
+// One walk over a slot's list in the chunk's format; NT: the list words'
+// loads are non-temporal (the list_nt policy, chosen per launch).
+template <int L, bool NT, typename F>
 __device__ __forceinline__ void walk_either(bool w16, const uint32_t* nbr,
                                             int i, int kmax, int s, int nnb,
                                             F body) {
   if (w16)
-    list_walk16<L>(list_words16<L>(nbr, i, kmax, s), nnb, body);
+    list_walk16<L, NT>(list_words16<L>(nbr, i, kmax, s), nnb, body);
   else
-    list_walk<L>(list_words<L>(nbr, i, kmax, s), nnb, body);
+    list_walk<L, NT>(list_words<L>(nbr, i, kmax, s), nnb, body);
 }
 
 // Single-species column of the density loop.
@@ -1186,8 +1213,9 @@ __device__ __forceinline__ void force_group_sgf(
 // Several species on one grid (FP32): the record pair S[2 (p len + k)] of
 // type pair p = it*nt + jt is (phi p0, phi A, phi B, phi A_lo) of pair
 // column p and (A, D) of rho columns p and jt*nt + it; gradients in units
-// of 1/(2 istep) and the virial's A_lo sum as in force_group_sgf.  rho'_j (column p) goes with 2F'_i,
-// rho'_i (column jt*nt + it) with 2F'_j (src/imd_forces_nbl.c:1156-1203).
+// of 1/(2 istep) and the virial's A_lo sum as in force_group_sgf.  rho'_j
+// (column p) goes with 2F'_i, rho'_i (column jt*nt + it) with 2F'_j
+// (src/imd_forces_nbl.c:1156-1203).
 template <bool TB, typename PF>
 __device__ __forceinline__ void force_group_sgn(
     uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
@@ -1291,8 +1319,10 @@ __device__ __forceinline__ void density_group_polyn(
 // 1079-1112): rho_i = sum_j rho_{type(i) type(j)}(r_ij^2) over the full
 // list, then F and 2F' from the embedding column of type(i).  Writes the
 // (u, 2F') records the force loop gathers.  L lanes per slot; CUB: the
-// tables are cubic segments (4-point or spline interpolation).
-template <typename R, bool NT1, bool LT, int L, bool CUB, bool EE>
+// tables are cubic segments (4-point or spline interpolation); NTL: the
+// FP32 one-grid walks load the list words non-temporally.
+template <typename R, bool NT1, bool LT, int L, bool CUB, bool EE,
+          bool NTL = false>
 __global__ void __launch_bounds__(IMDG_STEP_THREADS)
     k_density(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
   typedef typename V4<R>::T R4;
@@ -1348,7 +1378,7 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
           const float4* __restrict__ DQ =
               reinterpret_cast<const float4*>(K + (a.dpoly - klo));
           const TArg ta = targ(tr);
-          walk_either<L>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
+          walk_either<L, NTL>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
                          [&](uint4 e4, auto pf, uint4& slot) {
                            density_group_poly(e4, pf, slot, rec, off, olo,
                                               ui, DQ, ta, rho, r2min);
@@ -1368,7 +1398,7 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
               const float4* __restrict__ DQ =
                   reinterpret_cast<const float4*>(a.knots + a.dpolyn);
               const TArg ta = targ(tr);
-              walk_either<L>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
+              walk_either<L, NTL>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
                              [&](uint4 e4, auto pf, uint4& slot) {
                                density_group_polyn(e4, pf, slot, rec, off,
                                                    olo, ui, DQ, it * nt, len0,
@@ -1487,9 +1517,10 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
 // over the full list of (u, 2F') records; with INTEGRATE the leapfrog step,
 // the skin test and the step's reductions follow in the same thread
 // (calc_forces, move_atoms and check_nblist of one main_loop iteration in
-// one launch).  L lanes per slot.
+// one launch).  L lanes per slot; NTL: the FP32 one-grid walks load the
+// list words non-temporally.
 template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT,
-          int L, bool CUB, bool EE>
+          int L, bool CUB, bool EE, bool NTL = false>
 __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
     k_force(StepArgs<R> a, const TabDev<R>* __restrict__ tabs) {
   constexpr bool NT1 = MODE != FM_NTN && MODE != FM_SGN;
@@ -1560,7 +1591,7 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
               reinterpret_cast<const float4*>(a.knots + a.jointn);
           const TArg ta = targ(tp);
           if (a.type_bit)
-            walk_either<L>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
+            walk_either<L, NTL>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
                            [&](uint4 e4, auto pf, uint4& slot) {
                              force_group_sgn<true>(
                                  e4, pf, slot, rec, atype, off, olo, ui,
@@ -1568,7 +1599,7 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
                                  ep, vir, vlo, r2min);
                            });
           else
-            walk_either<L>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
+            walk_either<L, NTL>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
                            [&](uint4 e4, auto pf, uint4& slot) {
                              force_group_sgn<false>(
                                  e4, pf, slot, rec, atype, off, olo, ui,
@@ -1585,7 +1616,7 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
             const float2* __restrict__ Rr =
                 reinterpret_cast<const float2*>(K + (a.joint - klo) + 2 * plen);
             const TArg ta = targ(tp);
-            walk_either<L>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
+            walk_either<L, NTL>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
                            [&](uint4 e4, auto pf, uint4& slot) {
                              force_group_sgf(e4, pf, slot, rec, off, olo, ui,
                                              0.5f * dFi, Q, Rr, ta, fx, fy,
@@ -1872,62 +1903,100 @@ __global__ void __launch_bounds__(IMDG_BLOCK)
 // launch plans and launchers
 // ---------------------------------------------------------------------------
 
-template <typename R, bool NT1, bool LT, bool CUB, bool EE>
-const void* density_fn(int lpa) {
+template <typename R, bool NT1, bool LT, bool CUB, bool EE, bool NTL>
+const void* density_fn_l(int lpa) {
   if (lpa == 4)
-    return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 4, CUB, EE>);
+    return reinterpret_cast<const void*>(
+        &k_density<R, NT1, LT, 4, CUB, EE, NTL>);
   if (lpa == 2)
-    return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 2, CUB, EE>);
-  return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 1, CUB, EE>);
+    return reinterpret_cast<const void*>(
+        &k_density<R, NT1, LT, 2, CUB, EE, NTL>);
+  return reinterpret_cast<const void*>(&k_density<R, NT1, LT, 1, CUB, EE, NTL>);
+}
+
+// ntl: the list is loaded non-temporally (StepArgs::list_nt); only the
+// variants with FP32 one-grid walks have such an instance, the others load
+// the list words with the default policy whatever ntl says.
+template <typename R, bool NT1, bool LT, bool CUB, bool EE>
+const void* density_fn(int lpa, bool ntl) {
+  constexpr bool ONEGRID =
+      sizeof(R) == 4 && !CUB && !EE && (NT1 || !LT);
+  if constexpr (ONEGRID)
+    if (ntl) return density_fn_l<R, NT1, LT, CUB, EE, true>(lpa);
+  return density_fn_l<R, NT1, LT, CUB, EE, false>(lpa);
 }
 
 template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT,
-          bool CUB, bool EE>
-const void* force_fn3(int lpa) {
+          bool CUB, bool EE, bool NTL>
+const void* force_fn4(int lpa) {
   if (lpa == 4)
     return reinterpret_cast<const void*>(
-        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 4, CUB, EE>);
+        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 4, CUB, EE, NTL>);
   if (lpa == 2)
     return reinterpret_cast<const void*>(
-        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 2, CUB, EE>);
+        &k_force<R, MODE, INTEGRATE, HP, HE, LT, 2, CUB, EE, NTL>);
   return reinterpret_cast<const void*>(
-      &k_force<R, MODE, INTEGRATE, HP, HE, LT, 1, CUB, EE>);
+      &k_force<R, MODE, INTEGRATE, HP, HE, LT, 1, CUB, EE, NTL>);
+}
+
+// ntl as for density_fn; the non-temporal instances exist for the step
+// (INTEGRATE) only.
+template <typename R, int MODE, bool INTEGRATE, bool HP, bool HE, bool LT,
+          bool CUB, bool EE>
+const void* force_fn3(int lpa, bool ntl) {
+  constexpr bool ONEGRID = sizeof(R) == 4 && INTEGRATE && !CUB && !EE &&
+                           (MODE == FM_SG || MODE == FM_SGN);
+  if constexpr (ONEGRID)
+    if (ntl)
+      return force_fn4<R, MODE, INTEGRATE, HP, HE, LT, CUB, EE, true>(lpa);
+  return force_fn4<R, MODE, INTEGRATE, HP, HE, LT, CUB, EE, false>(lpa);
 }
 
 template <typename R, int MODE, bool INTEGRATE, bool LT, bool CUB, bool EE>
-const void* force_fn2(bool hp, bool he, int lpa) {
-  if (MODE == FM_SG || MODE == FM_SGN)
-    return force_fn3<R, MODE, INTEGRATE, true, true, LT, CUB, EE>(lpa);
-  if (hp && he)
-    return force_fn3<R, MODE, INTEGRATE, true, true, LT, CUB, EE>(lpa);
-  if (EE) return nullptr;  // EEAM needs the EAM tables
-  if (hp) return force_fn3<R, MODE, INTEGRATE, true, false, LT, CUB, EE>(lpa);
-  return force_fn3<R, MODE, INTEGRATE, false, true, LT, CUB, EE>(lpa);
+const void* force_fn2(bool hp, bool he, int lpa, bool ntl) {
+  if constexpr (MODE == FM_SG || MODE == FM_SGN) {
+    return force_fn3<R, MODE, INTEGRATE, true, true, LT, CUB, EE>(lpa, ntl);
+  } else {
+    if (hp && he)
+      return force_fn3<R, MODE, INTEGRATE, true, true, LT, CUB, EE>(lpa, ntl);
+    if (EE) return nullptr;  // EEAM needs the EAM tables
+    if (hp)
+      return force_fn3<R, MODE, INTEGRATE, true, false, LT, CUB, EE>(lpa,
+                                                                     ntl);
+    return force_fn3<R, MODE, INTEGRATE, false, true, LT, CUB, EE>(lpa, ntl);
+  }
 }
 
 template <typename R, int MODE>
-const void* force_fn1(bool integrate, bool lt, bool hp, bool he, int lpa) {
+const void* force_fn1(bool integrate, bool lt, bool hp, bool he, int lpa,
+                      bool ntl) {
   if (integrate)
-    return lt ? force_fn2<R, MODE, true, true, false, false>(hp, he, lpa)
-              : force_fn2<R, MODE, true, false, false, false>(hp, he, lpa);
-  return lt ? force_fn2<R, MODE, false, true, false, false>(hp, he, lpa)
-            : force_fn2<R, MODE, false, false, false, false>(hp, he, lpa);
+    return lt ? force_fn2<R, MODE, true, true, false, false>(hp, he, lpa, ntl)
+              : force_fn2<R, MODE, true, false, false, false>(hp, he, lpa,
+                                                              ntl);
+  return lt ? force_fn2<R, MODE, false, true, false, false>(hp, he, lpa, ntl)
+            : force_fn2<R, MODE, false, false, false, false>(hp, he, lpa,
+                                                             ntl);
 }
 
 // The table variants (cubic segments of the 4-point and spline
 // interpolations, EEAM) take the generic loops with the tables in HBM/L2.
 template <typename R, int MODE, bool CUB, bool EE>
-const void* force_fn1_var(bool integrate, bool hp, bool he, int lpa) {
-  return integrate ? force_fn2<R, MODE, true, false, CUB, EE>(hp, he, lpa)
-                   : force_fn2<R, MODE, false, false, CUB, EE>(hp, he, lpa);
+const void* force_fn1_var(bool integrate, bool hp, bool he, int lpa,
+                          bool ntl) {
+  return integrate
+             ? force_fn2<R, MODE, true, false, CUB, EE>(hp, he, lpa, ntl)
+             : force_fn2<R, MODE, false, false, CUB, EE>(hp, he, lpa, ntl);
 }
 
 template <typename R, int MODE>
-const void* force_fn_var(bool integrate, bool hp, bool he, int lpa, bool cub,
-                         bool ee) {
-  if (cub && ee) return force_fn1_var<R, MODE, true, true>(integrate, hp, he, lpa);
-  if (cub) return force_fn1_var<R, MODE, true, false>(integrate, hp, he, lpa);
-  return force_fn1_var<R, MODE, false, true>(integrate, hp, he, lpa);
+const void* force_fn_var(bool integrate, bool hp, bool he, int lpa, bool ntl,
+                         bool cub, bool ee) {
+  if (cub && ee)
+    return force_fn1_var<R, MODE, true, true>(integrate, hp, he, lpa, ntl);
+  if (cub)
+    return force_fn1_var<R, MODE, true, false>(integrate, hp, he, lpa, ntl);
+  return force_fn1_var<R, MODE, false, true>(integrate, hp, he, lpa, ntl);
 }
 
 // mode: 0 one species, 1 one species on a shared grid (needs hp and he),
@@ -1935,15 +2004,15 @@ const void* force_fn_var(bool integrate, bool hp, bool he, int lpa, bool cub,
 // tables, ee: EEAM (both: modes 0 and 2 only, lt false)
 template <typename R>
 const void* force_fn(int mode, bool integrate, bool lt, bool hp, bool he,
-                     int lpa, bool cub, bool ee) {
+                     int lpa, bool ntl, bool cub, bool ee) {
   if (cub || ee) {
     if (lt || mode == FM_SG) return nullptr;
     if (mode == FM_NT1)
-      return force_fn_var<R, FM_NT1>(integrate, hp, he, lpa, cub, ee);
-    return force_fn_var<R, FM_NTN>(integrate, hp, he, lpa, cub, ee);
+      return force_fn_var<R, FM_NT1>(integrate, hp, he, lpa, ntl, cub, ee);
+    return force_fn_var<R, FM_NTN>(integrate, hp, he, lpa, ntl, cub, ee);
   }
   if (mode == FM_SG)
-    return force_fn1<R, FM_SG>(integrate, lt, true, true, lpa);
+    return force_fn1<R, FM_SG>(integrate, lt, true, true, lpa, ntl);
   if (mode == FM_SGN) {
     // FP32 only, tables in HBM/L2
     if constexpr (sizeof(R) != 4) {
@@ -1952,34 +2021,36 @@ const void* force_fn(int mode, bool integrate, bool lt, bool hp, bool he,
       if (lt) return nullptr;
       return integrate
                  ? force_fn2<R, FM_SGN, true, false, false, false>(true, true,
-                                                                   lpa)
+                                                                   lpa, ntl)
                  : force_fn2<R, FM_SGN, false, false, false, false>(true, true,
-                                                                    lpa);
+                                                                    lpa, ntl);
     }
   }
-  if (mode == FM_NT1) return force_fn1<R, FM_NT1>(integrate, lt, hp, he, lpa);
-  return force_fn1<R, FM_NTN>(integrate, lt, hp, he, lpa);
+  if (mode == FM_NT1)
+    return force_fn1<R, FM_NT1>(integrate, lt, hp, he, lpa, ntl);
+  return force_fn1<R, FM_NTN>(integrate, lt, hp, he, lpa, ntl);
 }
 
 template <typename R, bool CUB, bool EE>
-const void* density_fn_var(bool nt1, int lpa) {
-  return nt1 ? density_fn<R, true, false, CUB, EE>(lpa)
-             : density_fn<R, false, false, CUB, EE>(lpa);
+const void* density_fn_var(bool nt1, int lpa, bool ntl) {
+  return nt1 ? density_fn<R, true, false, CUB, EE>(lpa, ntl)
+             : density_fn<R, false, false, CUB, EE>(lpa, ntl);
 }
 
 template <typename R>
-const void* density_fn(bool nt1, bool lt, int lpa, bool cub, bool ee) {
+const void* density_fn(bool nt1, bool lt, int lpa, bool ntl, bool cub,
+                       bool ee) {
   if (cub || ee) {
     if (lt) return nullptr;
-    if (cub && ee) return density_fn_var<R, true, true>(nt1, lpa);
-    if (cub) return density_fn_var<R, true, false>(nt1, lpa);
-    return density_fn_var<R, false, true>(nt1, lpa);
+    if (cub && ee) return density_fn_var<R, true, true>(nt1, lpa, ntl);
+    if (cub) return density_fn_var<R, true, false>(nt1, lpa, ntl);
+    return density_fn_var<R, false, true>(nt1, lpa, ntl);
   }
   if (nt1)
-    return lt ? density_fn<R, true, true, false, false>(lpa)
-              : density_fn<R, true, false, false, false>(lpa);
-  return lt ? density_fn<R, false, true, false, false>(lpa)
-            : density_fn<R, false, false, false, false>(lpa);
+    return lt ? density_fn<R, true, true, false, false>(lpa, ntl)
+              : density_fn<R, true, false, false, false>(lpa, ntl);
+  return lt ? density_fn<R, false, true, false, false>(lpa, ntl)
+            : density_fn<R, false, false, false, false>(lpa, ntl);
 }
 
 // Persistent grid of a kernel: resident blocks per CU (from the occupancy
@@ -2019,11 +2090,16 @@ static int persistent_grid(const void* fn, int threads, size_t lds, int ncu) {
   return g;
 }
 
+// The grid of a step loop fits both list policies' instances (and sets
+// the dynamic-LDS limit of both): list_nt may change with every rebuild.
 template <typename R>
 int step_grid_density(bool nt1, bool lt, size_t lds, int ncu, int lpa,
                       int threads, bool cub, bool ee) {
-  return persistent_grid(density_fn<R>(nt1, lt, lpa, cub, ee), threads, lds,
-                         ncu);
+  int g[2];
+  for (int q = 0; q < 2; q++)
+    g[q] = persistent_grid(density_fn<R>(nt1, lt, lpa, q != 0, cub, ee),
+                           threads, lds, ncu);
+  return g[0] < g[1] ? g[0] : g[1];
 }
 
 template <typename R>
@@ -2039,9 +2115,12 @@ int step_grid_force(int mode, bool integrate, bool lt, bool hp, bool he,
                     size_t lds, int ncu, int lpa, int threads, bool cub,
                     bool ee) {
   if (threads > force_threads<R>(mode)) return 0;
-  return persistent_grid(
-      force_fn<R>(mode, integrate, lt, hp, he, lpa, cub, ee), threads, lds,
-      ncu);
+  int g[2];
+  for (int q = 0; q < 2; q++)
+    g[q] = persistent_grid(
+        force_fn<R>(mode, integrate, lt, hp, he, lpa, q != 0, cub, ee),
+        threads, lds, ncu);
+  return g[0] < g[1] ? g[0] : g[1];
 }
 
 template <typename R>
@@ -2061,7 +2140,8 @@ hipError_t launch_density(const StepArgs<R>& a, const TabDev<R>* tabs,
   } pk = {a, tabs};
   void* kargs[] = {&pk.a, &pk.t};
   const void* fn =
-      density_fn<R>(nt1, lt, a.lpa_den, a.cubic != 0, a.eeam != 0);
+      density_fn<R>(nt1, lt, a.lpa_den, a.list_nt != 0, a.cubic != 0,
+                    a.eeam != 0);
   if (!fn) return hipErrorInvalidDeviceFunction;
   return hipLaunchKernel(fn, dim3(grid), dim3(a.thr_den), kargs, lds, st);
 }
@@ -2076,7 +2156,7 @@ hipError_t launch_force(const StepArgs<R>& a, const TabDev<R>* tabs,
   void* kargs[] = {&aa, &tt};
   const void* fn =
       force_fn<R>(mode, integrate, lt, a.has_pair != 0, a.has_eam != 0,
-                  a.lpa_frc, a.cubic != 0, a.eeam != 0);
+                  a.lpa_frc, a.list_nt != 0, a.cubic != 0, a.eeam != 0);
   if (!fn) return hipErrorInvalidDeviceFunction;
   return hipLaunchKernel(fn, dim3(grid), dim3(a.thr_frc), kargs, lds, st);
 }
@@ -2096,7 +2176,8 @@ hipError_t launch_finalize(const StepArgs<R>& a, bool forces, bool moved,
   // each); one replica: one wavefront (its lanes sum the block partials, and
   // no combine over wavefronts follows: the end of the step is a chain of
   // dependent operations, and a shorter chain ends it sooner)
-  int nblk = a.part_chunk ? (a.nrep + IMDG_STEP_WAVES - 1) / IMDG_STEP_WAVES : 1;
+  int nblk =
+      a.part_chunk ? (a.nrep + IMDG_STEP_WAVES - 1) / IMDG_STEP_WAVES : 1;
   int threads = a.part_chunk ? IMDG_STEP_THREADS : IMDG_WAVE;
   hipLaunchKernelGGL((k_finalize<R>), dim3(nblk), dim3(threads), 0, st, a,
                      forces ? 1 : 0, moved ? 1 : 0);
