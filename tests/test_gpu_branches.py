@@ -5,6 +5,11 @@
   (IMDG_TILE_LIMIT) sends blocks there; the pair sets and forces still
   match the oracle, and a fused run gives bit-identical results with and
   without the fallback (the list is the same).
+* The neighbour loops' kernel instances that load the list words
+  non-temporally (chosen for lists larger than half the last-level cache;
+  IMDG_LIST_NT=1 forces them): the cache policy changes no result, so
+  fused runs with and without them are bit-identical (one species, tables
+  in LDS; two species on a shared grid, tables in HBM/L2).
 * Pairs closer than a table's begin: IMD clamps r^2 to begin and sets
   is_short (src/potaccess.h:329-334, src/imd_forces_nbl.c:982, 1323); the
   forces at the clamped argument and the flag must match the oracle, and
@@ -69,6 +74,38 @@ def test_tile_fallback(precision, monkeypatch):
     assert n_fb > info["tiles_fallback"]
     for k in ("x", "p", "f", "epot"):
         assert np.array_equal(a_low[k], a_full[k]), k
+
+
+@pytest.mark.parametrize("nspecies", [1, 2])
+def test_list_nt_instances(nspecies, monkeypatch):
+    if nspecies == 1:
+        eam = eam_tables()
+        s = W.eam_crystal(n_cells=7, temp_k=900.0, seed=43, displace=0.04)
+    else:
+        eam = eam_tables(nspecies=2, rc=5.5)
+        s = W.fcc_system((7, 7, 5), 3.615, eam.masses, W.kT(700.0), 44,
+                         type_fractions=(0.7, 0.3), pbc=(1, 1, 0),
+                         vacuum_z=20.0, displace=0.04)
+    res = {}
+    for nt in ("1", "0"):
+        monkeypatch.setenv("IMDG_LIST_NT", nt)
+        ctx = G.eam_context(eam, [s], precision=G.MIXED,
+                            ensemble=G.NVT if nspecies == 1 else G.NVE,
+                            isq_tau_eta=1.0 / (100 * W.FS) ** 2)
+        monkeypatch.delenv("IMDG_LIST_NT")
+        with ctx:
+            ctx.calc_forces()
+            first = ctx.atoms()
+            ctx.run(60)
+            res[nt] = (first, ctx.atoms(), ctx.thermo(),
+                       ctx.info()["nbl_count"])
+    assert res["1"][3] == res["0"][3] > 1     # rebuilds inside the run
+    for k in ("f", "epot", "rho", "df"):
+        assert np.array_equal(res["1"][0][k], res["0"][0][k]), k
+    for k in ("x", "p", "f", "epot"):
+        assert np.array_equal(res["1"][1][k], res["0"][1][k]), k
+    for k in ("epot", "ekin", "virial"):
+        assert res["1"][2][k] == res["0"][2][k], k
 
 
 def _bring_close(s, i, dist):
