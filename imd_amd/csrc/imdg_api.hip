@@ -137,6 +137,8 @@ struct imdg_ctx {
   int knots_dpoly = -1;                // FP32 rho segment polynomials
   int knots_jointn = -1;               // FP32, several species, one grid:
   int knots_dpolyn = -1;               // force and rho segment records
+  int knots_frho = -1;                 // per-type rho segments (LDS), see
+  int knots_drho = -1;                 // StepArgs::frho / drho
   int* d_cmeta = nullptr;              // chunk metadata, 16-bit lists
   bool list16 = false;                 // lists in the 16-bit format
   int force_mode = 0;                  // 0 one species, 1 shared grid,
@@ -504,6 +506,7 @@ int upload_tables(imdg_ctx* c) {
   // (rho p0, A, B, 0) of rho column p; segment len-1 zero, as for one
   // species (the table argument is clamped instead of the cutoff tests)
   c->knots_jointn = c->knots_dpolyn = -1;
+  c->knots_frho = c->knots_drho = -1;
   const int nt = c->cfg.ntypes;
   bool grid_n = sizeof(R) == 4 && !c->cubic && !c->eeam && nt > 1 &&
                 hp.present && hr.present;
@@ -557,6 +560,48 @@ int upload_tables(imdg_ctx* c) {
         seg(hr, p, k, rc);
         push4(rc[0], rc[1], rc[2], 0.0);
       }
+    // rho column it*nt + jt the same for every it (the density at an atom
+    // due to a jt-atom, the usual EAM form): the rho parts of the records
+    // above once per type, small enough for LDS, with the same FP32 values
+    // (the loops that read them give bit-identical results)
+    bool by_jt = true;
+    const size_t rows = hr.table.size() / hr.ncols;
+    for (int it = 1; by_jt && it < nt; it++)
+      for (int jt = 0; by_jt && jt < nt; jt++)
+        for (size_t k = 0; by_jt && k < rows; k++)
+          by_jt = hr.table[k * hr.ncols + it * nt + jt] ==
+                  hr.table[k * hr.ncols + jt];
+    if (const char* e = getenv("IMDG_RHO_BY_TYPE"))
+      if (atoi(e) == 0) by_jt = false;
+    if (by_jt) {
+      c->knots_frho = (int)kn.size();
+      for (int t = 0; t < nt; t++)
+        for (int k = 0; k < len; k++) {
+          double rc[4];
+          seg(hr, t, k, rc);
+          R2 x;
+          x.x = (R)rc[1];
+          x.y = (R)rc[3];
+          kn.push_back(x);
+        }
+      c->knots_drho = (int)kn.size();
+      std::vector<R> f3;
+      for (int t = 0; t < nt; t++)
+        for (int k = 0; k < len; k++) {
+          double rc[4];
+          seg(hr, t, k, rc);
+          f3.push_back((R)rc[0]);
+          f3.push_back((R)rc[1]);
+          f3.push_back((R)rc[2]);
+        }
+      if (f3.size() & 1) f3.push_back((R)0);
+      for (size_t k = 0; k < f3.size(); k += 2) {
+        R2 x;
+        x.x = f3[k];
+        x.y = f3[k + 1];
+        kn.push_back(x);
+      }
+    }
   }
   // the FP32 step kernels evaluate the embedding function and the EEAM
   // energy term (once per atom) in FP64: their knots or cubic segments again
@@ -650,8 +695,13 @@ int plan_steps(imdg_ctx* c) {
     c->lds_den_lo = c->knots_dpoly;
     c->lds_den_n = 2 * c->tab[1].len[0];
   }
+  const int nty = c->cfg.ntypes;
+  if (c->knots_drho >= 0) {  // three floats per segment and type
+    c->lds_den_lo = c->knots_drho;
+    c->lds_den_n = (3 * nty * c->tab[1].len[0] + 1) / 2;
+  }
   c->grid_den = 0;
-  if (he && !no_lds && c->knots_dpolyn < 0)
+  if (he && !no_lds && (c->knots_dpolyn < 0 || c->knots_drho >= 0))
     c->grid_den = step_grid_density<R>(nt1, true, c->lds_den_n * sizeof(R2),
                                        c->ncu, c->lpa_den, td, cub, c->eeam);
   if (he && c->grid_den == 0) {
@@ -665,13 +715,16 @@ int plan_steps(imdg_ctx* c) {
     int len = c->tab[0].len[0];
     c->lds_frc_lo = c->knots_joint;
     c->lds_frc_n = sizeof(R) == 8 ? 2 * len + (len + 1) / 2 : 3 * len;
+  } else if (mode == 3) {
+    c->lds_frc_lo = c->knots_frho;
+    c->lds_frc_n = nty * c->tab[1].len[0];
   } else {
     c->lds_frc_lo = hp ? c->knots_pair : c->knots_rho;
     c->lds_frc_n = c->knots_embed - c->lds_frc_lo;
   }
   for (int integ = 0; integ < 2; integ++) {
     int g = 0;
-    if (!no_lds && mode != 3)
+    if (!no_lds && (mode != 3 || c->knots_frho >= 0))
       g = step_grid_force<R>(mode, integ, true, hp, he,
                              c->lds_frc_n * sizeof(R2), c->ncu, c->lpa_frc,
                              tf, cub, c->eeam);
@@ -738,6 +791,8 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.dpoly = c->knots_dpoly;
   a.jointn = c->knots_jointn;
   a.dpolyn = c->knots_dpolyn;
+  a.frho = c->knots_frho;
+  a.drho = c->knots_drho;
   a.cubic = c->cubic ? 1 : 0;
   a.eeam = c->eeam ? 1 : 0;
   a.eeP = (R*)c->d_eeP;
@@ -2087,6 +2142,8 @@ int imdg_get_info(imdg_ctx* c, imdg_info* info) {
   info->nbl_count = st.nbl_count;
   info->tiles_fallback = h.tiles_fallback;
   info->exact_pairs = h.exact_pairs;
+  info->tables_in_lds =
+      (c->lds_den_n > 0 ? 1 : 0) | (c->lds_frc_n > 0 ? 2 : 0);
   std::vector<int> hn(c->nslots);
   HIPCHK(c, hipMemcpy(hn.data(), c->d_nnb, sizeof(int) * c->nslots,
                       hipMemcpyDeviceToHost));

@@ -260,6 +260,12 @@ struct StepArgs {
                           // (one species), -1 if there are none
   int jointn, dpolyn;     // first knot of the FP32 several-species
                           // segment records (one grid), -1 if none
+  int frho, drho;         // first knot of the per-type rho segment tables
+                          // the several-species loops stage in LDS when
+                          // rho column it*nt + jt depends on jt only: (A,
+                          // D) per segment for the force loop, (p0, A, B)
+                          // as three packed floats for the density loop;
+                          // -1 if none
   int cubic;              // tables hold cubic segments, two knots each
                           // (4-point or spline interpolation)
   int eeam;               // EEAM: second density P = sum rho^2, energy E(P)

@@ -1216,14 +1216,19 @@ __device__ __forceinline__ void force_group_sgf(
 // of 1/(2 istep) and the virial's A_lo sum as in force_group_sgf.  rho'_j
 // (column p) goes with 2F'_i, rho'_i (column jt*nt + it) with 2F'_j
 // (src/imd_forces_nbl.c:1156-1203).
-template <bool TB, typename PF>
+//
+// LR: the rho parts come from the per-type table FR in LDS instead (rho
+// column it*nt + jt the same for every it): FR[t len + k] = (A, D) of the
+// density due to a t-atom, the same FP32 values as the records' second
+// half, so the results do not change; one gather per neighbour less.
+template <bool TB, bool LR, typename PF>
 __device__ __forceinline__ void force_group_sgn(
     uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
     const signed char* __restrict__ atype, const float4* off,
     const float4* olo, const float4& ui, float hFi,
-    const float4* __restrict__ S, int pbase, int len, const TArg& ta,
-    float& sx, float& sy, float& sz, float& ep, float& svir, float& svlo,
-    float& r2min) {
+    const float4* __restrict__ S, const float2* FR, int it, int pbase,
+    int len, const TArg& ta, float& sx, float& sy, float& sz, float& ep,
+    float& svir, float& svlo, float& r2min) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
   float4 uj[4], o[4], ol[4];
   int jt[4];
@@ -1242,20 +1247,26 @@ __device__ __forceinline__ void force_group_sgn(
 #pragma unroll
   for (int u = 0; u < 4; u++) ol[u] = olo[entry_code<PF::W16>(en[u])];
   float dx[4], dy[4], dz[4], r2[4], chi[4];
-  int kk[4];
+  int kk[4], sg[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     dx[u] = (uj[u].x - ui.x) + o[u].x;
     dy[u] = (uj[u].y - ui.y) + o[u].y;
     dz[u] = (uj[u].z - ui.z) + o[u].z;
     r2[u] = sep_r2(dx[u], dy[u], dz[u], ol[u]);
-    kk[u] = 2 * ((pbase + jt[u]) * len + tseg(r2[u], ta, chi[u]));
+    sg[u] = tseg(r2[u], ta, chi[u]);
+    kk[u] = 2 * ((pbase + jt[u]) * len + sg[u]);
   }
   float4 q[4], rr[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     q[u] = S[kk[u]];
-    rr[u] = S[kk[u] + 1];
+    if (LR) {
+      const float2 fj = FR[jt[u] * len + sg[u]], fi = FR[it * len + sg[u]];
+      rr[u] = make_float4(fj.x, fj.y, fi.x, fi.y);
+    } else {
+      rr[u] = S[kk[u] + 1];
+    }
   }
 #pragma unroll
   for (int u = 0; u < 4; u++) {
@@ -1307,6 +1318,48 @@ __device__ __forceinline__ void density_group_polyn(
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     rho += fmaf(chi[u], fmaf(chi[u], q[u].z, q[u].y), q[u].x);
+    r2min = fminf(r2min, r2[u]);
+  }
+}
+
+// Several species on one grid, rho column it*nt + jt the same for every
+// it: DR[3 (jt len + k) + 0..2] = (rho p0, A, B) of the density due to a
+// jt-atom, staged in LDS (the same FP32 values as the records of
+// density_group_polyn, so the results do not change).
+template <typename PF>
+__device__ __forceinline__ void density_group_polyl(
+    uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
+    const float4* off, const float4* olo, const float4& ui, const float* DR,
+    int len, const TArg& ta, float& rho, float& r2min) {
+  uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
+  float4 uj[4], o[4], ol[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+    uj[u] = rec_at(rec, (uint32_t)entry_slot<PF::W16>(en[u], off, o[u]));
+  __builtin_amdgcn_sched_barrier(0);
+  slot = pf.load();
+#pragma unroll
+  for (int u = 0; u < 4; u++) ol[u] = olo[entry_code<PF::W16>(en[u])];
+  float r2[4], chi[4];
+  int kk[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    float dx = (uj[u].x - ui.x) + o[u].x;
+    float dy = (uj[u].y - ui.y) + o[u].y;
+    float dz = (uj[u].z - ui.z) + o[u].z;
+    r2[u] = sep_r2(dx, dy, dz, ol[u]);
+    kk[u] = 3 * ((int)uj[u].w * len + tseg(r2[u], ta, chi[u]));
+  }
+  float q0[4], q1[4], q2[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    q0[u] = DR[kk[u]];
+    q1[u] = DR[kk[u] + 1];
+    q2[u] = DR[kk[u] + 2];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    rho += fmaf(chi[u], fmaf(chi[u], q2[u], q1[u]), q0[u]);
     r2min = fminf(r2min, r2[u]);
   }
 }
@@ -1392,7 +1445,23 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
                                col0, rho, pee, is_short, r2min);
                          });
           };
-          if constexpr (sizeof(R) == 4 && !NT1 && !CUB && !EE && !LT) {
+          if constexpr (sizeof(R) == 4 && !NT1 && !CUB && !EE && LT) {
+            if (a.drho >= 0) {
+              // several species on one grid, per-type rho segments in LDS
+              const float* DR =
+                  reinterpret_cast<const float*>(K + (a.drho - klo));
+              const TArg ta = targ(tr);
+              walk_either<L, NTL>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
+                             [&](uint4 e4, auto pf, uint4& slot) {
+                               density_group_polyl(e4, pf, slot, rec, off,
+                                                   olo, ui, DR, len0, ta, rho,
+                                                   r2min);
+                             });
+              is_short |= (r2min < b0);
+            } else {
+              generic();
+            }
+          } else if constexpr (sizeof(R) == 4 && !NT1 && !CUB && !EE) {
             if (a.dpolyn >= 0) {
               // several species on one grid: segment records in HBM/L2
               const float4* __restrict__ DQ =
@@ -1589,22 +1658,27 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
           R r2min = (R)3.0e38;
           const float4* __restrict__ S =
               reinterpret_cast<const float4*>(a.knots + a.jointn);
+          // LT: the per-type rho segments are in LDS (the launch stages
+          // nothing else for this loop)
+          const float2* FR =
+              LT ? reinterpret_cast<const float2*>(K + (a.frho - klo))
+                 : nullptr;
           const TArg ta = targ(tp);
           if (a.type_bit)
             walk_either<L, NTL>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
                            [&](uint4 e4, auto pf, uint4& slot) {
-                             force_group_sgn<true>(
+                             force_group_sgn<true, LT>(
                                  e4, pf, slot, rec, atype, off, olo, ui,
-                                 0.5f * dFi, S, it * nt, plen, ta, fx, fy, fz,
-                                 ep, vir, vlo, r2min);
+                                 0.5f * dFi, S, FR, it, it * nt, plen, ta,
+                                 fx, fy, fz, ep, vir, vlo, r2min);
                            });
           else
             walk_either<L, NTL>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
                            [&](uint4 e4, auto pf, uint4& slot) {
-                             force_group_sgn<false>(
+                             force_group_sgn<false, LT>(
                                  e4, pf, slot, rec, atype, off, olo, ui,
-                                 0.5f * dFi, S, it * nt, plen, ta, fx, fy, fz,
-                                 ep, vir, vlo, r2min);
+                                 0.5f * dFi, S, FR, it, it * nt, plen, ta,
+                                 fx, fy, fz, ep, vir, vlo, r2min);
                            });
           is_short |= (r2min < pb);
         } else if constexpr (MODE == FM_SG && !CUB && !EE) {
@@ -1919,8 +1993,7 @@ const void* density_fn_l(int lpa) {
 // the list words with the default policy whatever ntl says.
 template <typename R, bool NT1, bool LT, bool CUB, bool EE>
 const void* density_fn(int lpa, bool ntl) {
-  constexpr bool ONEGRID =
-      sizeof(R) == 4 && !CUB && !EE && (NT1 || !LT);
+  constexpr bool ONEGRID = sizeof(R) == 4 && !CUB && !EE;
   if constexpr (ONEGRID)
     if (ntl) return density_fn_l<R, NT1, LT, CUB, EE, true>(lpa);
   return density_fn_l<R, NT1, LT, CUB, EE, false>(lpa);
@@ -2014,11 +2087,16 @@ const void* force_fn(int mode, bool integrate, bool lt, bool hp, bool he,
   if (mode == FM_SG)
     return force_fn1<R, FM_SG>(integrate, lt, true, true, lpa, ntl);
   if (mode == FM_SGN) {
-    // FP32 only, tables in HBM/L2
+    // FP32 only; lt: the per-type rho segments in LDS
     if constexpr (sizeof(R) != 4) {
       return nullptr;
     } else {
-      if (lt) return nullptr;
+      if (lt)
+        return integrate
+                   ? force_fn2<R, FM_SGN, true, true, false, false>(
+                         true, true, lpa, ntl)
+                   : force_fn2<R, FM_SGN, false, true, false, false>(
+                         true, true, lpa, ntl);
       return integrate
                  ? force_fn2<R, FM_SGN, true, false, false, false>(true, true,
                                                                    lpa, ntl)

@@ -74,7 +74,8 @@ class Info(C.Structure):
                 ("nbl_count", C.c_long), ("steps_done", C.c_long),
                 ("is_short", C.c_int), ("lanes_per_atom", C.c_int),
                 ("list_bytes", C.c_double), ("mean_k", C.c_double),
-                ("tiles_fallback", C.c_int), ("exact_pairs", C.c_long)]
+                ("tiles_fallback", C.c_int), ("exact_pairs", C.c_long),
+                ("tables_in_lds", C.c_int)]
 
 
 class Tuning(C.Structure):

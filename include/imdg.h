@@ -139,6 +139,9 @@ typedef struct {
                            of the list radius, decided by IMD's FP64
                            expression r^2 < cellsz
                            (src/imd_forces_nbl.c:249-261) */
+  int    tables_in_lds; /* the step loops read their table segments from
+                           LDS: bit 0 the density loop, bit 1 the force
+                           loop (otherwise from HBM/L2) */
 } imdg_info;
 
 /* lifecycle */

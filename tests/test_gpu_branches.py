@@ -10,11 +10,19 @@
   IMDG_LIST_NT=1 forces them): the cache policy changes no result, so
   fused runs with and without them are bit-identical (one species, tables
   in LDS; two species on a shared grid, tables in HBM/L2).
+* Two species on one grid whose density columns depend on the
+  neighbour's type only (the usual EAM form): the loops read per-type
+  density segments from LDS, with the same FP32 values as the records in
+  HBM/L2 (IMDG_RHO_BY_TYPE=0), so runs with and without them are
+  bit-identical; density columns that depend on both types keep the
+  records, and match the oracle.
 * Pairs closer than a table's begin: IMD clamps r^2 to begin and sets
   is_short (src/potaccess.h:329-334, src/imd_forces_nbl.c:982, 1323); the
   forces at the clamped argument and the flag must match the oracle, and
   imdg_get_info reports the flag once.
 """
+import copy
+
 import numpy as np
 import pytest
 
@@ -106,6 +114,57 @@ def test_list_nt_instances(nspecies, monkeypatch):
         assert np.array_equal(res["1"][1][k], res["0"][1][k]), k
     for k in ("epot", "ekin", "virial"):
         assert res["1"][2][k] == res["0"][2][k], k
+
+
+def _alloy(seed):
+    eam = eam_tables(nspecies=2, rc=5.5)
+    s = W.fcc_system((7, 7, 5), 3.615, eam.masses, W.kT(700.0), seed,
+                     type_fractions=(0.7, 0.3), pbc=(1, 1, 0), vacuum_z=20.0,
+                     displace=0.04)
+    return eam, s
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+def test_rho_by_type_in_lds(lanes, monkeypatch):
+    eam, s = _alloy(45)
+    res = {}
+    for name, by_type in (("lds", "1"), ("records", "0")):
+        monkeypatch.setenv("IMDG_RHO_BY_TYPE", by_type)
+        ctx = G.eam_context(eam, [s], precision=G.MIXED)
+        monkeypatch.delenv("IMDG_RHO_BY_TYPE")
+        with ctx:
+            ctx.set_tuning(lanes, 512, lanes, 512)
+            ctx.calc_forces()
+            first = ctx.atoms()
+            th0 = ctx.thermo()
+            ctx.run(60)
+            res[name] = (first, ctx.atoms(), th0, ctx.info())
+    assert res["lds"][3]["tables_in_lds"] == 3
+    assert res["records"][3]["tables_in_lds"] == 0
+    assert res["lds"][3]["nbl_count"] == res["records"][3]["nbl_count"] > 1
+    for k in ("f", "epot", "rho", "df"):
+        assert np.array_equal(res["lds"][0][k], res["records"][0][k]), k
+    for k in ("x", "p", "f", "epot"):
+        assert np.array_equal(res["lds"][1][k], res["records"][1][k]), k
+    # the step sums combine per-block partials (the grids may differ)
+    for k in ("epot", "virial"):
+        assert abs(res["lds"][2][k] - res["records"][2][k]) <= \
+            1e-13 * abs(res["records"][2][k]), k
+
+
+def test_rho_by_both_types_keeps_records():
+    """A density column that depends on the type of the host atom too
+    (column 1*nt + 0 scaled): no per-type LDS table; the mixed path still
+    matches the oracle."""
+    eam, s = _alloy(46)
+    eam = copy.deepcopy(eam)      # the builder's tables are shared (cached)
+    eam.rho.values[2] = 1.1 * np.asarray(eam.rho.values[2])
+    orc = oracle_eam(s, eam, nbl_margin=0.4, timestep=W.FS)
+    orc.calc_forces()
+    with G.eam_context(eam, [s], precision=G.MIXED) as gpu:
+        gpu.calc_forces()
+        assert gpu.info()["tables_in_lds"] == 0
+        _static(gpu, orc, G.MIXED)
 
 
 def _bring_close(s, i, dist):
