@@ -139,6 +139,7 @@ struct imdg_ctx {
   int knots_dpolyn = -1;               // force and rho segment records
   int knots_frho = -1;                 // per-type rho segments (LDS), see
   int knots_drho = -1;                 // StepArgs::frho / drho
+  int knots_pairn = -1;                // StepArgs::pairn
   int* d_cmeta = nullptr;              // chunk metadata, 16-bit lists
   bool list16 = false;                 // lists in the 16-bit format
   int force_mode = 0;                  // 0 one species, 1 shared grid,
@@ -506,7 +507,7 @@ int upload_tables(imdg_ctx* c) {
   // (rho p0, A, B, 0) of rho column p; segment len-1 zero, as for one
   // species (the table argument is clamped instead of the cutoff tests)
   c->knots_jointn = c->knots_dpolyn = -1;
-  c->knots_frho = c->knots_drho = -1;
+  c->knots_frho = c->knots_drho = c->knots_pairn = -1;
   const int nt = c->cfg.ntypes;
   bool grid_n = sizeof(R) == 4 && !c->cubic && !c->eeam && nt > 1 &&
                 hp.present && hr.present;
@@ -574,6 +575,15 @@ int upload_tables(imdg_ctx* c) {
     if (const char* e = getenv("IMDG_RHO_BY_TYPE"))
       if (atoi(e) == 0) by_jt = false;
     if (by_jt) {
+      // the pair halves of the force records again, 16 B apart (the loop
+      // that takes rho from LDS reads only them)
+      c->knots_pairn = (int)kn.size();
+      for (int p = 0; p < nt * nt; p++)
+        for (int k = 0; k < len; k++) {
+          double ph[4];
+          seg(hp, p, k, ph);
+          push4(ph[0], ph[1], ph[2], ph[1] - (double)(R)ph[1]);
+        }
       c->knots_frho = (int)kn.size();
       for (int t = 0; t < nt; t++)
         for (int k = 0; k < len; k++) {
@@ -792,6 +802,7 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.jointn = c->knots_jointn;
   a.dpolyn = c->knots_dpolyn;
   a.frho = c->knots_frho;
+  a.pairn = c->knots_pairn;
   a.drho = c->knots_drho;
   a.cubic = c->cubic ? 1 : 0;
   a.eeam = c->eeam ? 1 : 0;

@@ -266,6 +266,8 @@ struct StepArgs {
                           // D) per segment for the force loop, (p0, A, B)
                           // as three packed floats for the density loop;
                           // -1 if none
+  int pairn;              // with frho: the pair halves of the jointn
+                          // records, 16 B apart (p len + k)
   int cubic;              // tables hold cubic segments, two knots each
                           // (4-point or spline interpolation)
   int eeam;               // EEAM: second density P = sum rho^2, energy E(P)

@@ -1220,7 +1220,8 @@ __device__ __forceinline__ void force_group_sgf(
 // LR: the rho parts come from the per-type table FR in LDS instead (rho
 // column it*nt + jt the same for every it): FR[t len + k] = (A, D) of the
 // density due to a t-atom, the same FP32 values as the records' second
-// half, so the results do not change; one gather per neighbour less.
+// half, so the results do not change; one gather per neighbour less.  S
+// then holds only the pair halves, S[p len + k].
 template <bool TB, bool LR, typename PF>
 __device__ __forceinline__ void force_group_sgn(
     uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
@@ -1255,7 +1256,7 @@ __device__ __forceinline__ void force_group_sgn(
     dz[u] = (uj[u].z - ui.z) + o[u].z;
     r2[u] = sep_r2(dx[u], dy[u], dz[u], ol[u]);
     sg[u] = tseg(r2[u], ta, chi[u]);
-    kk[u] = 2 * ((pbase + jt[u]) * len + sg[u]);
+    kk[u] = (LR ? 1 : 2) * ((pbase + jt[u]) * len + sg[u]);
   }
   float4 q[4], rr[4];
 #pragma unroll
@@ -1656,10 +1657,11 @@ __global__ void __launch_bounds__((ForceThreads<R, MODE>::value))
         if constexpr (MODE == FM_SGN && sizeof(R) == 4 && !CUB && !EE) {
           // several species on one grid: sums in units of 1/(2 istep)
           R r2min = (R)3.0e38;
-          const float4* __restrict__ S =
-              reinterpret_cast<const float4*>(a.knots + a.jointn);
           // LT: the per-type rho segments are in LDS (the launch stages
-          // nothing else for this loop)
+          // nothing else for this loop), the pair halves of the records
+          // in their own table
+          const float4* __restrict__ S = reinterpret_cast<const float4*>(
+              a.knots + (LT ? a.pairn : a.jointn));
           const float2* FR =
               LT ? reinterpret_cast<const float2*>(K + (a.frho - klo))
                  : nullptr;
