@@ -116,7 +116,7 @@ struct imdg_ctx {
   Tile* d_tiles = nullptr;
   uint32_t* d_nbr = nullptr;
   size_t nbr_bytes = 0;
-  int list_k = 0;  // entries of the longest list at the last checked build
+  int list_k = 0;  // longest list built so far, as last read by the host
   int* d_nnb = nullptr;
   void *d_reca = nullptr, *d_recb = nullptr;
   void *d_rho = nullptr, *d_epot = nullptr, *d_frc = nullptr;
@@ -788,8 +788,11 @@ StepArgs<R> step_args(imdg_ctx* c) {
   a.knots64 = (const double2*)c->d_knots64;
   a.cmeta = c->d_cmeta;
   a.list16 = c->list16 ? 1 : 0;
+  // list words read per step: slots x the longest list of the last build
+  // the host has seen (the capacity before any), at the format's entry size
+  const int klist = c->list_k > 0 ? std::min(c->list_k, c->kmax) : c->kmax;
   const size_t list_read =
-      (size_t)c->nslots * std::min(c->list_k, c->kmax) * (c->list16 ? 2 : 4);
+      (size_t)c->nslots * klist * (c->list16 ? 2 : 4);
   a.list_nt = c->list_nt < 0 ? (list_read > IMDG_LIST_NT_BYTES ? 1 : 0)
                              : c->list_nt;
   a.type_bit = (c->mixed && c->cfg.ntypes == 2 && c->tab[1].present &&
@@ -1040,6 +1043,7 @@ int read_ctl(imdg_ctx* c, Ctl* h) {
   HIPCHK(c, hipMemcpyAsync(h, c->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->list_k = h->kmax_seen;  // the list policy's size estimate
   return IMDG_OK;
 }
 
@@ -1053,7 +1057,6 @@ int fix_overflow(imdg_ctx* c, int* fixed) {
   Ctl h;
   int rc = read_ctl(c, &h);
   if (rc) return rc;
-  c->list_k = h.kmax_seen;
   if (!h.frozen) return IMDG_OK;
   int newk = std::max(c->kmax + IMDG_LIST_PAD, (int)ceil(h.kmax_seen * 1.2));
   newk = (newk + IMDG_LIST_PAD - 1) / IMDG_LIST_PAD * IMDG_LIST_PAD;

@@ -636,8 +636,9 @@ __device__ __forceinline__ void list_cell(
       float d1 = uj[u].y + w1;
       float d2 = uj[u].z + w2;
       float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-      in[u] = valid && r2 < L.lo;
-      band[u] = valid && !in[u] && r2 < L.hi;
+      const bool inside = r2 < L.lo, near = r2 < L.hi;
+      in[u] = valid & inside;
+      band[u] = (valid & near) ^ in[u];  // near but not inside
     }
     if (band[0] || band[1] || band[2] || band[3]) {
       atomicAdd(&a.ctl->exact_pairs,

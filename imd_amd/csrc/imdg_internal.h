@@ -67,10 +67,10 @@
 #define IMDG_META 32          // ints of chunk metadata: bases, wide flag
 #define IMDG_META_WIDE 28
 // Lists whose words read per step may exceed this (slots x the longest
-// list of the last build x 2 or 4 bytes) are loaded non-temporally by the
-// FP32 neighbour loops: half of MI355X's 256 MB last-level cache, which
-// keeps a smaller list (and the step's other data) from one step to the
-// next.
+// list built so far, or the capacity until the host has read it, x 2 or 4
+// bytes) are loaded non-temporally by the FP32 neighbour loops: half of
+// MI355X's 256 MB last-level cache, which keeps a smaller list (and the
+// step's other data) from one step to the next.
 #define IMDG_LIST_NT_BYTES (128ull << 20)
 #define IMDG_TILE_CAP 2048    // largest tile cap (records of one block's
                               // LDS tile, IMDG_TILE_LIMIT)
