@@ -476,28 +476,30 @@ int upload_tables(imdg_ctx* c) {
     }
   }
   // FP32, one species: the density loop's rho table as segment polynomials
-  // (p0, A, B, 0) (value p0 + chi (A + B chi), A = dv - d2v/2, B = d2v/2),
-  // segment len-1 zero, like the force loop's segment table
+  // (p0, A, B) (value p0 + chi (A + B chi), A = dv - d2v/2, B = d2v/2),
+  // three packed floats per segment, segment len-1 zero, like the force
+  // loop's segment table
   c->knots_dpoly = -1;
   if (sizeof(R) == 4 && !c->cubic && !c->eeam && c->cfg.ntypes == 1 &&
       hr.present) {
     if (kn.size() & 1) kn.push_back(kn.back());
     c->knots_dpoly = (int)kn.size();
     const int len = hr.len[0];
-    for (int k = 0; k < len; k++) {
-      R2 a, b;
-      a.x = a.y = b.x = b.y = (R)0;
-      if (k < len - 1) {
-        const double* t = hr.table.data();
-        const size_t nc = hr.ncols;
-        double p0 = t[k * nc], p1 = t[(k + 1) * nc], p2 = t[(k + 2) * nc];
-        double dv = p1 - p0, d2v = p2 - 2 * p1 + p0;
-        a.x = (R)p0;
-        a.y = (R)(dv - 0.5 * d2v);
-        b.x = (R)(0.5 * d2v);
-      }
-      kn.push_back(a);
-      kn.push_back(b);
+    std::vector<R> f3(3 * (size_t)len + 1, (R)0);
+    for (int k = 0; k < len - 1; k++) {
+      const double* t = hr.table.data();
+      const size_t nc = hr.ncols;
+      double p0 = t[k * nc], p1 = t[(k + 1) * nc], p2 = t[(k + 2) * nc];
+      double dv = p1 - p0, d2v = p2 - 2 * p1 + p0;
+      f3[3 * k] = (R)p0;
+      f3[3 * k + 1] = (R)(dv - 0.5 * d2v);
+      f3[3 * k + 2] = (R)(0.5 * d2v);
+    }
+    for (size_t k = 0; k + 1 < f3.size(); k += 2) {
+      R2 x;
+      x.x = f3[k];
+      x.y = f3[k + 1];
+      kn.push_back(x);
     }
   }
   // FP32, several species, every pair and rho column on one grid: per type
@@ -701,9 +703,9 @@ int plan_steps(imdg_ctx* c) {
   // HBM/L2: nt^2 columns)
   c->lds_den_lo = c->knots_rho;
   c->lds_den_n = he ? c->knots_embed - c->knots_rho : 0;
-  if (c->knots_dpoly >= 0) {
+  if (c->knots_dpoly >= 0) {  // three floats per segment
     c->lds_den_lo = c->knots_dpoly;
-    c->lds_den_n = 2 * c->tab[1].len[0];
+    c->lds_den_n = (3 * c->tab[1].len[0] + 1) / 2;
   }
   const int nty = c->cfg.ntypes;
   if (c->knots_drho >= 0) {  // three floats per segment and type

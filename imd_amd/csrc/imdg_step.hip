@@ -908,7 +908,8 @@ __device__ __forceinline__ void density_group(
 }
 
 // FP32 single-species variant of density_group on the rho segment
-// polynomials DQ[k] = (p0, A, B, 0): value p0 + chi (A + B chi); the table
+// polynomials DQ[3k .. 3k+2] = (p0, A, B) (three packed floats: 12 B of LDS
+// per segment instead of 16): value p0 + chi (A + B chi); the table
 // argument is clamped to [0, len - 1/2] (med3), so r^2 at or beyond end
 // reads the zero segment len - 1 and no cutoff test is needed.  r^2 keeps
 // the offset remainder (sep_r2): without it every rho shifts by a uniform
@@ -919,7 +920,7 @@ template <typename PF>
 __device__ __forceinline__ void density_group_poly(
     uint4 e4, PF pf, uint4& slot, const float4* __restrict__ rec,
     const float4* off, const float4* olo, const float4& ui,
-    const float4* __restrict__ DQ, const TArg& ta, float& rho,
+    const float* __restrict__ DQ, const TArg& ta, float& rho,
     float& r2min) {
   uint32_t en[4] = {e4.x, e4.y, e4.z, e4.w};
   float ux[4], uy[4], uz[4];
@@ -949,12 +950,16 @@ __device__ __forceinline__ void density_group_poly(
     r2[u] = sep_r2(dx, dy, dz, ol[u]);
     kk[u] = tseg(r2[u], ta, chi[u]);
   }
-  float4 q[4];
-#pragma unroll
-  for (int u = 0; u < 4; u++) q[u] = DQ[kk[u]];
+  float q0[4], q1[4], q2[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    rho += fmaf(chi[u], fmaf(chi[u], q[u].z, q[u].y), q[u].x);
+    q0[u] = DQ[3 * kk[u]];
+    q1[u] = DQ[3 * kk[u] + 1];
+    q2[u] = DQ[3 * kk[u] + 2];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    rho += fmaf(chi[u], fmaf(chi[u], q2[u], q1[u]), q0[u]);
     r2min = fminf(r2min, r2[u]);
   }
 }
@@ -1429,8 +1434,8 @@ __global__ void __launch_bounds__(IMDG_STEP_THREADS)
         R r2min = (R)3.0e38;
         const uint32_t* nw = list_words<L>(a.nbr, i, a.kmax, gs);
         if constexpr (sizeof(R) == 4 && NT1 && !CUB && !EE) {
-          const float4* __restrict__ DQ =
-              reinterpret_cast<const float4*>(K + (a.dpoly - klo));
+          const float* __restrict__ DQ =
+              reinterpret_cast<const float*>(K + (a.dpoly - klo));
           const TArg ta = targ(tr);
           walk_either<L, NTL>(w16, a.nbr, i, a.kmax, gs, a.nnb[i],
                          [&](uint4 e4, auto pf, uint4& slot) {
