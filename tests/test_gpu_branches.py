@@ -15,7 +15,8 @@
   density segments from LDS, with the same FP32 values as the records in
   HBM/L2 (IMDG_RHO_BY_TYPE=0), so runs with and without them are
   bit-identical; density columns that depend on both types keep the
-  records, and match the oracle.
+  records, and match the oracle; with three species only the force loop's
+  per-type segments fit in LDS.
 * Pairs closer than a table's begin: IMD clamps r^2 to begin and sets
   is_short (src/potaccess.h:329-334, src/imd_forces_nbl.c:982, 1323); the
   forces at the clamped argument and the flag must match the oracle, and
@@ -165,6 +166,39 @@ def test_rho_by_both_types_keeps_records():
         gpu.calc_forces()
         assert gpu.info()["tables_in_lds"] == 0
         _static(gpu, orc, G.MIXED)
+
+
+def test_three_species_split_placement(monkeypatch):
+    """Three species on one grid: the force loop's per-type density
+    segments (3 x 5,000 x 8 B) fit in LDS, the density loop's (3 x 5,000 x
+    12 B) do not and it reads the records; parity with the oracle, and
+    bit-identical to the records everywhere (IMDG_RHO_BY_TYPE=0)."""
+    third = W.JohnsonParams(r_e=2.60, f_e=0.95, phi_e=0.57, E_c=3.9,
+                            alpha=5.0, beta=5.9, gamma=8.2, mass=60.0,
+                            a0=3.6)
+    eam = W.johnson_eam((W.CU, W.NI, third), rc=5.5)
+    s = W.fcc_system((6, 6, 5), 3.6, eam.masses, W.kT(600.0), 47,
+                     type_fractions=(0.5, 0.3, 0.2), displace=0.04)
+    orc = oracle_eam(s, eam, nbl_margin=0.4, timestep=W.FS)
+    orc.calc_forces()
+    res = {}
+    for name, by_type in (("lds", "1"), ("records", "0")):
+        monkeypatch.setenv("IMDG_RHO_BY_TYPE", by_type)
+        ctx = G.eam_context(eam, [s], precision=G.MIXED)
+        monkeypatch.delenv("IMDG_RHO_BY_TYPE")
+        with ctx:
+            ctx.calc_forces()
+            if name == "lds":
+                _static(ctx, orc, G.MIXED)
+            first = ctx.atoms()
+            ctx.run(30)
+            res[name] = (first, ctx.atoms(), ctx.info())
+    assert res["lds"][2]["tables_in_lds"] == 2
+    assert res["records"][2]["tables_in_lds"] == 0
+    for k in ("f", "epot", "rho", "df"):
+        assert np.array_equal(res["lds"][0][k], res["records"][0][k]), k
+    for k in ("x", "p", "f", "epot"):
+        assert np.array_equal(res["lds"][1][k], res["records"][1][k]), k
 
 
 def _bring_close(s, i, dist):
